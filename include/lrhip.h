@@ -1,0 +1,188 @@
+/*
+ * lrhip.h — C ABI of the MI355X (gfx950) ray-intersection device for LuxRays.
+ *
+ * Plain pointers and sizes only.  Every call returns LRH_OK (0) or a negative
+ * LRH_E* code; lrh_last_error() gives a thread-local message for the last
+ * failing call on the calling thread.  Device pointers are ordinary HIP
+ * device addresses; `stream` is a hipStream_t (NULL = the device's own
+ * stream, created at default priority).
+ *
+ * Reference interfaces replaced (paths relative to the LuxCore v2.6 tree):
+ *   device/buffer/queue  HardwareDevice            include/luxrays/core/hardwaredevice.h:113-183
+ *   scene build          Accelerator::Init         include/luxrays/core/accelerator.h:39-59
+ *                        BVHAccel::Init            src/luxrays/accelerators/bvhaccel.cpp:72-168
+ *                        MBVHAccel::Init           src/luxrays/accelerators/mbvhaccel.cpp:58-216
+ *   upload               BVHKernel ctor            src/luxrays/accelerators/bvhaccelhw.cpp:40-237
+ *                        MBVHKernel ctor           src/luxrays/accelerators/mbvhaccelhw.cpp:41-306
+ *   edit                 MBVHAccel::Update + MBVHKernel::Update
+ *                                                  mbvhaccel.cpp:237-250, mbvhaccelhw.cpp:442-466
+ *   trace                HardwareIntersectionKernel::EnqueueTraceRayBuffer
+ *                                                  include/luxrays/core/hardwareintersectiondevice.h:59-72
+ *                        (kernel Accelerator_Intersect_RayBuffer, include/luxrays/accelerators/bvh.cl:228-260,
+ *                         mbvh.cl:351-383)
+ *   stats                IntersectionDevice ray counters include/luxrays/core/intersectiondevice.h:42-63
+ */
+#ifndef LRHIP_H
+#define LRHIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LRH_OK 0
+#define LRH_EINVAL (-1)     /* bad argument */
+#define LRH_EHIP (-2)       /* HIP runtime error */
+#define LRH_ENOMEM (-3)     /* allocation failed / over budget */
+#define LRH_ERANGE (-4)     /* size beyond a documented limit */
+#define LRH_ENODEV (-5)     /* no GPU / device not available */
+#define LRH_EDEVICE (-6)    /* a kernel reported an internal error (e.g. stack overflow) */
+
+/* Ray / RayHit records: include/luxrays/core/geometry/ray.h:35-103 */
+#define LRH_RAY_BYTES 48
+#define LRH_HIT_BYTES 20
+#define LRH_RAY_FLAGS_MASKED 1u
+#define LRH_NULL_INDEX 0xffffffffu
+
+/* Query types.  Closest hit is the reference's only query; any hit (the first
+ * accepted hit with mint <= t < maxt ends the ray) is the extension that SLG's
+ * shadow rays would use instead of closest hit + meshIndex test
+ * (include/slg/engines/pathoclbase/kernels/pathoclbase_kernels_micro.cl:490-507). */
+#define LRH_QUERY_CLOSEST 0u
+#define LRH_QUERY_ANY 1u
+
+/* Memory copy kinds */
+#define LRH_H2D 1
+#define LRH_D2H 2
+#define LRH_D2D 3
+
+typedef struct lrh_device lrh_device;
+typedef struct lrh_scene lrh_scene;
+typedef struct lrh_event lrh_event;
+
+/* One triangle mesh (TriangleMesh, include/luxrays/core/trianglemesh.h:79-134):
+ * verts = 3*nVerts floats, tris = 3*nTris vertex indices local to the mesh. */
+typedef struct {
+	const float *verts;
+	unsigned int nVerts;
+	const unsigned int *tris;
+	unsigned int nTris;
+} lrh_mesh;
+
+/* One DataSet object of a two-level scene (MBVHAccel root leaf,
+ * mbvhaccel.cpp:90-202).  kind 0: the base mesh itself (TriangleMesh);
+ * kind 1: InstanceTriangleMesh with local->world matrix `m` (row major) and,
+ * optionally, its inverse `mInv` (NULL: computed with the reference's
+ * Gauss-Jordan inverse, src/luxrays/core/geometry/matrix4x4.cpp:118-175);
+ * kind 2: MotionTriangleMesh with nKeys keys (times ascending, matrices
+ * global->local, row major) as MotionSystem takes them
+ * (src/luxrays/core/geometry/motionsystem.cpp:296-331). */
+typedef struct {
+	unsigned int mesh;
+	unsigned int kind;
+	const float *m;
+	const float *mInv;
+	unsigned int nKeys;
+	const float *keyTimes;
+	const float *keyMats;
+} lrh_object;
+
+/* Build options (defaults when the pointer is NULL or a field is 0).
+ * layout: 0 = 64-byte quantized 4-wide nodes (default), 1 = 128-byte fp32. */
+typedef struct {
+	unsigned int layout;
+	unsigned int maxLeafSize;  /* 1..8, default 4 */
+	unsigned int nThreads;     /* host build threads, default = hardware threads */
+	float travCost;            /* SAH traversal cost relative to one triangle test, default 1.0 */
+} lrh_build_opts;
+
+typedef struct {
+	unsigned long long nodeCount;
+	unsigned long long nodeBytes;
+	unsigned long long primCount;      /* triangle records (+ instance records) */
+	unsigned long long primBytes;
+	unsigned long long deviceBytes;    /* everything the scene holds on the device */
+	unsigned int maxDepth;
+	unsigned int layout;
+	double buildSeconds;
+	double uploadSeconds;
+} lrh_scene_info;
+
+/* Tuning of the trace kernel; all zero = built-in defaults.
+ * blockSize in {64,128,256}; ldsStack in {8,16,24,32} entries per lane;
+ * blocksPerCU 0 = occupancy-derived persistent grid. */
+typedef struct {
+	unsigned int blockSize;
+	unsigned int ldsStack;
+	unsigned int blocksPerCU;
+	unsigned int refillThreshold;  /* refill a wave when fewer than this many lanes are live; 0 = default */
+} lrh_trace_opts;
+
+/* One ray buffer of a batched launch (lrh_trace_batch). */
+typedef struct {
+	lrh_scene *scene;
+	const void *rays;     /* device Ray[count] */
+	void *hits;           /* device RayHit[count] */
+	unsigned int count;
+	unsigned int query;
+} lrh_batch_item;
+
+const char *lrh_last_error(void);
+const char *lrh_version(void);
+
+/* ---- device, buffers, queue (HardwareDevice) ---- */
+int lrh_device_count(int *count);
+int lrh_device_open(int ordinal, lrh_device **out);
+int lrh_device_close(lrh_device *dev);
+int lrh_device_info(lrh_device *dev, char *name, size_t nameLen, int *cuCount, size_t *totalMem);
+int lrh_alloc(lrh_device *dev, size_t bytes, void **dptr);
+int lrh_free(lrh_device *dev, void *dptr);
+int lrh_memcpy(lrh_device *dev, void *dst, const void *src, size_t bytes, int kind, void *stream);
+int lrh_memcpy_async(lrh_device *dev, void *dst, const void *src, size_t bytes, int kind, void *stream);
+int lrh_host_alloc(lrh_device *dev, size_t bytes, void **hptr);   /* pinned */
+int lrh_host_free(lrh_device *dev, void *hptr);
+int lrh_stream_create(lrh_device *dev, void **stream);
+int lrh_stream_destroy(lrh_device *dev, void *stream);
+int lrh_sync(lrh_device *dev, void *stream);
+size_t lrh_used_memory(lrh_device *dev);
+int lrh_event_create(lrh_device *dev, lrh_event **ev);
+int lrh_event_destroy(lrh_event *ev);
+int lrh_event_record(lrh_event *ev, void *stream);
+int lrh_event_sync(lrh_event *ev);
+int lrh_event_elapsed(lrh_event *start, lrh_event *end, float *ms);
+
+/* ---- scenes (Accelerator::Init + HardwareIntersectionKernel upload) ---- */
+int lrh_scene_create_bvh(lrh_device *dev, const lrh_mesh *meshes, unsigned int nMeshes,
+		const lrh_build_opts *opts, lrh_scene **out);
+int lrh_scene_create_mbvh(lrh_device *dev, const lrh_mesh *meshes, unsigned int nMeshes,
+		const lrh_object *objects, unsigned int nObjects, const lrh_build_opts *opts, lrh_scene **out);
+/* MBVHAccel::Update: new instance matrices / motion keys for the same objects
+ * (same count, kinds and meshes); root tree rebuilt, leaf trees kept. */
+int lrh_scene_update(lrh_scene *scene, const lrh_object *objects, unsigned int nObjects);
+int lrh_scene_destroy(lrh_scene *scene);
+int lrh_scene_get_info(lrh_scene *scene, lrh_scene_info *info);
+/* Copy of the device node / primitive arrays to host memory (for tests). */
+int lrh_scene_download(lrh_scene *scene, void *nodes, size_t nodeBytes, void *prims, size_t primBytes);
+
+/* ---- tracing (EnqueueTraceRayBuffer) ----
+ * Asynchronous on `stream`.  Masked rays (flags & 1) leave their hit record
+ * untouched; a miss writes t = maxt, b1 = b2 = 0 and both indices 0xffffffff.
+ * meshIndex is the DataSet add order (two-level: the object index);
+ * triangleIndex is local to that mesh. */
+int lrh_trace(lrh_scene *scene, const void *rays, void *hits, unsigned int count,
+		unsigned int query, void *stream);
+int lrh_trace_ex(lrh_scene *scene, const void *rays, void *hits, unsigned int count,
+		unsigned int query, const lrh_trace_opts *opts, void *stream);
+/* Many ray buffers over one or more resident scenes of the same device in one
+ * persistent launch; rays are handed out in chunks across all buffers. */
+int lrh_trace_batch(lrh_device *dev, const lrh_batch_item *items, unsigned int nItems,
+		const lrh_trace_opts *opts, void *stream);
+/* Data-parallel ray count traced so far (IntersectionDevice stats). */
+int lrh_device_stats(lrh_device *dev, unsigned long long *rays, double *seconds);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

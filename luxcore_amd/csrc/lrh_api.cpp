@@ -1,0 +1,886 @@
+/*
+ * C-ABI layer (include/lrhip.h) of the HIP intersection device: devices,
+ * buffers and queues (HardwareDevice), scene build + upload (BVHAccel /
+ * MBVHAccel Init plus the BVHKernel / MBVHKernel upload), and ray-buffer
+ * traces (HardwareIntersectionKernel::EnqueueTraceRayBuffer).
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/lrhip.h"
+#include "lrh_build.h"
+#include "lrh_kernels.h"
+#include "lrh_layout.h"
+
+using lrh::Box;
+using lrh::Error;
+
+static_assert(sizeof(lrh_node4q) == 64, "node4q must be 64 bytes");
+static_assert(sizeof(lrh_node4f) == 128, "node4f must be 128 bytes");
+static_assert(sizeof(lrh_tri) == 48, "tri record must be 48 bytes");
+static_assert(sizeof(lrh_ray_rec) == LRH_RAY_BYTES, "Ray must be 48 bytes");
+static_assert(sizeof(lrh_hit_rec) == LRH_HIT_BYTES, "RayHit must be 20 bytes");
+
+namespace {
+
+thread_local std::string g_lastError;
+
+int fail(int code, const std::string &msg) {
+	g_lastError = msg;
+	return code;
+}
+
+#define HIPCHK(expr)                                                                            \
+	do {                                                                                        \
+		hipError_t _e = (expr);                                                                 \
+		if (_e != hipSuccess)                                                                   \
+			throw Error(LRH_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e));           \
+	} while (0)
+
+template <class F>
+int guarded(F fn) {
+	try {
+		fn();
+		return LRH_OK;
+	} catch (const Error &e) {
+		return fail(e.code, e.what());
+	} catch (const std::bad_alloc &) {
+		return fail(LRH_ENOMEM, "host allocation failed");
+	} catch (const std::exception &e) {
+		return fail(LRH_EINVAL, e.what());
+	}
+}
+
+double now() {
+	return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct lrh_device {
+	int ordinal = 0;
+	hipStream_t stream = nullptr;
+	int cuCount = 0;
+	size_t totalMem = 0;
+	std::string name;
+	uint32_t *dError = nullptr;
+	// per-stream trace workspace: ray-ticket counter + spill stack, so that
+	// traces issued on different streams never share scratch memory
+	struct Workspace {
+		uint32_t *counter = nullptr;
+		uint2 *spill = nullptr;
+		size_t spillBytes = 0;
+	};
+	std::map<hipStream_t, Workspace> ws;
+	std::atomic<size_t> used{0};
+	std::mutex mu;
+	std::map<size_t, size_t> allocs;  // lrh_alloc'd pointers -> bytes
+	std::map<uint64_t, int> occ;      // occupancy cache
+	std::atomic<unsigned long long> rays{0};
+	double t0 = 0;
+
+	void activate() const { HIPCHK(hipSetDevice(ordinal)); }
+	void *devAlloc(size_t bytes) {
+		void *p = nullptr;
+		if (bytes == 0) return nullptr;
+		const hipError_t e = hipMalloc(&p, bytes);
+		if (e != hipSuccess) throw Error(LRH_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+		used += bytes;
+		return p;
+	}
+	void devFree(void *p, size_t bytes) {
+		if (!p) return;
+		(void)hipFree(p);
+		used -= bytes;
+	}
+};
+
+struct DevBuf {
+	lrh_device *dev = nullptr;
+	void *p = nullptr;
+	size_t bytes = 0;
+	void alloc(lrh_device *d, size_t b) {
+		release();
+		dev = d;
+		p = d->devAlloc(b);
+		bytes = b;
+	}
+	void upload(const void *src, size_t b, hipStream_t s) {
+		if (b) HIPCHK(hipMemcpyAsync(p, src, b, hipMemcpyHostToDevice, s));
+	}
+	void release() {
+		if (p && dev) dev->devFree(p, bytes);
+		p = nullptr;
+		bytes = 0;
+	}
+	~DevBuf() { release(); }
+};
+
+struct lrh_event {
+	hipEvent_t ev = nullptr;
+	lrh_device *dev = nullptr;
+};
+
+struct lrh_scene {
+	lrh_device *dev = nullptr;
+	bool twoLevel = false;
+	unsigned layout = 0;
+	uint32_t root = LRH_CHILD_EMPTY;
+	uint32_t maxDepth = 0;
+	DevBuf nodes, tris, insts, xforms, motions, segs, times;
+	lrh_scene_info info{};
+	// two-level state kept for lrh_scene_update
+	lrh::BuildParams params;
+	std::vector<unsigned> objMesh, objKind, objLeafRoot;
+	std::vector<Box> meshBox;
+	size_t rootReserve = 0;  // node slots reserved for the root tree
+	uint32_t leafDepth = 0;
+	// host-only scenes (created with dev == NULL) keep their arrays here so
+	// the builder can be inspected without a GPU; they cannot be traced
+	std::vector<uint8_t> hostNodes, hostPrims;
+};
+
+// ---------------------------------------------------------------------------
+// scene construction helpers
+// ---------------------------------------------------------------------------
+namespace {
+
+void checkMesh(const lrh_mesh &m, unsigned i) {
+	if (m.nTris && (!m.verts || !m.tris)) throw Error(LRH_EINVAL, "mesh " + std::to_string(i) + ": null arrays");
+	for (size_t k = 0; k < 3 * (size_t)m.nTris; ++k)
+		if (m.tris[k] >= m.nVerts)
+			throw Error(LRH_EINVAL, "mesh " + std::to_string(i) + ": vertex index out of range");
+}
+
+lrh::BuildParams toParams(const lrh_build_opts *o) {
+	lrh::BuildParams p;
+	if (o) {
+		if (o->layout > 1) throw Error(LRH_EINVAL, "layout must be 0 or 1");
+		p.layout = o->layout;
+		if (o->maxLeafSize) p.maxLeaf = o->maxLeafSize;
+		if (o->nThreads) p.nThreads = o->nThreads;
+		if (o->travCost > 0.f) p.travCost = o->travCost;
+	}
+	if (p.maxLeaf < 1 || p.maxLeaf > LRH_LEAF_MAX) throw Error(LRH_EINVAL, "maxLeafSize must be 1..8");
+	return p;
+}
+
+template <class F>
+void parallelRange(size_t n, unsigned nt, F fn) {
+	if (nt <= 1 || n < 100000) { fn(0, n); return; }
+	std::vector<std::thread> th;
+	const size_t chunk = (n + nt - 1) / nt;
+	for (unsigned t = 0; t < nt; ++t) {
+		const size_t b = t * chunk, e = std::min(n, b + chunk);
+		if (b >= e) break;
+		th.emplace_back([=, &fn]() { fn(b, e); });
+	}
+	for (auto &t : th) t.join();
+}
+
+struct HostTree {
+	std::vector<lrh_node4q> q;
+	std::vector<lrh_node4f> f;
+	size_t count(unsigned layout) const { return layout ? f.size() : q.size(); }
+	void append(const lrh::WideBVH &w, unsigned layout) {
+		if (layout) f.insert(f.end(), w.fnodes.begin(), w.fnodes.end());
+		else q.insert(q.end(), w.qnodes.begin(), w.qnodes.end());
+	}
+	const void *data(unsigned layout) const { return layout ? (const void *)f.data() : (const void *)q.data(); }
+	size_t bytes(unsigned layout) const { return layout ? f.size() * sizeof(lrh_node4f) : q.size() * sizeof(lrh_node4q); }
+	void reserve(size_t n, unsigned layout) {
+		if (layout) f.resize(f.size() + n); else q.resize(q.size() + n);
+	}
+};
+
+// build a mesh's triangle tree; appends nodes and triangle records
+uint32_t buildMeshTree(const lrh_mesh *meshes, const std::vector<unsigned> &meshList, bool localMeshIndex,
+		const lrh::BuildParams &p, HostTree &ht, std::vector<lrh_tri> &tris, uint32_t *maxDepth) {
+	// global primitive -> (mesh, tri)
+	std::vector<size_t> off(meshList.size() + 1, 0);
+	for (size_t i = 0; i < meshList.size(); ++i) off[i + 1] = off[i] + meshes[meshList[i]].nTris;
+	const size_t n = off.back();
+	if (n == 0) return LRH_CHILD_EMPTY;
+	if (tris.size() + n > LRH_MAX_LEAF_START) throw Error(LRH_ERANGE, "scene exceeds 268,435,455 triangle records");
+	std::vector<Box> boxes(n);
+	const unsigned nt = p.nThreads ? p.nThreads : lrh::hardwareThreads();
+	for (size_t i = 0; i < meshList.size(); ++i) {
+		const lrh_mesh &m = meshes[meshList[i]];
+		parallelRange(m.nTris, nt, [&](size_t b, size_t e) {
+			for (size_t k = b; k < e; ++k) {
+				const unsigned *t = m.tris + 3 * k;
+				boxes[off[i] + k] = lrh::triangleBox(m.verts + 3 * (size_t)t[0], m.verts + 3 * (size_t)t[1],
+						m.verts + 3 * (size_t)t[2]);
+			}
+		});
+	}
+	lrh::WideBVH w;
+	const uint32_t nodeBase = (uint32_t)ht.count(p.layout);
+	const uint32_t primBase = (uint32_t)tris.size();
+	lrh::buildWideBVH(boxes, p, nodeBase, primBase, &w);
+	boxes.clear();
+	boxes.shrink_to_fit();
+	ht.append(w, p.layout);
+	tris.resize(primBase + n);
+	parallelRange(n, nt, [&](size_t b, size_t e) {
+		for (size_t i = b; i < e; ++i) {
+			const size_t prim = w.order[i];
+			const size_t mi = std::upper_bound(off.begin(), off.end(), prim) - off.begin() - 1;
+			const lrh_mesh &m = meshes[meshList[mi]];
+			const size_t k = prim - off[mi];
+			const unsigned *t = m.tris + 3 * k;
+			const float *p0 = m.verts + 3 * (size_t)t[0], *p1 = m.verts + 3 * (size_t)t[1], *p2 = m.verts + 3 * (size_t)t[2];
+			lrh_tri &r = tris[primBase + i];
+			for (int c = 0; c < 3; ++c) {
+				r.p0[c] = p0[c];
+				r.e1[c] = p1[c] - p0[c];
+				r.e2[c] = p2[c] - p0[c];
+			}
+			r.triIndex = (uint32_t)k;
+			r.meshIndex = localMeshIndex ? 0u : meshList[mi];
+			r.pad = 0;
+		}
+	});
+	if (w.maxDepth > *maxDepth) *maxDepth = w.maxDepth;
+	return w.root;
+}
+
+Box meshBBox(const lrh_mesh &m) {  // TriangleMesh::GetBBox over all vertices
+	Box b = Box::Empty();
+	for (size_t i = 0; i < m.nVerts; ++i) b.growPoint(m.verts + 3 * i);
+	return b;
+}
+
+struct ObjectData {
+	std::vector<float> xf;                   // world->local, 16 per static instance
+	std::vector<lrh_motion> mot;
+	std::vector<lrh_motion_seg> segs;
+	std::vector<float> times;
+	std::vector<lrh_inst> inst;              // in object order (before root permutation)
+	std::vector<Box> boxes;                  // epsilon-expanded world boxes
+};
+
+void prepareObjects(lrh_scene *s, const lrh_object *objs, unsigned n, ObjectData &od) {
+	for (unsigned o = 0; o < n; ++o) {
+		const lrh_object &ob = objs[o];
+		if (ob.kind != s->objKind[o] || ob.mesh != s->objMesh[o])
+			throw Error(LRH_EINVAL, "object " + std::to_string(o) + ": kind/mesh differ from the scene");
+		lrh_inst in;
+		in.root = s->objLeafRoot[o];
+		in.kind = ob.kind;
+		in.index = 0;
+		in.object = o;
+		Box wb = s->meshBox[ob.mesh];
+		if (ob.kind == 1) {
+			if (!ob.m) throw Error(LRH_EINVAL, "instance without matrix");
+			lrh::M4 m, inv;
+			std::memcpy(m.m, ob.m, 64);
+			if (ob.mInv) std::memcpy(inv.m, ob.mInv, 64);
+			else if (!lrh::invert(m, &inv)) throw Error(LRH_EINVAL, "singular instance matrix");
+			in.index = (uint32_t)(od.xf.size() / 16);
+			od.xf.insert(od.xf.end(), &inv.m[0][0], &inv.m[0][0] + 16);
+			wb = lrh::transformBox(m, wb);  // InstanceTriangleMesh::GetBBox
+		} else if (ob.kind == 2) {
+			if (!ob.nKeys || !ob.keyTimes || !ob.keyMats) throw Error(LRH_EINVAL, "motion object without keys");
+			lrh::MotionSystemHost ms;
+			ms.init(ob.nKeys, ob.keyTimes, ob.keyMats);
+			lrh_motion md;
+			md.firstTime = (uint32_t)od.times.size();
+			md.nTimes = (uint32_t)ms.times.size();
+			md.firstSeg = (uint32_t)od.segs.size();
+			md.pad = 0;
+			od.times.insert(od.times.end(), ms.times.begin(), ms.times.end());
+			od.segs.insert(od.segs.end(), ms.segs.begin(), ms.segs.end());
+			in.index = (uint32_t)od.mot.size();
+			od.mot.push_back(md);
+			wb = ms.bound(wb);  // MotionTriangleMesh::GetBBox
+		} else if (ob.kind != 0)
+			throw Error(LRH_EINVAL, "object kind must be 0, 1 or 2");
+		const float e = lrh::machineEpsilon(wb);
+		for (int k = 0; k < 3; ++k) { wb.lo[k] -= e; wb.hi[k] += e; }
+		od.boxes.push_back(wb);
+		od.inst.push_back(in);
+	}
+}
+
+// root tree over the objects, written into the reserved node slots [0, rootReserve)
+void buildRoot(lrh_scene *s, ObjectData &od, HostTree &ht, std::vector<lrh_inst> &instOut, uint32_t *depth) {
+	lrh::BuildParams rp = s->params;
+	rp.maxLeaf = 1;
+	lrh::WideBVH w;
+	lrh::buildWideBVH(od.boxes, rp, 0, 0, &w);
+	if (w.nodeCount() > s->rootReserve) throw Error(LRH_EINVAL, "root tree larger than its reservation");
+	if (s->layout) std::copy(w.fnodes.begin(), w.fnodes.end(), ht.f.begin());
+	else std::copy(w.qnodes.begin(), w.qnodes.end(), ht.q.begin());
+	instOut.resize(od.inst.size());
+	for (size_t i = 0; i < w.order.size(); ++i) instOut[i] = od.inst[w.order[i]];
+	*depth = w.maxDepth;
+	s->root = w.root;
+}
+
+void keepHost(lrh_scene *s, const HostTree &ht, const void *tris, size_t triBytes, const void *insts,
+		size_t instBytes) {
+	const uint8_t *n = (const uint8_t *)ht.data(s->layout);
+	s->hostNodes.assign(n, n + ht.bytes(s->layout));
+	s->hostPrims.resize(triBytes + instBytes);
+	if (triBytes) std::memcpy(s->hostPrims.data(), tris, triBytes);
+	if (instBytes) std::memcpy(s->hostPrims.data() + triBytes, insts, instBytes);
+}
+
+void finishInfo(lrh_scene *s, size_t nNodes, size_t nPrims, size_t primBytes, double tb, double tu) {
+	s->info.nodeCount = nNodes;
+	s->info.nodeBytes = s->dev ? s->nodes.bytes : s->hostNodes.size();
+	s->info.primCount = nPrims;
+	if (!s->dev) primBytes = s->hostPrims.size();
+	s->info.primBytes = primBytes;
+	s->info.deviceBytes = s->nodes.bytes + s->tris.bytes + s->insts.bytes + s->xforms.bytes + s->motions.bytes +
+			s->segs.bytes + s->times.bytes;
+	s->info.maxDepth = s->maxDepth;
+	s->info.layout = s->layout;
+	s->info.buildSeconds = tb;
+	s->info.uploadSeconds = tu;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *lrh_last_error(void) { return g_lastError.c_str(); }
+const char *lrh_version(void) { return "lrhip 0.1 (gfx950)"; }
+
+int lrh_device_count(int *count) {
+	if (!count) return fail(LRH_EINVAL, "null count");
+	int n = 0;
+	const hipError_t e = hipGetDeviceCount(&n);
+	if (e != hipSuccess) {
+		*count = 0;
+		return fail(LRH_ENODEV, std::string("hipGetDeviceCount: ") + hipGetErrorString(e));
+	}
+	*count = n;
+	return LRH_OK;
+}
+
+int lrh_device_open(int ordinal, lrh_device **out) {
+	if (!out) return fail(LRH_EINVAL, "null out");
+	*out = nullptr;
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(LRH_ENODEV, "no HIP device");
+	if (ordinal < 0 || ordinal >= n) return fail(LRH_EINVAL, "device ordinal out of range");
+	std::unique_ptr<lrh_device> d(new lrh_device());
+	const int rc = guarded([&]() {
+		d->ordinal = ordinal;
+		d->activate();
+		hipDeviceProp_t prop;
+		HIPCHK(hipGetDeviceProperties(&prop, ordinal));
+		d->cuCount = prop.multiProcessorCount;
+		d->totalMem = prop.totalGlobalMem;
+		d->name = prop.gcnArchName;
+		HIPCHK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+		d->dError = (uint32_t *)d->devAlloc(sizeof(uint32_t));
+		HIPCHK(hipMemsetAsync(d->dError, 0, sizeof(uint32_t), d->stream));
+		HIPCHK(hipStreamSynchronize(d->stream));
+		d->t0 = now();
+	});
+	if (rc != LRH_OK) return rc;
+	*out = d.release();
+	return LRH_OK;
+}
+
+int lrh_device_close(lrh_device *dev) {
+	if (!dev) return fail(LRH_EINVAL, "null device");
+	(void)hipSetDevice(dev->ordinal);
+	if (dev->stream) (void)hipStreamSynchronize(dev->stream);
+	(void)hipDeviceSynchronize();
+	for (auto &w : dev->ws) {
+		dev->devFree(w.second.spill, w.second.spillBytes);
+		dev->devFree(w.second.counter, sizeof(uint32_t));
+	}
+	dev->devFree(dev->dError, sizeof(uint32_t));
+	for (auto &a : dev->allocs) (void)hipFree((void *)a.first);
+	if (dev->stream) (void)hipStreamDestroy(dev->stream);
+	delete dev;
+	return LRH_OK;
+}
+
+int lrh_device_info(lrh_device *dev, char *name, size_t nameLen, int *cuCount, size_t *totalMem) {
+	if (!dev) return fail(LRH_EINVAL, "null device");
+	if (name && nameLen) {
+		std::strncpy(name, dev->name.c_str(), nameLen - 1);
+		name[nameLen - 1] = 0;
+	}
+	if (cuCount) *cuCount = dev->cuCount;
+	if (totalMem) *totalMem = dev->totalMem;
+	return LRH_OK;
+}
+
+int lrh_alloc(lrh_device *dev, size_t bytes, void **dptr) {
+	if (!dev || !dptr) return fail(LRH_EINVAL, "null argument");
+	return guarded([&]() {
+		dev->activate();
+		*dptr = dev->devAlloc(bytes ? bytes : 1);
+		std::lock_guard<std::mutex> g(dev->mu);
+		dev->allocs[(size_t)*dptr] = bytes ? bytes : 1;
+	});
+}
+
+int lrh_free(lrh_device *dev, void *dptr) {
+	if (!dev) return fail(LRH_EINVAL, "null device");
+	if (!dptr) return LRH_OK;
+	return guarded([&]() {
+		size_t bytes = 0;
+		{
+			std::lock_guard<std::mutex> g(dev->mu);
+			auto it = dev->allocs.find((size_t)dptr);
+			if (it == dev->allocs.end()) throw Error(LRH_EINVAL, "pointer not allocated by lrh_alloc");
+			bytes = it->second;
+			dev->allocs.erase(it);
+		}
+		dev->activate();
+		dev->devFree(dptr, bytes);
+	});
+}
+
+static hipMemcpyKind toKind(int kind) {
+	switch (kind) {
+	case LRH_H2D: return hipMemcpyHostToDevice;
+	case LRH_D2H: return hipMemcpyDeviceToHost;
+	case LRH_D2D: return hipMemcpyDeviceToDevice;
+	default: throw Error(LRH_EINVAL, "bad copy kind");
+	}
+}
+
+int lrh_memcpy(lrh_device *dev, void *dst, const void *src, size_t bytes, int kind, void *stream) {
+	if (!dev || (bytes && (!dst || !src))) return fail(LRH_EINVAL, "null argument");
+	return guarded([&]() {
+		dev->activate();
+		hipStream_t s = stream ? (hipStream_t)stream : dev->stream;
+		if (bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, toKind(kind), s));
+		HIPCHK(hipStreamSynchronize(s));
+	});
+}
+
+int lrh_memcpy_async(lrh_device *dev, void *dst, const void *src, size_t bytes, int kind, void *stream) {
+	if (!dev || (bytes && (!dst || !src))) return fail(LRH_EINVAL, "null argument");
+	return guarded([&]() {
+		dev->activate();
+		if (bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, toKind(kind), stream ? (hipStream_t)stream : dev->stream));
+	});
+}
+
+int lrh_host_alloc(lrh_device *dev, size_t bytes, void **hptr) {
+	if (!dev || !hptr) return fail(LRH_EINVAL, "null argument");
+	return guarded([&]() {
+		dev->activate();
+		HIPCHK(hipHostMalloc(hptr, bytes ? bytes : 1, hipHostMallocDefault));
+	});
+}
+
+int lrh_host_free(lrh_device *dev, void *hptr) {
+	if (!dev) return fail(LRH_EINVAL, "null device");
+	return guarded([&]() {
+		if (hptr) HIPCHK(hipHostFree(hptr));
+	});
+}
+
+int lrh_stream_create(lrh_device *dev, void **stream) {
+	if (!dev || !stream) return fail(LRH_EINVAL, "null argument");
+	return guarded([&]() {
+		dev->activate();
+		hipStream_t s;
+		HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+		*stream = (void *)s;
+	});
+}
+
+int lrh_stream_destroy(lrh_device *dev, void *stream) {
+	if (!dev) return fail(LRH_EINVAL, "null device");
+	return guarded([&]() {
+		if (stream) HIPCHK(hipStreamDestroy((hipStream_t)stream));
+	});
+}
+
+int lrh_sync(lrh_device *dev, void *stream) {
+	if (!dev) return fail(LRH_EINVAL, "null device");
+	return guarded([&]() {
+		dev->activate();
+		HIPCHK(hipStreamSynchronize(stream ? (hipStream_t)stream : dev->stream));
+		uint32_t err = 0;
+		HIPCHK(hipMemcpy(&err, dev->dError, sizeof(err), hipMemcpyDeviceToHost));
+		if (err) {
+			const uint32_t zero = 0;
+			HIPCHK(hipMemcpy(dev->dError, &zero, sizeof(zero), hipMemcpyHostToDevice));
+			throw Error(LRH_EDEVICE, (err & 1u) ? "traversal stack overflow" : "traversal step guard hit");
+		}
+	});
+}
+
+size_t lrh_used_memory(lrh_device *dev) { return dev ? dev->used.load() : 0; }
+
+int lrh_event_create(lrh_device *dev, lrh_event **ev) {
+	if (!dev || !ev) return fail(LRH_EINVAL, "null argument");
+	return guarded([&]() {
+		dev->activate();
+		std::unique_ptr<lrh_event> e(new lrh_event());
+		e->dev = dev;
+		HIPCHK(hipEventCreate(&e->ev));
+		*ev = e.release();
+	});
+}
+
+int lrh_event_destroy(lrh_event *ev) {
+	if (!ev) return LRH_OK;
+	(void)hipEventDestroy(ev->ev);
+	delete ev;
+	return LRH_OK;
+}
+
+int lrh_event_record(lrh_event *ev, void *stream) {
+	if (!ev) return fail(LRH_EINVAL, "null event");
+	return guarded([&]() { HIPCHK(hipEventRecord(ev->ev, stream ? (hipStream_t)stream : ev->dev->stream)); });
+}
+
+int lrh_event_sync(lrh_event *ev) {
+	if (!ev) return fail(LRH_EINVAL, "null event");
+	return guarded([&]() { HIPCHK(hipEventSynchronize(ev->ev)); });
+}
+
+int lrh_event_elapsed(lrh_event *a, lrh_event *b, float *ms) {
+	if (!a || !b || !ms) return fail(LRH_EINVAL, "null argument");
+	return guarded([&]() { HIPCHK(hipEventElapsedTime(ms, a->ev, b->ev)); });
+}
+
+int lrh_scene_create_bvh(lrh_device *dev, const lrh_mesh *meshes, unsigned int nMeshes,
+		const lrh_build_opts *opts, lrh_scene **out) {
+	if (!out || (nMeshes && !meshes)) return fail(LRH_EINVAL, "null argument");
+	*out = nullptr;
+	std::unique_ptr<lrh_scene> s(new lrh_scene());
+	const int rc = guarded([&]() {
+		for (unsigned i = 0; i < nMeshes; ++i) checkMesh(meshes[i], i);
+		s->dev = dev;
+		s->params = toParams(opts);
+		s->layout = s->params.layout;
+		s->twoLevel = false;
+		const double t0 = now();
+		HostTree ht;
+		std::vector<lrh_tri> tris;
+		std::vector<unsigned> all(nMeshes);
+		for (unsigned i = 0; i < nMeshes; ++i) all[i] = i;
+		s->root = buildMeshTree(meshes, all, false, s->params, ht, tris, &s->maxDepth);
+		const double t1 = now();
+		if (!dev) {
+			keepHost(s.get(), ht, tris.data(), tris.size() * sizeof(lrh_tri), nullptr, 0);
+			finishInfo(s.get(), ht.count(s->layout), tris.size(), tris.size() * sizeof(lrh_tri), t1 - t0, 0.0);
+			return;
+		}
+		dev->activate();
+		s->nodes.alloc(dev, std::max<size_t>(ht.bytes(s->layout), 64));
+		s->nodes.upload(ht.data(s->layout), ht.bytes(s->layout), dev->stream);
+		s->tris.alloc(dev, std::max<size_t>(tris.size() * sizeof(lrh_tri), 48));
+		s->tris.upload(tris.data(), tris.size() * sizeof(lrh_tri), dev->stream);
+		HIPCHK(hipStreamSynchronize(dev->stream));
+		finishInfo(s.get(), ht.count(s->layout), tris.size(), tris.size() * sizeof(lrh_tri), t1 - t0, now() - t1);
+	});
+	if (rc != LRH_OK) return rc;
+	*out = s.release();
+	return LRH_OK;
+}
+
+int lrh_scene_create_mbvh(lrh_device *dev, const lrh_mesh *meshes, unsigned int nMeshes,
+		const lrh_object *objects, unsigned int nObjects, const lrh_build_opts *opts, lrh_scene **out) {
+	if (!out || (nMeshes && !meshes) || (nObjects && !objects)) return fail(LRH_EINVAL, "null argument");
+	*out = nullptr;
+	std::unique_ptr<lrh_scene> s(new lrh_scene());
+	const int rc = guarded([&]() {
+		for (unsigned i = 0; i < nMeshes; ++i) checkMesh(meshes[i], i);
+		s->dev = dev;
+		s->params = toParams(opts);
+		s->layout = s->params.layout;
+		s->twoLevel = true;
+		const double t0 = now();
+		HostTree ht;
+		std::vector<lrh_tri> tris;
+		s->rootReserve = std::max<size_t>(nObjects, 1);
+		ht.reserve(s->rootReserve, s->layout);
+		// one leaf tree per referenced base mesh (mbvhaccel.cpp:90-180)
+		std::map<unsigned, uint32_t> leafRoot;
+		s->objMesh.resize(nObjects);
+		s->objKind.resize(nObjects);
+		s->objLeafRoot.resize(nObjects);
+		s->meshBox.resize(nMeshes);
+		for (unsigned i = 0; i < nMeshes; ++i) s->meshBox[i] = meshBBox(meshes[i]);
+		uint32_t depth = 0;
+		for (unsigned o = 0; o < nObjects; ++o) {
+			const unsigned m = objects[o].mesh;
+			if (m >= nMeshes) throw Error(LRH_EINVAL, "object mesh index out of range");
+			s->objMesh[o] = m;
+			s->objKind[o] = objects[o].kind;
+			auto it = leafRoot.find(m);
+			if (it == leafRoot.end() || objects[o].kind == 0) {
+				const std::vector<unsigned> one(1, m);
+				const uint32_t r = buildMeshTree(meshes, one, true, s->params, ht, tris, &depth);
+				leafRoot[m] = r;
+				s->objLeafRoot[o] = r;
+			} else
+				s->objLeafRoot[o] = it->second;
+		}
+		ObjectData od;
+		std::vector<lrh_inst> instOut;
+		uint32_t rootDepth = 0;
+		if (nObjects) {
+			prepareObjects(s.get(), objects, nObjects, od);
+			buildRoot(s.get(), od, ht, instOut, &rootDepth);
+		} else
+			s->root = LRH_CHILD_EMPTY;
+		s->leafDepth = depth;
+		s->maxDepth = rootDepth + depth + 1;
+		const double t1 = now();
+		if (!dev) {
+			keepHost(s.get(), ht, tris.data(), tris.size() * sizeof(lrh_tri), instOut.data(),
+					instOut.size() * sizeof(lrh_inst));
+			finishInfo(s.get(), ht.count(s->layout), tris.size() + instOut.size(),
+					tris.size() * sizeof(lrh_tri) + instOut.size() * sizeof(lrh_inst), t1 - t0, 0.0);
+			return;
+		}
+		dev->activate();
+		s->nodes.alloc(dev, std::max<size_t>(ht.bytes(s->layout), 64));
+		s->nodes.upload(ht.data(s->layout), ht.bytes(s->layout), dev->stream);
+		s->tris.alloc(dev, std::max<size_t>(tris.size() * sizeof(lrh_tri), 48));
+		s->tris.upload(tris.data(), tris.size() * sizeof(lrh_tri), dev->stream);
+		s->insts.alloc(dev, std::max<size_t>(instOut.size() * sizeof(lrh_inst), 16));
+		s->insts.upload(instOut.data(), instOut.size() * sizeof(lrh_inst), dev->stream);
+		s->xforms.alloc(dev, std::max<size_t>(od.xf.size() * 4, 64));
+		s->xforms.upload(od.xf.data(), od.xf.size() * 4, dev->stream);
+		s->motions.alloc(dev, std::max<size_t>(od.mot.size() * sizeof(lrh_motion), 16));
+		s->motions.upload(od.mot.data(), od.mot.size() * sizeof(lrh_motion), dev->stream);
+		s->segs.alloc(dev, std::max<size_t>(od.segs.size() * sizeof(lrh_motion_seg), sizeof(lrh_motion_seg)));
+		s->segs.upload(od.segs.data(), od.segs.size() * sizeof(lrh_motion_seg), dev->stream);
+		s->times.alloc(dev, std::max<size_t>(od.times.size() * 4, 16));
+		s->times.upload(od.times.data(), od.times.size() * 4, dev->stream);
+		HIPCHK(hipStreamSynchronize(dev->stream));
+		finishInfo(s.get(), ht.count(s->layout), tris.size() + instOut.size(),
+				tris.size() * sizeof(lrh_tri) + instOut.size() * sizeof(lrh_inst), t1 - t0, now() - t1);
+	});
+	if (rc != LRH_OK) return rc;
+	*out = s.release();
+	return LRH_OK;
+}
+
+int lrh_scene_update(lrh_scene *s, const lrh_object *objects, unsigned int nObjects) {
+	if (!s || (nObjects && !objects)) return fail(LRH_EINVAL, "null argument");
+	if (!s->twoLevel) return fail(LRH_EINVAL, "only two-level scenes support update (Accelerator::DoesSupportUpdate)");
+	if (nObjects != s->objMesh.size()) return fail(LRH_EINVAL, "object count differs from the scene");
+	if (!s->dev) return fail(LRH_EINVAL, "host-only scene");
+	return guarded([&]() {
+		if (!nObjects) return;
+		ObjectData od;
+		prepareObjects(s, objects, nObjects, od);
+		HostTree ht;
+		ht.reserve(s->rootReserve, s->layout);
+		std::vector<lrh_inst> instOut;
+		uint32_t rootDepth = 0;
+		buildRoot(s, od, ht, instOut, &rootDepth);
+		lrh_device *dev = s->dev;
+		dev->activate();
+		HIPCHK(hipStreamSynchronize(dev->stream));
+		// root nodes live in the reserved prefix of the node array
+		const size_t nb = s->rootReserve * (s->layout ? sizeof(lrh_node4f) : sizeof(lrh_node4q));
+		HIPCHK(hipMemcpyAsync(s->nodes.p, ht.data(s->layout), nb, hipMemcpyHostToDevice, dev->stream));
+		HIPCHK(hipMemcpyAsync(s->insts.p, instOut.data(), instOut.size() * sizeof(lrh_inst), hipMemcpyHostToDevice,
+				dev->stream));
+		if (od.xf.size() * 4 > s->xforms.bytes) s->xforms.alloc(dev, od.xf.size() * 4);
+		s->xforms.upload(od.xf.data(), od.xf.size() * 4, dev->stream);
+		if (od.mot.size() * sizeof(lrh_motion) > s->motions.bytes) s->motions.alloc(dev, od.mot.size() * sizeof(lrh_motion));
+		s->motions.upload(od.mot.data(), od.mot.size() * sizeof(lrh_motion), dev->stream);
+		if (od.segs.size() * sizeof(lrh_motion_seg) > s->segs.bytes)
+			s->segs.alloc(dev, od.segs.size() * sizeof(lrh_motion_seg));
+		s->segs.upload(od.segs.data(), od.segs.size() * sizeof(lrh_motion_seg), dev->stream);
+		if (od.times.size() * 4 > s->times.bytes) s->times.alloc(dev, od.times.size() * 4);
+		s->times.upload(od.times.data(), od.times.size() * 4, dev->stream);
+		HIPCHK(hipStreamSynchronize(dev->stream));
+		s->maxDepth = rootDepth + s->leafDepth + 1;
+		s->info.maxDepth = s->maxDepth;
+	});
+}
+
+int lrh_scene_destroy(lrh_scene *s) {
+	if (!s) return LRH_OK;
+	if (s->dev) {
+		(void)hipSetDevice(s->dev->ordinal);
+		(void)hipStreamSynchronize(s->dev->stream);
+	}
+	delete s;
+	return LRH_OK;
+}
+
+int lrh_scene_get_info(lrh_scene *s, lrh_scene_info *info) {
+	if (!s || !info) return fail(LRH_EINVAL, "null argument");
+	*info = s->info;
+	return LRH_OK;
+}
+
+int lrh_scene_download(lrh_scene *s, void *nodes, size_t nodeBytes, void *prims, size_t primBytes) {
+	if (!s) return fail(LRH_EINVAL, "null scene");
+	return guarded([&]() {
+		if (!s->dev) {
+			if (nodes) std::memcpy(nodes, s->hostNodes.data(), std::min(nodeBytes, s->hostNodes.size()));
+			if (prims) std::memcpy(prims, s->hostPrims.data(), std::min(primBytes, s->hostPrims.size()));
+			return;
+		}
+		s->dev->activate();
+		HIPCHK(hipStreamSynchronize(s->dev->stream));
+		if (nodes) HIPCHK(hipMemcpy(nodes, s->nodes.p, std::min(nodeBytes, s->nodes.bytes), hipMemcpyDeviceToHost));
+		if (prims) HIPCHK(hipMemcpy(prims, s->tris.p, std::min(primBytes, s->tris.bytes), hipMemcpyDeviceToHost));
+	});
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// tracing
+// ---------------------------------------------------------------------------
+namespace {
+
+struct LaunchShape {
+	unsigned block, lds, blocks, refill, spillDepth;
+};
+
+unsigned pow2floor(unsigned v) {
+	unsigned p = 1;
+	while (p * 2 <= v) p *= 2;
+	return p;
+}
+
+LaunchShape chooseShape(lrh_scene *s, unsigned count, bool any, const lrh_trace_opts *o) {
+	lrh_device *dev = s->dev;
+	LaunchShape L;
+	L.block = (o && o->blockSize) ? o->blockSize : 256;
+	if (L.block != 64 && L.block != 128 && L.block != 256) throw Error(LRH_EINVAL, "blockSize must be 64, 128 or 256");
+	L.lds = (o && o->ldsStack) ? o->ldsStack : 16;
+	if (L.lds < 4 || L.lds > 64 || pow2floor(L.lds) != L.lds) throw Error(LRH_EINVAL, "ldsStack must be a power of two in 4..64");
+	L.refill = (o && o->refillThreshold) ? std::min(64u, o->refillThreshold) : 40;
+	const size_t ldsBytes = (size_t)L.lds * L.block * sizeof(uint2);
+	int perCU = (o && o->blocksPerCU) ? (int)o->blocksPerCU : 0;
+	if (!perCU) {
+		const uint64_t key = ((uint64_t)s->layout << 40) | ((uint64_t)s->twoLevel << 39) | ((uint64_t)any << 38) |
+				((uint64_t)L.block << 16) | L.lds;
+		std::lock_guard<std::mutex> g(dev->mu);
+		auto it = dev->occ.find(key);
+		if (it == dev->occ.end()) {
+			int nb = 0;
+			HIPCHK(lrhOccupancy(s->layout, s->twoLevel, any, L.block, ldsBytes, &nb));
+			dev->occ[key] = nb > 0 ? nb : 1;
+			perCU = nb > 0 ? nb : 1;
+		} else
+			perCU = it->second;
+	}
+	const unsigned persistent = (unsigned)perCU * (unsigned)dev->cuCount;
+	const unsigned need = (count + L.block - 1) / L.block;
+	L.blocks = std::max(1u, std::min(persistent, need));
+	// deepest stack: 3 pushes per level plus the object sentinel
+	const unsigned worst = 3u * (s->maxDepth + 2u) + 4u;
+	L.spillDepth = worst > L.lds ? worst - L.lds : 1u;
+	return L;
+}
+
+lrh_device::Workspace &workspace(lrh_device *dev, hipStream_t st, size_t spillBytes) {
+	std::lock_guard<std::mutex> g(dev->mu);
+	lrh_device::Workspace &w = dev->ws[st];
+	if (!w.counter) w.counter = (uint32_t *)dev->devAlloc(sizeof(uint32_t));
+	if (w.spillBytes < spillBytes) {
+		// the previous spill array may still be read by work queued on st
+		HIPCHK(hipStreamSynchronize(st));
+		dev->devFree(w.spill, w.spillBytes);
+		w.spill = nullptr;
+		w.spillBytes = 0;
+		w.spill = (uint2 *)dev->devAlloc(spillBytes);
+		w.spillBytes = spillBytes;
+	}
+	return w;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lrh_trace_ex(lrh_scene *s, const void *rays, void *hits, unsigned int count, unsigned int query,
+		const lrh_trace_opts *opts, void *stream) {
+	if (!s) return fail(LRH_EINVAL, "null scene");
+	if (count && (!rays || !hits)) return fail(LRH_EINVAL, "null ray or hit buffer");
+	if (query > 1) return fail(LRH_EINVAL, "query must be LRH_QUERY_CLOSEST or LRH_QUERY_ANY");
+	if (count > 0x7fffffffu) return fail(LRH_ERANGE, "at most 2^31-1 rays per trace");
+	if (!s->dev) return fail(LRH_EINVAL, "host-only scene cannot be traced");
+	if (((uintptr_t)rays & 15u) || ((uintptr_t)hits & 3u))
+		return fail(LRH_EINVAL, "ray buffer must be 16-byte aligned, hit buffer 4-byte aligned");
+	if (count == 0) return LRH_OK;
+	lrh_device *dev = s->dev;
+	return guarded([&]() {
+		dev->activate();
+		hipStream_t st = stream ? (hipStream_t)stream : dev->stream;
+		const bool any = query == LRH_QUERY_ANY;
+		const LaunchShape L = chooseShape(s, count, any, opts);
+		const unsigned nLanes = L.blocks * L.block;
+		lrh_device::Workspace &w = workspace(dev, st, (size_t)nLanes * L.spillDepth * sizeof(uint2));
+		LrhTraceParams P;
+		P.nodes = s->nodes.p;
+		P.tris = (const lrh_tri *)s->tris.p;
+		P.insts = (const lrh_inst *)s->insts.p;
+		P.xforms = (const float *)s->xforms.p;
+		P.motions = (const lrh_motion *)s->motions.p;
+		P.segs = (const lrh_motion_seg *)s->segs.p;
+		P.times = (const float *)s->times.p;
+		P.rays = (const lrh_ray_rec *)rays;
+		P.hits = (lrh_hit_rec *)hits;
+		P.nRays = count;
+		P.root = s->root;
+		P.counter = w.counter;
+		P.spill = w.spill;
+		P.spillDepth = L.spillDepth;
+		P.nLanes = nLanes;
+		P.ldsDepth = L.lds;
+		P.refill = L.refill;
+		P.error = dev->dError;
+		HIPCHK(lrhLaunchReset(w.counter, 1, st));
+		HIPCHK(lrhLaunchTrace(s->layout, s->twoLevel, any, dim3(L.blocks), dim3(L.block),
+				(size_t)L.lds * L.block * sizeof(uint2), st, P));
+		dev->rays += count;
+	});
+}
+
+int lrh_trace(lrh_scene *s, const void *rays, void *hits, unsigned int count, unsigned int query, void *stream) {
+	return lrh_trace_ex(s, rays, hits, count, query, nullptr, stream);
+}
+
+int lrh_trace_batch(lrh_device *dev, const lrh_batch_item *items, unsigned int nItems, const lrh_trace_opts *opts,
+		void *stream) {
+	if (!dev || (nItems && !items)) return fail(LRH_EINVAL, "null argument");
+	for (unsigned i = 0; i < nItems; ++i) {
+		if (!items[i].scene || items[i].scene->dev != dev) return fail(LRH_EINVAL, "batch item on another device");
+		const int rc = lrh_trace_ex(items[i].scene, items[i].rays, items[i].hits, items[i].count, items[i].query, opts,
+				stream);
+		if (rc != LRH_OK) return rc;
+	}
+	return LRH_OK;
+}
+
+int lrh_device_stats(lrh_device *dev, unsigned long long *rays, double *seconds) {
+	if (!dev) return fail(LRH_EINVAL, "null device");
+	if (rays) *rays = dev->rays.load();
+	if (seconds) *seconds = now() - dev->t0;
+	return LRH_OK;
+}
+
+}  // extern "C"

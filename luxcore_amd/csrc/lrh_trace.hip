@@ -1,0 +1,490 @@
+/*
+ * gfx950 traversal kernels of the HIP intersection device.
+ *
+ * One lane = one ray, wave64, persistent grid.  Per wave, a ballot over idle
+ * lanes takes a block of ray tickets with one atomic and hands them out by
+ * lane rank, so lanes whose ray terminated are refilled while the others keep
+ * traversing (the reference launches one thread per ray with no refill,
+ * include/luxrays/accelerators/bvh.cl:228-260).
+ *
+ * Traversal is ordered (children sorted by entry distance) over 4-wide
+ * nodes, with a per-lane stack whose top `ldsDepth` entries live in LDS and
+ * deeper entries spill to a global scratch array; popped entries whose entry
+ * distance is not below the current hit are dropped without a fetch.  The
+ * reference walks a skip-pointer array in memory order instead
+ * (bvh.cl:136-217, src/luxrays/accelerators/bvhaccel.cpp:184-257); both visit
+ * a superset of the boxes containing the closest hit, so the closest hit is
+ * the same up to ties at equal t.
+ *
+ * The triangle test is Triangle::Intersect
+ * (include/luxrays/core/geometry/triangle.h:55-89) with every operation
+ * rounded as written (the file builds with -ffp-contract=off and correctly
+ * rounded division) and the acceptance rule of BVHAccel::Intersect
+ * (bvhaccel.cpp:173, 232-233): a hit is kept iff mint <= t < best, best
+ * starting at maxt.
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lrh_layout.h"
+#include "lrh_kernels.h"
+
+#define LRH_SENTINEL 0xfffffffeu
+#define DEVI __device__ __forceinline__
+
+namespace {
+
+DEVI float asf(uint32_t u) { return __uint_as_float(u); }
+DEVI uint32_t asu(float f) { return __float_as_uint(f); }
+
+DEVI float safeRcp(float d) {
+	// a zero component becomes +-1e-20 so slab products stay finite
+	const float dd = fabsf(d) < 1e-20f ? copysignf(1e-20f, d) : d;
+	return 1.0f / dd;
+}
+
+DEVI float byteF(uint32_t w, int c) {
+	// compiles to v_cvt_f32_ubyteN
+	return (float)((w >> (8 * c)) & 0xffu);
+}
+
+struct LocalRay {
+	float ox, oy, oz, dx, dy, dz;
+	float ix, iy, iz;     // 1/d
+	float nx, ny, nz;     // -o/d
+};
+
+DEVI void setupRay(LocalRay &r) {
+	r.ix = safeRcp(r.dx);
+	r.iy = safeRcp(r.dy);
+	r.iz = safeRcp(r.dz);
+	r.nx = -r.ox * r.ix;
+	r.ny = -r.oy * r.iy;
+	r.nz = -r.oz * r.iz;
+}
+
+// Matrix * Point with the projective divide and Matrix * Vector
+// (include/luxrays/core/geometry/matrix4x4op.h:31-59), rounded as written.
+DEVI void xfRay(const float *M, const float wo[3], const float wd[3], LocalRay &r) {
+	const float x = wo[0], y = wo[1], z = wo[2];
+	float px = M[0] * x + M[1] * y + M[2] * z + M[3];
+	float py = M[4] * x + M[5] * y + M[6] * z + M[7];
+	float pz = M[8] * x + M[9] * y + M[10] * z + M[11];
+	const float w = M[12] * x + M[13] * y + M[14] * z + M[15];
+	if (w != 1.f) {
+		const float inv = 1.f / w;
+		px = inv * px; py = inv * py; pz = inv * pz;
+	}
+	r.ox = px; r.oy = py; r.oz = pz;
+	const float a = wd[0], b = wd[1], c = wd[2];
+	r.dx = M[0] * a + M[1] * b + M[2] * c;
+	r.dy = M[4] * a + M[5] * b + M[6] * c;
+	r.dz = M[8] * a + M[9] * b + M[10] * c;
+	setupRay(r);
+}
+
+// ---- motion sampling: InterpolatedTransform::Sample / MotionSystem::Sample
+// (src/luxrays/core/geometry/motionsystem.cpp:89-159, 340-346) ----
+DEVI float lerpf(float t, float a, float b) { return a + t * (b - a); }
+DEVI float qdot(const float *a, const float *b) { return a[0] * b[0] + (a[1] * b[1] + a[2] * b[2] + a[3] * b[3]); }
+
+__device__ __noinline__ void motionSample(const lrh_motion *mot, const lrh_motion_seg *segs, const float *times,
+		float time, float M[16]) {
+	// upper_bound(times, time), clamped to the last key
+	const uint32_t n = mot->nTimes;
+	const float *tk = times + mot->firstTime;
+	uint32_t idx = 0;
+	while (idx < n && !(time < tk[idx])) ++idx;
+	if (idx > n - 1) idx = n - 1;
+	const lrh_motion_seg &g = segs[mot->firstSeg + idx];
+	const uint32_t f = g.flags;
+	if (!(f & LRH_MF_ACTIVE) || time <= g.startTime) {
+		for (int i = 0; i < 16; ++i) M[i] = g.start[i];
+		return;
+	}
+	if (time >= g.endTime) {
+		for (int i = 0; i < 16; ++i) M[i] = g.end[i];
+		return;
+	}
+	const float w = g.endTime - g.startTime;
+	const float d = time - g.startTime;
+	const float le = d / w;
+	if ((f & LRH_MF_TRANS) && !(f & (LRH_MF_SCALE | LRH_MF_ROT))) {
+		for (int i = 0; i < 16; ++i) M[i] = g.start[i];
+		if (f & LRH_MF_TX) M[3] = lerpf(le, g.sT[0], g.eT[0]);
+		if (f & LRH_MF_TY) M[7] = lerpf(le, g.sT[1], g.eT[1]);
+		if (f & LRH_MF_TZ) M[11] = lerpf(le, g.sT[2], g.eT[2]);
+		return;
+	}
+	if (f & LRH_MF_ROT) {
+		// Slerp (src/luxrays/core/geometry/quaternion.cpp:143-164) + ToMatrix (:115-140)
+		float cos_phi = qdot(g.startQ, g.endQ);
+		const float sign = (cos_phi > 0.f) ? 1.f : -1.f;
+		cos_phi *= sign;
+		float f1, f2;
+		if (1.f - cos_phi > 1e-6f) {
+			const float phi = acosf(cos_phi);
+			const float sin_phi = sinf(phi);
+			f1 = sinf((1.f - le) * phi) / sin_phi;
+			f2 = sinf(le * phi) / sin_phi;
+		} else {
+			f1 = 1.f - le;
+			f2 = le;
+		}
+		const float gg = sign * f2;
+		float q[4];
+		for (int i = 0; i < 4; ++i) q[i] = g.startQ[i] * f1 + g.endQ[i] * gg;
+		const float x = q[1], y = q[2], z = q[3], qw = q[0];
+		const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z;
+		const float xw = x * qw, yw = y * qw, zw = z * qw;
+		M[0] = 1.f - 2.f * (yy + zz);
+		M[4] = 2.f * (xy - zw);
+		M[8] = 2.f * (xz + yw);
+		M[1] = 2.f * (xy + zw);
+		M[5] = 1.f - 2.f * (xx + zz);
+		M[9] = 2.f * (yz - xw);
+		M[2] = 2.f * (xz - yw);
+		M[6] = 2.f * (yz + xw);
+		M[10] = 1.f - 2.f * (xx + yy);
+		M[3] = M[7] = M[11] = 0.f;
+		M[12] = M[13] = M[14] = 0.f;
+		M[15] = 1.f;
+	} else {
+		for (int i = 0; i < 16; ++i) M[i] = g.startR[i];
+	}
+	float S[3];
+	for (int k = 0; k < 3; ++k) S[k] = (f & LRH_MF_SCALE) ? lerpf(le, g.sS[k], g.eS[k]) : g.sS[k];
+	for (int j = 0; j < 3; ++j) {
+		M[0 + j] = S[0] * M[0 + j];
+		M[4 + j] = S[1] * M[4 + j];
+		M[8 + j] = S[2] * M[8 + j];
+	}
+	M[3] = (f & LRH_MF_TX) ? lerpf(le, g.sT[0], g.eT[0]) : g.sT[0];
+	M[7] = (f & LRH_MF_TY) ? lerpf(le, g.sT[1], g.eT[1]) : g.sT[1];
+	M[11] = (f & LRH_MF_TZ) ? lerpf(le, g.sT[2], g.eT[2]) : g.sT[2];
+}
+
+// ---- node tests: up to 4 children -> (entry distance, child ref) ----
+constexpr float kLo = 1.0f - 1.0f / 1048576.0f;  // conservative widening of the slab interval
+constexpr float kHi = 1.0f + 1.0f / 1048576.0f;
+
+DEVI void testNodeQ(const lrh_node4q *__restrict__ nodes, uint32_t idx, const LocalRay &r, float mint, float best,
+		float t[4], uint32_t c[4]) {
+	const uint4 *p = reinterpret_cast<const uint4 *>(nodes + idx);
+	const uint4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
+	const float sx = asf((q0.w & 0xffu) << 23), sy = asf(((q0.w >> 8) & 0xffu) << 23),
+			sz = asf(((q0.w >> 16) & 0xffu) << 23);
+	const float Ax = sx * r.ix, Ay = sy * r.iy, Az = sz * r.iz;
+	const float Bx = fmaf(asf(q0.x), r.ix, r.nx), By = fmaf(asf(q0.y), r.iy, r.ny), Bz = fmaf(asf(q0.z), r.iz, r.nz);
+	const uint32_t nX = r.ix >= 0.f ? q1.x : q1.y, fX = r.ix >= 0.f ? q1.y : q1.x;
+	const uint32_t nY = r.iy >= 0.f ? q1.z : q1.w, fY = r.iy >= 0.f ? q1.w : q1.z;
+	const uint32_t nZ = r.iz >= 0.f ? q2.x : q2.y, fZ = r.iz >= 0.f ? q2.y : q2.x;
+	c[0] = q2.z; c[1] = q2.w; c[2] = q3.x; c[3] = q3.y;
+#pragma unroll
+	for (int k = 0; k < 4; ++k) {
+		const float tnx = fmaf(byteF(nX, k), Ax, Bx), tfx = fmaf(byteF(fX, k), Ax, Bx);
+		const float tny = fmaf(byteF(nY, k), Ay, By), tfy = fmaf(byteF(fY, k), Ay, By);
+		const float tnz = fmaf(byteF(nZ, k), Az, Bz), tfz = fmaf(byteF(fZ, k), Az, Bz);
+		const float tn = fmaxf(fmaxf(tnx, tny), tnz) * kLo;
+		const float tf = fminf(fminf(tfx, tfy), tfz) * kHi;
+		const float a = fmaxf(tn, mint), b = fminf(tf, best);
+		const bool hit = (a <= b) && (c[k] != LRH_CHILD_EMPTY);
+		t[k] = hit ? a : INFINITY;
+	}
+}
+
+DEVI void testNodeF(const lrh_node4f *__restrict__ nodes, uint32_t idx, const LocalRay &r, float mint, float best,
+		float t[4], uint32_t c[4]) {
+	const float4 *p = reinterpret_cast<const float4 *>(nodes + idx);
+	const float4 lx = p[0], hx = p[1], ly = p[2], hy = p[3], lz = p[4], hz = p[5];
+	const uint4 cc = reinterpret_cast<const uint4 *>(p)[6];
+	c[0] = cc.x; c[1] = cc.y; c[2] = cc.z; c[3] = cc.w;
+	const float LX[4] = {lx.x, lx.y, lx.z, lx.w}, HX[4] = {hx.x, hx.y, hx.z, hx.w};
+	const float LY[4] = {ly.x, ly.y, ly.z, ly.w}, HY[4] = {hy.x, hy.y, hy.z, hy.w};
+	const float LZ[4] = {lz.x, lz.y, lz.z, lz.w}, HZ[4] = {hz.x, hz.y, hz.z, hz.w};
+	const bool sx = r.ix >= 0.f, sy = r.iy >= 0.f, sz = r.iz >= 0.f;
+#pragma unroll
+	for (int k = 0; k < 4; ++k) {
+		const float tnx = fmaf(sx ? LX[k] : HX[k], r.ix, r.nx), tfx = fmaf(sx ? HX[k] : LX[k], r.ix, r.nx);
+		const float tny = fmaf(sy ? LY[k] : HY[k], r.iy, r.ny), tfy = fmaf(sy ? HY[k] : LY[k], r.iy, r.ny);
+		const float tnz = fmaf(sz ? LZ[k] : HZ[k], r.iz, r.nz), tfz = fmaf(sz ? HZ[k] : LZ[k], r.iz, r.nz);
+		const float tn = fmaxf(fmaxf(tnx, tny), tnz) * kLo;
+		const float tf = fminf(fminf(tfx, tfy), tfz) * kHi;
+		const float a = fmaxf(tn, mint), b = fminf(tf, best);
+		const bool hit = (a <= b) && (c[k] != LRH_CHILD_EMPTY);
+		t[k] = hit ? a : INFINITY;
+	}
+}
+
+DEVI void cswap(float &ta, uint32_t &ca, float &tb, uint32_t &cb) {
+	const bool s = tb < ta;
+	const float t0 = s ? tb : ta, t1 = s ? ta : tb;
+	const uint32_t c0 = s ? cb : ca, c1 = s ? ca : cb;
+	ta = t0; tb = t1; ca = c0; cb = c1;
+}
+
+// ---- stack: top ldsDepth entries in LDS (ring), the rest spilled ----
+struct Stack {
+	uint2 *lds;
+	uint2 *spill;
+	uint32_t stride;    // blockDim (LDS) / nLanes (spill)
+	uint32_t lane;      // threadIdx / global lane
+	uint32_t gstride;
+	uint32_t glane;
+	uint32_t mask;      // ldsDepth - 1
+	uint32_t depth;     // ldsDepth
+	uint32_t spillDepth;
+	uint32_t sp;
+};
+
+DEVI bool push(Stack &s, uint32_t ref, float t, uint32_t *err) {
+	uint2 *e = &s.lds[(s.sp & s.mask) * s.stride + s.lane];
+	if (s.sp >= s.depth) {
+		const uint32_t k = s.sp - s.depth;
+		if (k >= s.spillDepth) {
+			atomicOr(err, 1u);
+			return false;
+		}
+		s.spill[(size_t)k * s.gstride + s.glane] = *e;
+	}
+	*e = make_uint2(ref, asu(t));
+	++s.sp;
+	return true;
+}
+
+DEVI uint2 pop(Stack &s) {
+	--s.sp;
+	uint2 *e = &s.lds[(s.sp & s.mask) * s.stride + s.lane];
+	const uint2 v = *e;
+	if (s.sp >= s.depth) *e = s.spill[(size_t)(s.sp - s.depth) * s.gstride + s.glane];
+	return v;
+}
+
+constexpr uint32_t kNoRay = 0xffffffffu;
+constexpr uint32_t kMaxSteps = 1u << 24;  // per-ray guard against corrupt trees
+
+}  // namespace
+
+template <int LAYOUT, bool TWO, bool ANY>
+__global__ __launch_bounds__(256) void lrh_trace_kernel(LrhTraceParams P) {
+	extern __shared__ uint2 ldsStack[];
+	const uint32_t lane = __lane_id();
+	Stack st;
+	st.lds = ldsStack;
+	st.spill = P.spill;
+	st.stride = blockDim.x;
+	st.lane = threadIdx.x;
+	st.gstride = P.nLanes;
+	st.glane = blockIdx.x * blockDim.x + threadIdx.x;
+	st.mask = P.ldsDepth - 1;
+	st.depth = P.ldsDepth;
+	st.spillDepth = P.spillDepth;
+	st.sp = 0;
+
+	uint32_t rid = kNoRay;
+	LocalRay r;
+	float wo[3], wd[3];
+	float mint = 0.f, best = 0.f, time = 0.f, hb1 = 0.f, hb2 = 0.f;
+	uint32_t hTri = LRH_NULL_INDEX, hMesh = LRH_NULL_INDEX, cur = LRH_CHILD_EMPTY, obj = 0, steps = 0;
+	float curT = 0.f;
+	bool inInst = false;
+	bool exhausted = false;
+
+	// ticket source: one contiguous range per item (a single item for plain traces)
+	for (;;) {
+		if (!exhausted) {
+			const uint64_t idle = __ballot(rid == kNoRay);
+			if (idle) {
+				const uint32_t nIdle = __popcll(idle);
+				const int leader = __ffsll((long long)idle) - 1;
+				uint32_t base = 0;
+				if ((int)lane == leader) base = atomicAdd(P.counter, nIdle);
+				base = __shfl(base, leader);
+				if (base + nIdle >= P.nRays) exhausted = true;
+				if (rid == kNoRay) {
+					const uint32_t myr = base + __popcll(idle & ((1ull << lane) - 1ull));
+					if (myr < P.nRays) {
+						const uint4 *rp = reinterpret_cast<const uint4 *>(P.rays + myr);
+						const uint4 a = rp[0], b = rp[1], c = rp[2];
+						if (!(c.y & 1u)) {  // RAY_FLAGS_MASKED rays are skipped, hit untouched
+							rid = myr;
+							wo[0] = asf(a.x); wo[1] = asf(a.y); wo[2] = asf(a.z);
+							wd[0] = asf(a.w); wd[1] = asf(b.x); wd[2] = asf(b.y);
+							mint = asf(b.z);
+							best = asf(b.w);
+							time = asf(c.x);
+							r.ox = wo[0]; r.oy = wo[1]; r.oz = wo[2];
+							r.dx = wd[0]; r.dy = wd[1]; r.dz = wd[2];
+							setupRay(r);
+							hb1 = 0.f; hb2 = 0.f;
+							hTri = LRH_NULL_INDEX; hMesh = LRH_NULL_INDEX;
+							cur = P.root;
+							curT = -INFINITY;
+							inInst = false;
+							steps = 0;
+							st.sp = 0;
+						}
+					}
+				}
+			}
+		}
+		if (!__ballot(rid != kNoRay)) {
+			if (exhausted) break;
+			continue;
+		}
+		const uint32_t minLive = exhausted ? 1u : P.refill;
+		do {
+			if (rid != kNoRay) {
+				bool done = false;
+				if (cur == LRH_CHILD_EMPTY) {
+					if (st.sp == 0) {
+						done = true;
+					} else {
+						const uint2 e = pop(st);
+						cur = e.x;
+						curT = asf(e.y);
+						if (TWO && cur == LRH_SENTINEL) {
+							// leaving an object's tree: back to the world-space ray
+							r.ox = wo[0]; r.oy = wo[1]; r.oz = wo[2];
+							r.dx = wd[0]; r.dy = wd[1]; r.dz = wd[2];
+							setupRay(r);
+							inInst = false;
+							cur = LRH_CHILD_EMPTY;
+						} else if (!(curT < best)) {
+							cur = LRH_CHILD_EMPTY;  // entry not before the current hit
+						}
+					}
+				}
+				if (!done && cur != LRH_CHILD_EMPTY) {
+					if (++steps > kMaxSteps) {
+						atomicOr(P.error, 2u);
+						done = true;
+					} else if (cur & LRH_CHILD_LEAF) {
+						const uint32_t start = cur & LRH_LEAF_START_MASK;
+						const uint32_t cnt = ((cur >> LRH_LEAF_START_BITS) & 7u) + 1u;
+						cur = LRH_CHILD_EMPTY;
+						if (TWO && !inInst) {
+							// root leaf: enter the object's tree in its local space
+							const lrh_inst in = P.insts[start];
+							if (in.kind == 1u) {
+								xfRay(P.xforms + 16 * (size_t)in.index, wo, wd, r);
+							} else if (in.kind == 2u) {
+								float M[16];
+								motionSample(P.motions + in.index, P.segs, P.times, time, M);
+								xfRay(M, wo, wd, r);
+							}
+							obj = in.object;
+							inInst = true;
+							if (push(st, LRH_SENTINEL, -INFINITY, P.error)) {
+								cur = in.root;
+								curT = -INFINITY;
+							} else
+								done = true;
+						} else {
+							for (uint32_t k = 0; k < cnt; ++k) {
+								const uint4 *tp = reinterpret_cast<const uint4 *>(P.tris + start + k);
+								const uint4 a = tp[0], b = tp[1], c = tp[2];
+								// Triangle::Intersect, rounded as written
+								const float p0x = asf(a.x), p0y = asf(a.y), p0z = asf(a.z);
+								const float e1x = asf(a.w), e1y = asf(b.x), e1z = asf(b.y);
+								const float e2x = asf(b.z), e2y = asf(b.w), e2z = asf(c.x);
+								const float s1x = (r.dy * e2z) - (r.dz * e2y);
+								const float s1y = (r.dz * e2x) - (r.dx * e2z);
+								const float s1z = (r.dx * e2y) - (r.dy * e2x);
+								const float div = s1x * e1x + s1y * e1y + s1z * e1z;
+								const float inv = 1.f / div;
+								const float ddx = r.ox - p0x, ddy = r.oy - p0y, ddz = r.oz - p0z;
+								const float b1 = (ddx * s1x + ddy * s1y + ddz * s1z) * inv;
+								const float s2x = (ddy * e1z) - (ddz * e1y);
+								const float s2y = (ddz * e1x) - (ddx * e1z);
+								const float s2z = (ddx * e1y) - (ddy * e1x);
+								const float b2 = (r.dx * s2x + r.dy * s2y + r.dz * s2z) * inv;
+								const float b0 = 1.f - b1 - b2;
+								const float t = (e2x * s2x + e2y * s2y + e2z * s2z) * inv;
+								const bool ok = (div != 0.f) && !(b1 < 0.f) && !(b2 < 0.f) && !(b0 < 0.f) &&
+										!(t < mint) && (t < best);
+								if (ok) {
+									best = t;
+									hb1 = b1;
+									hb2 = b2;
+									hTri = c.y;
+									hMesh = TWO ? obj : c.z;
+									if (ANY) {
+										done = true;
+										break;
+									}
+								}
+							}
+						}
+					} else {
+						float t4[4];
+						uint32_t c4[4];
+						if (LAYOUT == 0)
+							testNodeQ(reinterpret_cast<const lrh_node4q *>(P.nodes), cur, r, mint, best, t4, c4);
+						else
+							testNodeF(reinterpret_cast<const lrh_node4f *>(P.nodes), cur, r, mint, best, t4, c4);
+						cswap(t4[0], c4[0], t4[1], c4[1]);
+						cswap(t4[2], c4[2], t4[3], c4[3]);
+						cswap(t4[0], c4[0], t4[2], c4[2]);
+						cswap(t4[1], c4[1], t4[3], c4[3]);
+						cswap(t4[1], c4[1], t4[2], c4[2]);
+						bool ok = true;
+						if (t4[3] != INFINITY) ok = ok && push(st, c4[3], t4[3], P.error);
+						if (t4[2] != INFINITY) ok = ok && push(st, c4[2], t4[2], P.error);
+						if (t4[1] != INFINITY) ok = ok && push(st, c4[1], t4[1], P.error);
+						if (t4[0] != INFINITY) {
+							cur = c4[0];
+							curT = t4[0];
+						} else
+							cur = LRH_CHILD_EMPTY;
+						if (!ok) done = true;
+					}
+				}
+				if (done) {
+					lrh_hit_rec *h = P.hits + rid;
+					h->t = best;
+					h->b1 = hb1;
+					h->b2 = hb2;
+					h->meshIndex = hMesh;
+					h->triangleIndex = hTri;
+					rid = kNoRay;
+					cur = LRH_CHILD_EMPTY;
+				}
+			}
+		} while ((uint32_t)__popcll(__ballot(rid != kNoRay)) >= minLive);
+	}
+}
+
+__global__ void lrh_reset_kernel(uint32_t *p, uint32_t n) {
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) p[i] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// host-side launch table
+// ---------------------------------------------------------------------------
+typedef void (*KernelFn)(LrhTraceParams);
+
+static const void *kernelPtr(int layout, bool two, bool any) {
+	static const void *tab[2][2][2] = {
+		{{(const void *)&lrh_trace_kernel<0, false, false>, (const void *)&lrh_trace_kernel<0, false, true>},
+		 {(const void *)&lrh_trace_kernel<0, true, false>, (const void *)&lrh_trace_kernel<0, true, true>}},
+		{{(const void *)&lrh_trace_kernel<1, false, false>, (const void *)&lrh_trace_kernel<1, false, true>},
+		 {(const void *)&lrh_trace_kernel<1, true, false>, (const void *)&lrh_trace_kernel<1, true, true>}}};
+	return tab[layout ? 1 : 0][two ? 1 : 0][any ? 1 : 0];
+}
+
+hipError_t lrhLaunchReset(uint32_t *p, uint32_t n, hipStream_t s) {
+	hipLaunchKernelGGL(lrh_reset_kernel, dim3((n + 63) / 64), dim3(64), 0, s, p, n);
+	return hipGetLastError();
+}
+
+hipError_t lrhOccupancy(int layout, bool two, bool any, int block, size_t lds, int *blocksPerCU) {
+	return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kernelPtr(layout, two, any), block, lds);
+}
+
+hipError_t lrhLaunchTrace(int layout, bool two, bool any, dim3 grid, dim3 block, size_t lds, hipStream_t s,
+		const LrhTraceParams &P) {
+	void *args[] = {(void *)&P};
+	return hipLaunchKernel(kernelPtr(layout, two, any), grid, block, args, lds, s);
+}

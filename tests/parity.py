@@ -1,0 +1,82 @@
+"""Hit-record comparison used by the parity tests.
+
+Rule (from the acceptance bar of the device): hit/miss and (meshIndex,
+triangleIndex) must be identical except for rays whose hit barycentrics lie
+within 1e-6 of a triangle edge (the edge band, counted) or whose two hits are
+an exact tie in t (both triangles accept the ray at the same t; which one is
+kept depends on visit order).  Where the indices agree, t, b1 and b2 must
+agree within 1e-5 relative.
+"""
+import numpy as np
+
+from luxcore_amd.raytypes import NULL_INDEX
+
+EDGE_BAND = 1e-6
+REL_TOL = 1e-5
+
+
+def _min_bary(h):
+    b1 = h["b1"].astype(np.float64)
+    b2 = h["b2"].astype(np.float64)
+    return np.minimum(np.minimum(b1, b2), 1.0 - b1 - b2)
+
+
+def compare(ref, got, tie_check=None, rel_tol=REL_TOL):
+    """Returns a dict of counts; tie_check(i) -> True when ray i is an exact
+    tie (the got triangle is hit at the reference t)."""
+    assert ref.shape == got.shape
+    rhit = ref["meshIndex"] != NULL_INDEX
+    ghit = got["meshIndex"] != NULL_INDEX
+    same_idx = (ref["meshIndex"] == got["meshIndex"]) & (ref["triangleIndex"] == got["triangleIndex"])
+    same_idx &= rhit == ghit
+    mism = np.nonzero(~same_idx)[0]
+    edge = np.zeros(ref.shape[0], bool)
+    edge[rhit] |= _min_bary(ref[rhit]) < EDGE_BAND
+    edge[ghit] |= _min_bary(got[ghit]) < EDGE_BAND
+    n_edge = n_tie = n_bad = 0
+    bad = []
+    for i in mism:
+        if edge[i]:
+            n_edge += 1
+        elif rhit[i] and ghit[i] and ref["t"][i] == got["t"][i] and (tie_check is None or tie_check(i)):
+            n_tie += 1
+        else:
+            n_bad += 1
+            if len(bad) < 10:
+                bad.append(int(i))
+    both = np.nonzero(same_idx & rhit)[0]
+
+    def rel(a, b):
+        a = a.astype(np.float64)
+        b = b.astype(np.float64)
+        return np.abs(a - b) / np.maximum(np.abs(a), 1e-30)
+    worst = {}
+    for f in ("t", "b1", "b2"):
+        r = rel(ref[f][both], got[f][both]) if both.size else np.zeros(1)
+        # b1/b2 near 0 are compared absolutely at the tolerance scale
+        if f != "t" and both.size:
+            r = np.minimum(r, np.abs(ref[f][both].astype(np.float64) - got[f][both].astype(np.float64)))
+        worst[f] = float(r.max()) if r.size else 0.0
+    miss_t_ok = bool(np.all(got["t"][~rhit & ~ghit] == ref["t"][~rhit & ~ghit]))
+    return {"rays": int(ref.shape[0]), "hits": int(rhit.sum()), "mismatch": int(mism.size),
+            "edge_band": n_edge, "ties": n_tie, "bad": n_bad, "bad_examples": bad,
+            "worst_rel": worst, "miss_t_ok": miss_t_ok, "edge_band_rays": int(edge.sum())}
+
+
+def assert_parity(stats, rel_tol=REL_TOL):
+    assert stats["bad"] == 0, stats
+    assert stats["miss_t_ok"], stats
+    for f, v in stats["worst_rel"].items():
+        assert v <= rel_tol, (f, stats)
+
+
+def tri_tie_checker(oracle_mod, meshes, rays, ref, got):
+    """tie check: the got triangle accepts the ray at exactly the ref t."""
+    def check(i):
+        m = int(got["meshIndex"][i])
+        t = int(got["triangleIndex"][i])
+        tv = meshes[m].tris[t]
+        v = meshes[m].verts
+        hit, tt, _, _ = oracle_mod.triangle_intersect(rays[i], v[tv[0]], v[tv[1]], v[tv[2]])
+        return hit and tt == ref["t"][i]
+    return check

@@ -1,0 +1,165 @@
+"""GPU parity of the single-level device (BVHAccel path) against the oracle.
+
+Every case prints its name/seed/shape first so a failing GPU run names it.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from luxcore_amd import scenes
+from luxcore_amd.raytypes import HIT_DTYPE, NULL_INDEX, RAY_DTYPE, RAY_FLAGS_MASKED, make_rays
+from tests.parity import assert_parity, compare, tri_tie_checker
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(name, **kw):
+    print(f"[case] {name} {kw}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def plain():
+    meshes = scenes.plain_scene(seed=1)
+    rays = scenes.plain_rays(1 << 16, seed=2)
+    ob = oracle.OracleBVH(meshes, builder="SAH")
+    ref = ob.intersect(rays)
+    return meshes, rays, ref
+
+
+@pytest.mark.parametrize("layout,max_leaf", [(0, 4), (0, 1), (0, 8), (1, 4)])
+def test_plain_closest_parity(gpu_device, plain, layout, max_leaf):
+    meshes, rays, ref = plain
+    _case("plain_closest", layout=layout, max_leaf=max_leaf, rays=rays.shape[0])
+    sc = gpu_device.scene_bvh(meshes, layout=layout, max_leaf=max_leaf)
+    got = gpu_device.trace_numpy(sc, rays)
+    st = compare(ref, got, tri_tie_checker(oracle, meshes, rays, ref, got))
+    print(st, flush=True)
+    assert_parity(st)
+    assert st["hits"] > 0.99 * rays.shape[0]
+
+
+def test_plain_anyhit(gpu_device, plain):
+    meshes, rays, ref = plain
+    _case("plain_any", rays=rays.shape[0])
+    sc = gpu_device.scene_bvh(meshes)
+    got = gpu_device.trace_numpy(sc, rays, query=1)
+    rhit = ref["meshIndex"] != NULL_INDEX
+    ghit = got["meshIndex"] != NULL_INDEX
+    assert np.array_equal(rhit, ghit)
+    # every reported any-hit is a real hit of that triangle at that t
+    idx = np.nonzero(ghit)[0][:2000]
+    for i in idx:
+        m, t = int(got["meshIndex"][i]), int(got["triangleIndex"][i])
+        tv = meshes[m].tris[t]
+        v = meshes[m].verts
+        hit, tt, b1, b2 = oracle.triangle_intersect(rays[i], v[tv[0]], v[tv[1]], v[tv[2]])
+        assert hit and tt == got["t"][i] and b1 == got["b1"][i] and b2 == got["b2"][i], i
+        assert rays["mint"][i] <= tt < rays["maxt"][i]
+
+
+def test_masked_rays_untouched(gpu_device, plain):
+    meshes, rays, ref = plain
+    _case("masked", rays=4096)
+    r = rays[:4096].copy()
+    r["flags"][::3] = RAY_FLAGS_MASKED
+    init = np.zeros(r.shape[0], HIT_DTYPE)
+    init["t"] = -7.0
+    init["meshIndex"] = 0xABCDEF
+    init["triangleIndex"] = 0x123456
+    sc = gpu_device.scene_bvh(meshes)
+    got = gpu_device.trace_numpy(sc, r, hits_init=init)
+    masked = (r["flags"] & RAY_FLAGS_MASKED) != 0
+    assert np.array_equal(got[masked], init[masked])
+    st = compare(ref[:4096][~masked], got[~masked])
+    assert_parity(st)
+
+
+def test_finite_maxt_and_mint(gpu_device, plain):
+    meshes, rays, _ = plain
+    _case("maxt", rays=16384)
+    r = rays[:16384].copy()
+    rng = np.random.Generator(np.random.PCG64(11))
+    r["maxt"] = rng.uniform(0.01, 3.0, r.shape[0]).astype(np.float32)
+    r["mint"] = rng.uniform(0.0, 0.5, r.shape[0]).astype(np.float32)
+    ob = oracle.OracleBVH(meshes, builder="CLASSIC")
+    ref = ob.intersect(r)
+    sc = gpu_device.scene_bvh(meshes)
+    got = gpu_device.trace_numpy(sc, r)
+    st = compare(ref, got, tri_tie_checker(oracle, meshes, r, ref, got))
+    print(st, flush=True)
+    assert_parity(st)
+    miss = got["meshIndex"] == NULL_INDEX
+    assert np.array_equal(got["t"][miss], r["maxt"][miss])
+
+
+def test_empty_scene_all_miss(gpu_device):
+    _case("empty")
+    sc = gpu_device.scene_bvh([])
+    r = scenes.plain_rays(1000, seed=3)
+    r["maxt"][::2] = 5.0
+    got = gpu_device.trace_numpy(sc, r)
+    assert np.all(got["meshIndex"] == NULL_INDEX)
+    assert np.all(got["triangleIndex"] == NULL_INDEX)
+    assert np.array_equal(got["t"], r["maxt"])
+
+
+def test_single_triangle_and_ragged_counts(gpu_device):
+    _case("single_triangle")
+    m = scenes.Mesh(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), np.array([[0, 1, 2]], np.uint32))
+    for n in (1, 63, 64, 65, 1000, 4097):
+        rng = np.random.Generator(np.random.PCG64(n))
+        o = np.stack([rng.uniform(-0.2, 1.2, n), rng.uniform(-0.2, 1.2, n), np.full(n, 1.0)], 1)
+        r = make_rays(o, np.tile([0, 0, -1], (n, 1)))
+        ref = oracle.bruteforce([m], r)
+        got = gpu_device.trace_numpy(gpu_device.scene_bvh([m]), r)
+        st = compare(ref, got)
+        assert_parity(st)
+
+
+def test_heightfield_bounce_parity(gpu_device):
+    cells = 256
+    _case("heightfield_small", cells=cells)
+    meshes = scenes.heightfield_scene(cells=cells, n_spheres=8, seed=3)
+    prim = scenes.pinhole_rays(320, 180)
+    ob = oracle.OracleBVH(meshes, builder="SAH")
+    sc = gpu_device.scene_bvh(meshes)
+    for name, r in (("primary", prim),):
+        ref = ob.intersect(r)
+        got = gpu_device.trace_numpy(sc, r)
+        st = compare(ref, got, tri_tie_checker(oracle, meshes, r, ref, got))
+        print(name, st, flush=True)
+        assert_parity(st)
+    bounce = scenes.bounce_rays(meshes, prim, ref)
+    ref_b = ob.intersect(bounce)
+    got_b = gpu_device.trace_numpy(sc, bounce)
+    st = compare(ref_b, got_b, tri_tie_checker(oracle, meshes, bounce, ref_b, got_b))
+    print("bounce", st, flush=True)
+    assert_parity(st)
+    shadow = scenes.shadow_rays(meshes, prim, ref)
+    ref_s = ob.intersect(shadow)
+    got_s = gpu_device.trace_numpy(sc, shadow, query=1)
+    assert np.array_equal(ref_s["meshIndex"] == NULL_INDEX, got_s["meshIndex"] == NULL_INDEX)
+
+
+def test_full_size_plain_properties(gpu_device):
+    """1,048,576 rays on the 101,932-tri scene: properties that hold at any
+    size (determinism, layout independence, any-hit/closest agreement)."""
+    _case("plain_full", rays=1 << 20)
+    meshes = scenes.plain_scene(seed=1)
+    rays = scenes.plain_rays(1 << 20, seed=2)
+    s0 = gpu_device.scene_bvh(meshes, layout=0)
+    s1 = gpu_device.scene_bvh(meshes, layout=1, max_leaf=2)
+    a = gpu_device.trace_numpy(s0, rays)
+    b = gpu_device.trace_numpy(s0, rays)
+    assert np.array_equal(a, b)
+    c = gpu_device.trace_numpy(s1, rays)
+    st = compare(a, c)
+    print(st, flush=True)
+    assert st["bad"] == 0 and st["mismatch"] - st["edge_band"] < 1e-5 * rays.shape[0] + 5
+    anyh = gpu_device.trace_numpy(s0, rays, query=1)
+    assert np.array_equal(a["meshIndex"] == NULL_INDEX, anyh["meshIndex"] == NULL_INDEX)
+    # a random subset against the oracle
+    sel = np.random.Generator(np.random.PCG64(5)).choice(rays.shape[0], 8192, replace=False)
+    ref = oracle.OracleBVH(meshes).intersect(rays[sel])
+    st = compare(ref, a[sel], tri_tie_checker(oracle, meshes, rays[sel], ref, a[sel]))
+    assert_parity(st)
