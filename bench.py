@@ -1,0 +1,173 @@
+"""Headline benchmark: incoherent closest-hit Mrays/s on one MI355X.
+
+Workload (default): the 3,240 x 3,240-cell fBm heightfield + 64 level-6
+icospheres (26,238,080 triangles, one single-level tree); the rays are one
+cosine-weighted diffuse bounce from every hit of a 1920x1080 pinhole camera
+(the incoherent set).  One step = one trace of the whole bounce buffer
+(inputs resident in HBM), timed with HIP events on the trace stream.
+
+Prints one JSON line (see the keys below).  The CPU baseline is the oracle
+(C++ restatement of the reference BVHAccel::Intersect over a 4-ary binned-SAH
+tree in the reference node layout) timed on this host's cores over a bounded
+sample of the same rays.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+from luxcore_amd import device as D  # noqa: E402
+from luxcore_amd import scenes  # noqa: E402
+from luxcore_amd.raytypes import HIT_DTYPE, NULL_INDEX  # noqa: E402
+
+METRIC = "Mrays/s incoherent closest-hit, 10M tris, 1 GPU; p50/p99 wavefront latency"
+HBM_PEAK_GBS = 8000.0
+RAY_BYTES, HIT_BYTES = 48, 20
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def build_workload(args, dev):
+    t0 = time.time()
+    meshes = scenes.heightfield_scene(cells=args.cells, n_spheres=64, seed=3)
+    ntris = sum(m.ntris for m in meshes)
+    log(f"scene: {ntris} tris in {len(meshes)} meshes ({time.time() - t0:.1f}s)")
+    scene = dev.scene_bvh(meshes, layout=args.layout, max_leaf=args.max_leaf)
+    info = scene.info()
+    log(f"device tree: {info['nodeCount']} nodes, {info['nodeBytes'] / 2**20:.0f} MiB nodes, "
+        f"{info['primBytes'] / 2**20:.0f} MiB tris, build {info['buildSeconds']:.1f}s")
+    prim = scenes.pinhole_rays(1920, 1080)
+    hits = dev.trace_numpy(scene, prim)
+    nhit = int((hits["meshIndex"] != NULL_INDEX).sum())
+    bounce = scenes.bounce_rays(meshes, prim, hits, seed=5)
+    log(f"primaries {prim.shape[0]} ({nhit} hit) -> {bounce.shape[0]} bounce rays")
+    return meshes, scene, info, bounce, ntris
+
+
+def cpu_baseline(meshes, rays, target_s):
+    import oracle  # the checker, used here only as the timed CPU baseline
+    threads = os.cpu_count() or 1
+    t0 = time.time()
+    ob = oracle.OracleBVH(meshes, builder="SAH", tree_type=4, threads=threads)
+    log(f"cpu oracle tree built in {time.time() - t0:.1f}s ({threads} threads)")
+    probe = rays[: 1 << 14]
+    t0 = time.time()
+    ob.intersect(probe, threads=threads)
+    rate = probe.shape[0] / max(time.time() - t0, 1e-6)
+    n = int(min(rays.shape[0], max(1 << 14, rate * target_s)))
+    sample = rays[:n]
+    t0 = time.time()
+    ob.intersect(sample, threads=threads)
+    dt = time.time() - t0
+    return {"value": n / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} of the {rays.shape[0]} bounce rays, oracle BVHAccel::Intersect over a 4-ary "
+                      f"binned-SAH tree, {threads} std::threads, {dt:.1f}s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cells", type=int, default=3240)
+    ap.add_argument("--layout", type=int, default=0)
+    ap.add_argument("--max-leaf", type=int, default=4)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--lds", type=int, default=0)
+    ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--refill", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build runs on a single "
+              "MI355X: the BVH would be replicated per GPU and rays split by range, with no exchange step)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    dev = D.Device(0)
+    log(f"device {dev.name}, {dev.cu_count} CUs")
+    meshes, scene, info, rays, ntris = build_workload(args, dev)
+    n = rays.shape[0]
+    rb = dev.upload(rays)
+    hb = dev.alloc(n * HIT_DTYPE.itemsize)
+    opts = D.trace_opts(args.block, args.lds, args.blocks_per_cu, args.refill)
+    use_opts = opts if any((args.block, args.lds, args.blocks_per_cu, args.refill)) else None
+
+    # counting pass: node and triangle-record fetches per ray of this traversal
+    nodes, prims, counted = dev.trace_stats(scene, rb, hb, n, 0, use_opts)
+    node_bytes = 64 if args.layout == 0 else 128
+    b_ray = RAY_BYTES + HIT_BYTES + node_bytes * nodes / counted + 48 * prims / counted
+    log(f"per ray: {nodes / counted:.2f} node visits, {prims / counted:.2f} triangle tests, "
+        f"B_ray = {b_ray:.1f} bytes")
+
+    for _ in range(args.warmup):
+        dev.trace(scene, rb, hb, n, 0, use_opts)
+    dev.sync()
+    evs = [(dev.event(), dev.event()) for _ in range(args.steps)]
+    dev.sync()
+    t0 = time.perf_counter()
+    for a, b in evs:
+        a.record()
+        dev.trace(scene, rb, hb, n, 0, use_opts)
+        b.record()
+    dev.sync()
+    wall = time.perf_counter() - t0
+    lat = np.array([a.elapsed_ms(b) for a, b in evs])
+    hits = dev.download(hb, HIT_DTYPE, n)
+    value = n * args.steps / wall / 1e6
+    kernel_ms = float(lat.mean())
+    achieved = b_ray * n / (kernel_ms * 1e-3) / 1e9
+    log(f"{value:.1f} Mrays/s wall, kernel {kernel_ms:.3f} ms avg, p50 {np.percentile(lat, 50):.3f} "
+        f"p99 {np.percentile(lat, 99):.3f} ms, hit rate {(hits['meshIndex'] != NULL_INDEX).mean():.3f}")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(meshes, rays, args.cpu_seconds)
+        log(f"cpu baseline {cpu['value']:.3f} Mrays/s on {cpu['cores']} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "heightfield 3240x3240 fBm + 64 L6 icospheres, 1920x1080 primaries -> 1 cosine "
+                               "bounce per hit (incoherent closest-hit)",
+                   "triangles": ntris, "rays_per_step": int(n), "node_layout": "bvh4 64B quantized"
+                   if args.layout == 0 else "bvh4 128B fp32", "max_leaf": args.max_leaf,
+                   "parallelism": "single GPU"},
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
+                       "p99": round(float(np.percentile(lat, 99)), 4),
+                       "per": "one wavefront = one lrh_trace of the bounce buffer"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_ray": round(b_ray, 1),
+                     "per_ray": {"node_visits": round(nodes / counted, 3),
+                                 "tri_tests": round(prims / counted, 3)},
+                     "kernel_ms": round(kernel_ms, 4)},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -174,6 +174,12 @@ int lrh_trace(lrh_scene *scene, const void *rays, void *hits, unsigned int count
 		unsigned int query, void *stream);
 int lrh_trace_ex(lrh_scene *scene, const void *rays, void *hits, unsigned int count,
 		unsigned int query, const lrh_trace_opts *opts, void *stream);
+/* Same trace through a counting variant of the kernel, synchronous:
+ * out[0] = inner-node visits, out[1] = primitive records tested (triangles,
+ * plus instance records in two-level scenes), out[2] = rays traced.  Used for
+ * the bytes-per-ray roofline accounting. */
+int lrh_trace_stats(lrh_scene *scene, const void *rays, void *hits, unsigned int count,
+		unsigned int query, const lrh_trace_opts *opts, void *stream, unsigned long long out[3]);
 /* Many ray buffers over one or more resident scenes of the same device in one
  * persistent launch; rays are handed out in chunks across all buffers. */
 int lrh_trace_batch(lrh_device *dev, const lrh_batch_item *items, unsigned int nItems,

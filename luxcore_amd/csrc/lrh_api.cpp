@@ -78,7 +78,8 @@ struct lrh_device {
 	// per-stream trace workspace: ray-ticket counter + spill stack, so that
 	// traces issued on different streams never share scratch memory
 	struct Workspace {
-		uint32_t *counter = nullptr;
+		uint32_t *counter = nullptr;      // [0] tickets, [1] finished blocks
+		unsigned long long *stats = nullptr;
 		uint2 *spill = nullptr;
 		size_t spillBytes = 0;
 	};
@@ -407,7 +408,8 @@ int lrh_device_close(lrh_device *dev) {
 	(void)hipDeviceSynchronize();
 	for (auto &w : dev->ws) {
 		dev->devFree(w.second.spill, w.second.spillBytes);
-		dev->devFree(w.second.counter, sizeof(uint32_t));
+		dev->devFree(w.second.counter, 2 * sizeof(uint32_t));
+		dev->devFree(w.second.stats, 4 * sizeof(unsigned long long));
 	}
 	dev->devFree(dev->dError, sizeof(uint32_t));
 	for (auto &a : dev->allocs) (void)hipFree((void *)a.first);
@@ -799,7 +801,11 @@ LaunchShape chooseShape(lrh_scene *s, unsigned count, bool any, const lrh_trace_
 lrh_device::Workspace &workspace(lrh_device *dev, hipStream_t st, size_t spillBytes) {
 	std::lock_guard<std::mutex> g(dev->mu);
 	lrh_device::Workspace &w = dev->ws[st];
-	if (!w.counter) w.counter = (uint32_t *)dev->devAlloc(sizeof(uint32_t));
+	if (!w.counter) {
+		w.counter = (uint32_t *)dev->devAlloc(2 * sizeof(uint32_t));
+		w.stats = (unsigned long long *)dev->devAlloc(4 * sizeof(unsigned long long));
+		HIPCHK(hipMemsetAsync(w.counter, 0, 2 * sizeof(uint32_t), st));
+	}
 	if (w.spillBytes < spillBytes) {
 		// the previous spill array may still be read by work queued on st
 		HIPCHK(hipStreamSynchronize(st));
@@ -816,8 +822,8 @@ lrh_device::Workspace &workspace(lrh_device *dev, hipStream_t st, size_t spillBy
 
 extern "C" {
 
-int lrh_trace_ex(lrh_scene *s, const void *rays, void *hits, unsigned int count, unsigned int query,
-		const lrh_trace_opts *opts, void *stream) {
+static int traceImpl(lrh_scene *s, const void *rays, void *hits, unsigned int count, unsigned int query,
+		const lrh_trace_opts *opts, void *stream, unsigned long long *statsOut) {
 	if (!s) return fail(LRH_EINVAL, "null scene");
 	if (count && (!rays || !hits)) return fail(LRH_EINVAL, "null ray or hit buffer");
 	if (query > 1) return fail(LRH_EINVAL, "query must be LRH_QUERY_CLOSEST or LRH_QUERY_ANY");
@@ -853,11 +859,28 @@ int lrh_trace_ex(lrh_scene *s, const void *rays, void *hits, unsigned int count,
 		P.ldsDepth = L.lds;
 		P.refill = L.refill;
 		P.error = dev->dError;
-		HIPCHK(lrhLaunchReset(w.counter, 1, st));
-		HIPCHK(lrhLaunchTrace(s->layout, s->twoLevel, any, dim3(L.blocks), dim3(L.block),
+		P.stats = w.stats;
+		if (statsOut) HIPCHK(hipMemsetAsync(w.stats, 0, 4 * sizeof(unsigned long long), st));
+		HIPCHK(lrhLaunchTrace(s->layout, s->twoLevel, any, statsOut != nullptr, dim3(L.blocks), dim3(L.block),
 				(size_t)L.lds * L.block * sizeof(uint2), st, P));
 		dev->rays += count;
+		if (statsOut) {
+			HIPCHK(hipMemcpyAsync(statsOut, w.stats, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+			HIPCHK(hipStreamSynchronize(st));
+		}
 	});
+}
+
+int lrh_trace_ex(lrh_scene *s, const void *rays, void *hits, unsigned int count, unsigned int query,
+		const lrh_trace_opts *opts, void *stream) {
+	return traceImpl(s, rays, hits, count, query, opts, stream, nullptr);
+}
+
+int lrh_trace_stats(lrh_scene *s, const void *rays, void *hits, unsigned int count, unsigned int query,
+		const lrh_trace_opts *opts, void *stream, unsigned long long out[3]) {
+	if (!out) return fail(LRH_EINVAL, "null stats output");
+	out[0] = out[1] = out[2] = 0;
+	return traceImpl(s, rays, hits, count, query, opts, stream, out);
 }
 
 int lrh_trace(lrh_scene *s, const void *rays, void *hits, unsigned int count, unsigned int query, void *stream) {

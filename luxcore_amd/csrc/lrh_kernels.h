@@ -19,18 +19,19 @@ struct LrhTraceParams {
 	lrh_hit_rec *hits;
 	uint32_t nRays;
 	uint32_t root;                  /* child reference of the (root) tree */
-	uint32_t *counter;              /* ray tickets, zeroed before the launch */
+	uint32_t *counter;              /* [0] ray tickets, [1] finished blocks; the last block rewinds both */
 	uint2 *spill;                   /* spill stack: spillDepth entries x nLanes */
 	uint32_t spillDepth;
 	uint32_t nLanes;
 	uint32_t ldsDepth;              /* power of two */
 	uint32_t refill;                /* refill a wave when fewer lanes than this are live */
 	uint32_t *error;                /* bit 0 stack overflow, bit 1 step guard */
+	unsigned long long *stats;      /* counting variant: node visits, prim tests, rays */
 };
 
 hipError_t lrhLaunchReset(uint32_t *p, uint32_t n, hipStream_t s);
 hipError_t lrhOccupancy(int layout, bool two, bool any, int block, size_t lds, int *blocksPerCU);
-hipError_t lrhLaunchTrace(int layout, bool two, bool any, dim3 grid, dim3 block, size_t lds, hipStream_t s,
-		const LrhTraceParams &P);
+hipError_t lrhLaunchTrace(int layout, bool two, bool any, bool count, dim3 grid, dim3 block, size_t lds,
+		hipStream_t s, const LrhTraceParams &P);
 
 #endif

@@ -265,7 +265,7 @@ constexpr uint32_t kMaxSteps = 1u << 24;  // per-ray guard against corrupt trees
 
 }  // namespace
 
-template <int LAYOUT, bool TWO, bool ANY>
+template <int LAYOUT, bool TWO, bool ANY, bool COUNT>
 __global__ __launch_bounds__(256) void lrh_trace_kernel(LrhTraceParams P) {
 	extern __shared__ uint2 ldsStack[];
 	const uint32_t lane = __lane_id();
@@ -289,6 +289,7 @@ __global__ __launch_bounds__(256) void lrh_trace_kernel(LrhTraceParams P) {
 	float curT = 0.f;
 	bool inInst = false;
 	bool exhausted = false;
+	uint32_t cNode = 0, cPrim = 0, cRay = 0;  // COUNT variant only
 
 	// ticket source: one contiguous range per item (a single item for plain traces)
 	for (;;) {
@@ -323,6 +324,7 @@ __global__ __launch_bounds__(256) void lrh_trace_kernel(LrhTraceParams P) {
 							inInst = false;
 							steps = 0;
 							st.sp = 0;
+							if (COUNT) ++cRay;
 						}
 					}
 				}
@@ -363,6 +365,7 @@ __global__ __launch_bounds__(256) void lrh_trace_kernel(LrhTraceParams P) {
 						const uint32_t start = cur & LRH_LEAF_START_MASK;
 						const uint32_t cnt = ((cur >> LRH_LEAF_START_BITS) & 7u) + 1u;
 						cur = LRH_CHILD_EMPTY;
+						if (COUNT) cPrim += cnt;
 						if (TWO && !inInst) {
 							// root leaf: enter the object's tree in its local space
 							const lrh_inst in = P.insts[start];
@@ -419,6 +422,7 @@ __global__ __launch_bounds__(256) void lrh_trace_kernel(LrhTraceParams P) {
 					} else {
 						float t4[4];
 						uint32_t c4[4];
+						if (COUNT) ++cNode;
 						if (LAYOUT == 0)
 							testNodeQ(reinterpret_cast<const lrh_node4q *>(P.nodes), cur, r, mint, best, t4, c4);
 						else
@@ -453,6 +457,29 @@ __global__ __launch_bounds__(256) void lrh_trace_kernel(LrhTraceParams P) {
 			}
 		} while ((uint32_t)__popcll(__ballot(rid != kNoRay)) >= minLive);
 	}
+	if (COUNT) {
+		unsigned long long a = cNode, b = cPrim, c = cRay;
+		for (int o = 32; o > 0; o >>= 1) {
+			a += __shfl_xor(a, o);
+			b += __shfl_xor(b, o);
+			c += __shfl_xor(c, o);
+		}
+		if (lane == 0) {
+			atomicAdd(P.stats + 0, a);
+			atomicAdd(P.stats + 1, b);
+			atomicAdd(P.stats + 2, c);
+		}
+	}
+	// the last block to finish rewinds the ticket counter for the next launch
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		__threadfence();
+		const uint32_t d = atomicAdd(P.counter + 1, 1u);
+		if (d == gridDim.x - 1) {
+			atomicExch(P.counter + 0, 0u);
+			atomicExch(P.counter + 1, 0u);
+		}
+	}
 }
 
 __global__ void lrh_reset_kernel(uint32_t *p, uint32_t n) {
@@ -465,13 +492,18 @@ __global__ void lrh_reset_kernel(uint32_t *p, uint32_t n) {
 // ---------------------------------------------------------------------------
 typedef void (*KernelFn)(LrhTraceParams);
 
-static const void *kernelPtr(int layout, bool two, bool any) {
-	static const void *tab[2][2][2] = {
-		{{(const void *)&lrh_trace_kernel<0, false, false>, (const void *)&lrh_trace_kernel<0, false, true>},
-		 {(const void *)&lrh_trace_kernel<0, true, false>, (const void *)&lrh_trace_kernel<0, true, true>}},
-		{{(const void *)&lrh_trace_kernel<1, false, false>, (const void *)&lrh_trace_kernel<1, false, true>},
-		 {(const void *)&lrh_trace_kernel<1, true, false>, (const void *)&lrh_trace_kernel<1, true, true>}}};
-	return tab[layout ? 1 : 0][two ? 1 : 0][any ? 1 : 0];
+#define LRH_K(L, T, A, C) (const void *)&lrh_trace_kernel<L, T, A, C>
+static const void *kernelPtr(int layout, bool two, bool any, bool count) {
+	static const void *tab[2][2][2][2] = {
+		{{{LRH_K(0, false, false, false), LRH_K(0, false, false, true)},
+		  {LRH_K(0, false, true, false), LRH_K(0, false, true, true)}},
+		 {{LRH_K(0, true, false, false), LRH_K(0, true, false, true)},
+		  {LRH_K(0, true, true, false), LRH_K(0, true, true, true)}}},
+		{{{LRH_K(1, false, false, false), LRH_K(1, false, false, true)},
+		  {LRH_K(1, false, true, false), LRH_K(1, false, true, true)}},
+		 {{LRH_K(1, true, false, false), LRH_K(1, true, false, true)},
+		  {LRH_K(1, true, true, false), LRH_K(1, true, true, true)}}}};
+	return tab[layout ? 1 : 0][two ? 1 : 0][any ? 1 : 0][count ? 1 : 0];
 }
 
 hipError_t lrhLaunchReset(uint32_t *p, uint32_t n, hipStream_t s) {
@@ -480,11 +512,11 @@ hipError_t lrhLaunchReset(uint32_t *p, uint32_t n, hipStream_t s) {
 }
 
 hipError_t lrhOccupancy(int layout, bool two, bool any, int block, size_t lds, int *blocksPerCU) {
-	return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kernelPtr(layout, two, any), block, lds);
+	return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kernelPtr(layout, two, any, false), block, lds);
 }
 
-hipError_t lrhLaunchTrace(int layout, bool two, bool any, dim3 grid, dim3 block, size_t lds, hipStream_t s,
-		const LrhTraceParams &P) {
+hipError_t lrhLaunchTrace(int layout, bool two, bool any, bool count, dim3 grid, dim3 block, size_t lds,
+		hipStream_t s, const LrhTraceParams &P) {
 	void *args[] = {(void *)&P};
-	return hipLaunchKernel(kernelPtr(layout, two, any), grid, block, args, lds, s);
+	return hipLaunchKernel(kernelPtr(layout, two, any, count), grid, block, args, lds, s);
 }

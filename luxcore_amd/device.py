@@ -97,6 +97,8 @@ SIGNATURES = {
     "lrh_trace": ([_vp, _vp, _vp, ctypes.c_uint, ctypes.c_uint, _vp], ctypes.c_int),
     "lrh_trace_ex": ([_vp, _vp, _vp, ctypes.c_uint, ctypes.c_uint, ctypes.POINTER(LrhTraceOpts), _vp],
                      ctypes.c_int),
+    "lrh_trace_stats": ([_vp, _vp, _vp, ctypes.c_uint, ctypes.c_uint, ctypes.POINTER(LrhTraceOpts), _vp,
+                         ctypes.POINTER(ctypes.c_ulonglong)], ctypes.c_int),
     "lrh_trace_batch": ([_vp, ctypes.POINTER(LrhBatchItem), ctypes.c_uint, ctypes.POINTER(LrhTraceOpts), _vp],
                         ctypes.c_int),
     "lrh_device_stats": ([_vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_double)],
@@ -376,6 +378,13 @@ class Device:
             check(lib().lrh_trace(scene.h, rays_buf.ptr, hits_buf.ptr, count, query, stream))
         else:
             check(lib().lrh_trace_ex(scene.h, rays_buf.ptr, hits_buf.ptr, count, query, ctypes.byref(opts), stream))
+
+    def trace_stats(self, scene, rays_buf, hits_buf, count, query=0, opts=None, stream=None):
+        """(node visits, primitive tests, rays) of one trace (counting kernel variant)."""
+        out = (ctypes.c_ulonglong * 3)()
+        check(lib().lrh_trace_stats(scene.h, rays_buf.ptr, hits_buf.ptr, count, query,
+                                    ctypes.byref(opts) if opts else None, stream, out))
+        return out[0], out[1], out[2]
 
     def trace_batch(self, items, opts=None, stream=None):
         arr = (LrhBatchItem * max(1, len(items)))()

@@ -243,8 +243,8 @@ def heightfield_scene(cells=3240, n_spheres=64, seed=3):
     return meshes
 
 
-def pinhole_rays(width=1920, height=1080, eye=(0.0, -75.0, 45.0), look=(0.0, 0.0, 0.0),
-                 up=(0.0, 0.0, 1.0), vfov_deg=40.0):
+def pinhole_rays(width=1920, height=1080, eye=(0.0, -30.0, 45.0), look=(0.0, 8.0, 0.0),
+                 up=(0.0, 0.0, 1.0), vfov_deg=36.0):
     """Pixel-centre primaries of a pinhole camera (coherent set)."""
     eye = np.asarray(eye, np.float64)
     fwd = np.asarray(look, np.float64) - eye
