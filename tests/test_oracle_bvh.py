@@ -1,0 +1,197 @@
+"""Oracle BVHAccel / MBVHAccel: tree invariants, brute force, topology
+independence, and that the parity checker catches planted mistakes (CPU)."""
+import numpy as np
+import pytest
+
+import oracle
+from luxcore_amd import scenes
+from luxcore_amd.raytypes import HIT_DTYPE, NULL_INDEX, make_rays
+from tests.parity import assert_parity, compare, tri_tie_checker
+
+
+def small_scene(seed=0, n=1500):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    soup = scenes.random_triangles(n, (0, 0, 0), (10, 10, 10), rng)
+    # shrink the soup triangles around their centroids so the tree matters
+    v = soup.verts.reshape(-1, 3, 3)
+    c = v.mean(axis=1, keepdims=True)
+    soup = scenes.Mesh((c + 0.08 * (v - c)).reshape(-1, 3).astype(np.float32), soup.tris)
+    return [scenes.box_mesh(), scenes.geodesic_sphere(12, 2.0, (5, 5, 5)), soup]
+
+
+def tri_boxes(meshes):
+    """Epsilon-expanded triangle boxes in DataSet order (bvhaccel.cpp:116-122)."""
+    out = []
+    for m in meshes:
+        p = m.verts[m.tris]
+        lo = p.min(axis=1)
+        hi = p.max(axis=1)
+        e = np.maximum(np.array([[oracle.epsilon(x) for x in row] for row in lo]).max(axis=1),
+                       np.array([[oracle.epsilon(x) for x in row] for row in hi]).max(axis=1))
+        out.append((lo - e[:, None], hi + e[:, None]))
+    return out
+
+
+@pytest.mark.parametrize("builder", ["CLASSIC", "SAH"])
+@pytest.mark.parametrize("tree_type", [2, 4, 8])
+def test_tree_invariants(builder, tree_type):
+    meshes = small_scene(1, 400)
+    ob = oracle.OracleBVH(meshes, builder=builder, tree_type=tree_type)
+    nodes = ob.nodes()
+    n = nodes.shape[0]
+    assert (nodes["nodeData"][0] & 0x7FFFFFFF) == n
+    leaf = (nodes["nodeData"] & 0x80000000) != 0
+    seen = set()
+    boxes = tri_boxes(meshes)
+    # every triangle exactly once; leaves skip to the next record
+    for i in np.nonzero(leaf)[0]:
+        a = nodes["a"][i]
+        key = (int(a[3]), int(a[4]))
+        assert key not in seen
+        seen.add(key)
+        assert (nodes["nodeData"][i] & 0x7FFFFFFF) == i + 1
+        m = meshes[key[0]]
+        assert list(a[:3]) == list(m.tris[key[1]])
+    assert len(seen) == sum(m.ntris for m in meshes)
+    # inner skip pointers point past the subtree; each inner box contains the
+    # boxes of every leaf in [i+1, skip)
+    for i in np.nonzero(~leaf)[0]:
+        skip = int(nodes["nodeData"][i])
+        assert i < skip <= n
+        bmin = nodes["a"][i][:3].view(np.float32)
+        bmax = nodes["a"][i][3:6].view(np.float32)
+        for j in range(i + 1, skip):
+            if leaf[j]:
+                mi, ti = int(nodes["a"][j][3]), int(nodes["a"][j][4])
+                lo, hi = boxes[mi][0][ti], boxes[mi][1][ti]
+                assert np.all(bmin <= lo) and np.all(bmax >= hi)
+
+
+@pytest.mark.parametrize("builder", ["CLASSIC", "SAH"])
+def test_bvh_equals_bruteforce(builder):
+    meshes = small_scene(2, 2000)
+    rng = np.random.Generator(np.random.PCG64(3))
+    rays = scenes.random_rays(20000, (0, 0, 0), (10, 10, 10), rng)
+    rays["maxt"][::7] = 2.0
+    ref = oracle.bruteforce(meshes, rays)
+    got = oracle.OracleBVH(meshes, builder=builder).intersect(rays)
+    st = compare(ref, got, tri_tie_checker(oracle, meshes, rays, ref, got))
+    assert st["mismatch"] == st["ties"] + st["edge_band"]
+    assert_parity(st, rel_tol=0.0)  # same arithmetic: bit-exact where indices agree
+
+
+def test_classic_vs_sah_topology_independent():
+    meshes = scenes.plain_scene(seed=1, n_random=3000)
+    rays = scenes.plain_rays(20000, seed=4)
+    a = oracle.OracleBVH(meshes, builder="CLASSIC").intersect(rays)
+    b = oracle.OracleBVH(meshes, builder="SAH").intersect(rays)
+    st = compare(a, b, tri_tie_checker(oracle, meshes, rays, a, b))
+    assert_parity(st, rel_tol=0.0)
+
+
+def test_masked_and_empty():
+    meshes = small_scene(5, 100)
+    rays = scenes.plain_rays(500, seed=7)
+    rays["flags"][::2] = 1
+    ob = oracle.OracleBVH(meshes)
+    hits = np.zeros(rays.shape[0], HIT_DTYPE)
+    hits["t"] = -3.0
+    ob.intersect(rays, hits=hits)
+    assert np.all(hits["t"][::2] == -3.0)
+    empty = oracle.OracleBVH([])
+    h = empty.intersect(rays)
+    assert np.all(h["meshIndex"][1::2] == NULL_INDEX)
+
+
+def test_checker_catches_planted_mistakes():
+    """Mistakes planted in a copy of oracle output must be flagged."""
+    meshes = small_scene(8, 800)
+    rays = scenes.random_rays(4000, (0, 0, 0), (10, 10, 10), np.random.Generator(np.random.PCG64(9)))
+    ref = oracle.OracleBVH(meshes).intersect(rays)
+    hit = np.nonzero(ref["meshIndex"] != NULL_INDEX)[0]
+    assert hit.size > 1000
+    # 1. wrong triangle index on one non-edge hit
+    bary = np.minimum(np.minimum(ref["b1"], ref["b2"]), 1 - ref["b1"] - ref["b2"])
+    i = hit[np.argmax(bary[hit])]
+    bad = ref.copy()
+    bad["triangleIndex"][i] ^= 1
+    st = compare(ref, bad, tri_tie_checker(oracle, meshes, rays, ref, bad))
+    assert st["bad"] == 1
+    # 2. t off by 2e-5 relative
+    bad = ref.copy()
+    bad["t"][hit[:10]] *= np.float32(1 + 2e-5)
+    with pytest.raises(AssertionError):
+        assert_parity(compare(ref, bad))
+    # 3. a miss reported as a hit
+    bad = ref.copy()
+    j = np.nonzero(ref["meshIndex"] == NULL_INDEX)[0]
+    if j.size:
+        bad["meshIndex"][j[0]] = 0
+        bad["triangleIndex"][j[0]] = 0
+        assert compare(ref, bad)["bad"] == 1
+    # 4. the sign of a barycentric flipped in a restated triangle test would
+    #    accept rays outside the triangle: emulate by swapping b1/b2 origins
+    m = scenes.Mesh(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), np.array([[0, 1, 2]], np.uint32))
+    r = make_rays([[0.75, 0.75, 1.0]], [[0, 0, -1]])
+    assert oracle.bruteforce([m], r)["meshIndex"][0] == NULL_INDEX
+
+
+# ---------------------------------------------------------------------------
+# two-level oracle
+# ---------------------------------------------------------------------------
+def _inst_scene(base, mats, kinds=None, keys=None):
+    n = len(mats)
+    kinds = np.ones(n, np.uint32) if kinds is None else np.asarray(kinds, np.uint32)
+    key_off = np.zeros(n + 1, np.uint32)
+    kt, km = [], []
+    for o in range(n):
+        if kinds[o] == 2:
+            ts, ms = keys[o]
+            kt += list(ts)
+            km += list(ms)
+            key_off[o + 1] = key_off[o] + len(ts)
+        else:
+            key_off[o + 1] = key_off[o]
+    return scenes.InstancedScene([base], np.zeros(n, np.uint32), kinds, np.asarray(mats, np.float32), key_off,
+                                 np.asarray(kt, np.float32), np.asarray(km, np.float32).reshape(-1, 4, 4))
+
+
+def test_mbvh_identity_instance_equals_bvh():
+    base = scenes.displaced_sphere(freq=10, seed=1)
+    sc = _inst_scene(base, [np.eye(4)])
+    rays = scenes.random_rays(5000, (-2, -2, -2), (2, 2, 2), np.random.Generator(np.random.PCG64(2)))
+    a = oracle.OracleMBVH(sc).intersect(rays)
+    b = oracle.OracleBVH([base]).intersect(rays)
+    st = compare(b, a, tri_tie_checker(oracle, [base], rays, b, a))
+    assert_parity(st, rel_tol=0.0)
+
+
+def test_mbvh_static_instances_equal_pretransformed_copies():
+    base = scenes.displaced_sphere(freq=8, seed=1)
+    rng = np.random.Generator(np.random.PCG64(3))
+    mats = [scenes.random_trs(rng, 6.0) for _ in range(12)]
+    sc = _inst_scene(base, mats)
+    copies = []
+    for M in mats:
+        v = np.c_[base.verts.astype(np.float64), np.ones(base.verts.shape[0])] @ M.astype(np.float64).T
+        copies.append(scenes.Mesh(v[:, :3].astype(np.float32), base.tris))
+    rays = scenes.random_rays(6000, (-8, -8, -8), (8, 8, 8), rng)
+    a = oracle.OracleMBVH(sc).intersect(rays)
+    b = oracle.OracleBVH(copies).intersect(rays)
+    st = compare(b, a, tri_tie_checker(oracle, copies, rays, b, a), rel_tol=1e-4)
+    # world-space copies vs local-space tests differ by rounding: edge band only
+    assert st["bad"] <= 2, st
+    assert st["hits"] > 500
+
+
+@pytest.mark.parametrize("builder", ["CLASSIC", "SAH"])
+def test_mbvh_equals_bruteforce_with_motion(builder):
+    sc = scenes.instanced_scene(n_inst=40, motion_frac=0.3, freq=8, extent=10.0, seed=5)
+    rays = scenes.instanced_rays(6000, extent=10.0, seed=6)
+    ob = oracle.OracleMBVH(sc, builder=builder)
+    got = ob.intersect(rays)
+    ref = ob.bruteforce(rays)
+    st = compare(ref, got)
+    assert st["mismatch"] == st["ties"] + st["edge_band"] and st["bad"] == 0, st
+    assert_parity(st, rel_tol=0.0)
+    assert st["hits"] > 300
