@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <set>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -88,6 +89,7 @@ struct lrh_device {
 	std::mutex mu;
 	std::map<size_t, size_t> allocs;  // lrh_alloc'd pointers -> bytes
 	std::map<uint64_t, int> occ;      // occupancy cache
+	std::set<lrh_scene *> scenes;     // live scenes (destroyed with the device)
 	std::atomic<unsigned long long> rays{0};
 	double t0 = 0;
 
@@ -406,6 +408,13 @@ int lrh_device_close(lrh_device *dev) {
 	(void)hipSetDevice(dev->ordinal);
 	if (dev->stream) (void)hipStreamSynchronize(dev->stream);
 	(void)hipDeviceSynchronize();
+	// scenes still alive are released with their device; their handles become invalid
+	std::set<lrh_scene *> live;
+	{
+		std::lock_guard<std::mutex> g(dev->mu);
+		live.swap(dev->scenes);
+	}
+	for (lrh_scene *s : live) delete s;
 	for (auto &w : dev->ws) {
 		dev->devFree(w.second.spill, w.second.spillBytes);
 		dev->devFree(w.second.counter, 2 * sizeof(uint32_t));
@@ -597,6 +606,10 @@ int lrh_scene_create_bvh(lrh_device *dev, const lrh_mesh *meshes, unsigned int n
 		finishInfo(s.get(), ht.count(s->layout), tris.size(), tris.size() * sizeof(lrh_tri), t1 - t0, now() - t1);
 	});
 	if (rc != LRH_OK) return rc;
+	if (dev) {
+		std::lock_guard<std::mutex> g(dev->mu);
+		dev->scenes.insert(s.get());
+	}
 	*out = s.release();
 	return LRH_OK;
 }
@@ -677,6 +690,10 @@ int lrh_scene_create_mbvh(lrh_device *dev, const lrh_mesh *meshes, unsigned int 
 				tris.size() * sizeof(lrh_tri) + instOut.size() * sizeof(lrh_inst), t1 - t0, now() - t1);
 	});
 	if (rc != LRH_OK) return rc;
+	if (dev) {
+		std::lock_guard<std::mutex> g(dev->mu);
+		dev->scenes.insert(s.get());
+	}
 	*out = s.release();
 	return LRH_OK;
 }
@@ -723,6 +740,8 @@ int lrh_scene_destroy(lrh_scene *s) {
 	if (s->dev) {
 		(void)hipSetDevice(s->dev->ordinal);
 		(void)hipStreamSynchronize(s->dev->stream);
+		std::lock_guard<std::mutex> g(s->dev->mu);
+		s->dev->scenes.erase(s);
 	}
 	delete s;
 	return LRH_OK;

@@ -326,10 +326,12 @@ void slerp(float t, const float q1[4], const float q2[4], float out[4]) {  // qu
 	cos_phi *= sign;
 	float f1, f2;
 	if (1.f - cos_phi > 1e-6f) {
-		const float phi = acosf(cos_phi);
-		const float sin_phi = sinf(phi);
-		f1 = sinf((1.f - t) * phi) / sin_phi;
-		f2 = sinf(t * phi) / sin_phi;
+		// correctly rounded fp32 acos/sin (double evaluation, one rounding),
+		// the same values the device computes
+		const float phi = (float)acos((double)cos_phi);
+		const float sin_phi = (float)sin((double)phi);
+		f1 = (float)sin((double)((1.f - t) * phi)) / sin_phi;
+		f2 = (float)sin((double)(t * phi)) / sin_phi;
 	} else {
 		f1 = 1.f - t;
 		f2 = t;

@@ -123,10 +123,12 @@ __device__ __noinline__ void motionSample(const lrh_motion *mot, const lrh_motio
 		cos_phi *= sign;
 		float f1, f2;
 		if (1.f - cos_phi > 1e-6f) {
-			const float phi = acosf(cos_phi);
-			const float sin_phi = sinf(phi);
-			f1 = sinf((1.f - le) * phi) / sin_phi;
-			f2 = sinf(le * phi) / sin_phi;
+			// correctly rounded fp32 acos/sin: evaluated in double, rounded once,
+			// so every platform gets the same bits
+			const float phi = (float)acos((double)cos_phi);
+			const float sin_phi = (float)sin((double)phi);
+			f1 = (float)sin((double)((1.f - le) * phi)) / sin_phi;
+			f2 = (float)sin((double)(le * phi)) / sin_phi;
 		} else {
 			f1 = 1.f - le;
 			f2 = le;

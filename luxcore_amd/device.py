@@ -7,6 +7,7 @@ missing or no GPU is present, the calls raise.
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -152,9 +153,9 @@ class DeviceBuffer:
         self.ptr = p.value
 
     def free(self):
-        if self.ptr:
+        if self.ptr and self.dev.h:
             check(lib().lrh_free(self.dev.h, self.ptr))
-            self.ptr = None
+        self.ptr = None
 
     def __del__(self):
         try:
@@ -248,6 +249,11 @@ class Scene:
         self.dev = dev
         self.h = handle
         self.two_level = two_level
+        if dev is not None:
+            dev._scenes.add(self)
+
+    def _alive(self):
+        return self.h is not None and (self.dev is None or self.dev.h is not None)
 
     def info(self):
         i = LrhSceneInfo()
@@ -266,13 +272,13 @@ class Scene:
         return nodes, prims
 
     def close(self):
-        if self.h:
+        if self._alive():
             check(lib().lrh_scene_destroy(self.h))
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
-            if self.h:
+            if self._alive():
                 lib().lrh_scene_destroy(self.h)
         except Exception:
             pass
@@ -302,6 +308,7 @@ class Device:
     """One GPU: buffers, streams, scenes and traces."""
 
     def __init__(self, ordinal=0):
+        self._scenes = weakref.WeakSet()
         p = ctypes.c_void_p()
         check(lib().lrh_device_open(ordinal, ctypes.byref(p)))
         self.h = p.value
@@ -314,7 +321,10 @@ class Device:
         self.total_mem = mem.value
 
     def close(self):
+        """Closes the device; scenes and buffers created on it are released."""
         if self.h:
+            for sc in list(self._scenes):
+                sc.h = None  # released by lrh_device_close
             check(lib().lrh_device_close(self.h))
             self.h = None
 

@@ -56,6 +56,8 @@ def lib():
                                         ctypes.POINTER(ctypes.c_ulonglong)]
         L.orc_bruteforce.argtypes = [ctypes.c_uint, _u32p, _u32p, _f32p, _u32p, _vp, _vp, ctypes.c_size_t,
                                      ctypes.c_int]
+        L.orc_set_libm.argtypes = [ctypes.c_int]
+        L.orc_set_libm.restype = None
         L.orc_matrix_inverse.argtypes = [_f32p, _f32p]
         L.orc_matrix_inverse.restype = ctypes.c_int
         L.orc_motion_sample.argtypes = [ctypes.c_uint, _f32p, _f32p, ctypes.c_float, _f32p]
@@ -111,6 +113,11 @@ def epsilon(v):
 def epsilon_bbox(pmin, pmax):
     a, b = _f3(pmin), _f3(pmax)
     return lib().orc_epsilon_bbox(_p(a, _f32p), _p(b, _f32p))
+
+
+def set_libm(correctly_rounded=True):
+    """Slerp's acosf/sinf: correctly rounded (default) or the host libm's."""
+    lib().orc_set_libm(1 if correctly_rounded else 0)
 
 
 def matrix_inverse(m):

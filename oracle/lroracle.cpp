@@ -37,6 +37,15 @@ template <class T> inline T Max(T a, T b) { return a > b ? a : b; }
 template <class T> inline T Clamp(T v, T lo, T hi) { return v > lo ? (v < hi ? v : hi) : lo; }
 inline float Lerp(float t, float v1, float v2) { return v1 + t * (v2 - v1); }
 
+// The reference's slerp calls the platform's acosf/sinf
+// (src/luxrays/core/geometry/quaternion.cpp:151-155), whose last bit is
+// libm-specific (glibc's are not correctly rounded).  The oracle pins them to
+// correctly rounded results (computed in double, then rounded once) unless
+// orc_set_libm(0) selects the host libm.
+int g_crMath = 1;
+inline float AcosF(float x) { return g_crMath ? (float)acos((double)x) : acosf(x); }
+inline float SinF(float x) { return g_crMath ? (float)sin((double)x) : sinf(x); }
+
 // ---------------------------------------------------------------------------
 // Point / Vector: include/luxrays/core/geometry/{point,vector}.h
 // ---------------------------------------------------------------------------
@@ -385,10 +394,10 @@ Quat Slerp(float t, const Quat &q1, const Quat &q2) {  // quaternion.cpp:143-164
 	cos_phi *= sign;
 	float f1, f2;
 	if (1.f - cos_phi > 1e-6f) {
-		const float phi = acosf(cos_phi);
-		const float sin_phi = sinf(phi);
-		f1 = sinf((1.f - t) * phi) / sin_phi;
-		f2 = sinf(t * phi) / sin_phi;
+		const float phi = AcosF(cos_phi);
+		const float sin_phi = SinF(phi);
+		f1 = SinF((1.f - t) * phi) / sin_phi;
+		f2 = SinF(t * phi) / sin_phi;
 	} else {
 		f1 = 1.f - t;
 		f2 = t;
@@ -1084,6 +1093,8 @@ int orc_bbox_intersectp(const orc_ray *r, const float pmin[3], const float pmax[
 }
 
 float orc_epsilon(float v) { return E(v); }
+
+void orc_set_libm(int correctlyRounded) { g_crMath = correctlyRounded ? 1 : 0; }
 
 float orc_epsilon_bbox(const float pmin[3], const float pmax[3]) {
 	BBox b;

@@ -1,0 +1,167 @@
+"""GPU parity of the two-level device (MBVHAccel path: instances and 2-key
+motion) against the oracle MBVHAccel::Intersect."""
+import numpy as np
+import pytest
+
+import oracle
+from luxcore_amd import scenes
+from luxcore_amd.raytypes import NULL_INDEX
+from tests.parity import assert_parity, compare
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(name, **kw):
+    print(f"[case] {name} {kw}", flush=True)
+
+
+def _check(ob, dev, sc_dev, rays, query=0, rel_tol=1e-5, motion=()):
+    ref = ob.intersect(rays)
+    got = dev.trace_numpy(sc_dev, rays, query=query)
+    if query == 1:
+        diff = np.nonzero((ref["meshIndex"] == NULL_INDEX) != (got["meshIndex"] == NULL_INDEX))[0]
+        # a flip is only allowed on motion objects within the motion band
+        for i in diff:
+            h = ref if ref["meshIndex"][i] != NULL_INDEX else got
+            assert int(h["meshIndex"][i]) in set(int(x) for x in motion), i
+        assert diff.size <= 1e-5 * rays.shape[0] + 5
+        return None
+    st = compare(ref, got, motion_objects=motion, mint=rays["mint"])
+    print(st, flush=True)
+    assert_parity(st, rel_tol=rel_tol)
+    return st
+
+
+@pytest.fixture(scope="module")
+def full_instanced():
+    sc = scenes.instanced_scene()
+    rays = scenes.instanced_rays(1 << 22)
+    return sc, rays
+
+
+def test_instanced_motion_full_size(gpu_device, full_instanced):
+    """1,000 instances of a 50,000-tri displaced sphere, 200 with 2-key
+    motion, 4,194,304 incoherent rays with time ~ U[0,1].  Slerp's acos/sin
+    are correctly rounded on both sides, so motion hits are held to the same
+    bar as static ones."""
+    _case("instanced_full", rays=1 << 22)
+    sc, rays = full_instanced
+    oracle.set_libm(True)
+    ob = oracle.OracleMBVH(sc)
+    dev_sc = gpu_device.scene_mbvh(sc)
+    st = _check(ob, gpu_device, dev_sc, rays)
+    assert st["hits"] > 100000
+    assert st["edge_band"] <= 1e-5 * rays.shape[0]
+    _check(ob, gpu_device, dev_sc, rays[: 1 << 20], query=1)
+
+
+def test_instanced_motion_against_host_libm(gpu_device, full_instanced):
+    """Same workload with the oracle's slerp on the host libm's acosf/sinf
+    (not correctly rounded): the differences stay inside the motion band."""
+    _case("instanced_full_host_libm", rays=1 << 22)
+    sc, rays = full_instanced
+    oracle.set_libm(False)
+    try:
+        ob = oracle.OracleMBVH(sc)
+        dev_sc = gpu_device.scene_mbvh(sc)
+        st = _check(ob, gpu_device, dev_sc, rays, motion=np.nonzero(sc.obj_kind == 2)[0])
+    finally:
+        oracle.set_libm(True)
+    assert st["motion_band"] + st["edge_band"] <= 2e-5 * rays.shape[0]
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_dense_instances_and_motion(gpu_device, layout):
+    _case("dense", layout=layout)
+    sc = scenes.instanced_scene(n_inst=60, motion_frac=0.5, freq=20, extent=8.0, seed=21)
+    rays = scenes.instanced_rays(200000, extent=8.0, seed=22)
+    ob = oracle.OracleMBVH(sc)
+    st = _check(ob, gpu_device, gpu_device.scene_mbvh(sc, layout=layout), rays)
+    assert st["hits"] > 50000
+
+
+def test_plain_object_and_shared_mesh(gpu_device):
+    """kind 0 (the base mesh itself) next to instances of the same mesh and of
+    a second mesh; meshIndex is the object index."""
+    _case("mixed_kinds")
+    base0 = scenes.displaced_sphere(freq=12, seed=1)
+    base1 = scenes.geodesic_sphere(10, 1.0)
+    rng = np.random.Generator(np.random.PCG64(5))
+    n = 25
+    obj_mesh = np.array([0] + [i % 2 for i in range(1, n)], np.uint32)
+    obj_kind = np.array([0] + [1] * (n - 1), np.uint32)
+    mats = np.stack([np.eye(4, dtype=np.float32)] + [scenes.random_trs(rng, 6.0) for _ in range(n - 1)])
+    sc = scenes.InstancedScene([base0, base1], obj_mesh, obj_kind, mats, np.zeros(n + 1, np.uint32),
+                               np.zeros(0, np.float32), np.zeros((0, 4, 4), np.float32))
+    rays = scenes.random_rays(100000, (-7, -7, -7), (7, 7, 7), rng)
+    ob = oracle.OracleMBVH(sc)
+    st = _check(ob, gpu_device, gpu_device.scene_mbvh(sc), rays)
+    assert st["hits"] > 10000
+
+
+def test_update_moves_instances(gpu_device):
+    """MBVHAccel::Update: new matrices for the same objects; results follow."""
+    _case("update")
+    sc = scenes.instanced_scene(n_inst=40, motion_frac=0.25, freq=10, extent=8.0, seed=31)
+    rays = scenes.instanced_rays(100000, extent=8.0, seed=32)
+    dev_sc = gpu_device.scene_mbvh(sc)
+    _check(oracle.OracleMBVH(sc), gpu_device, dev_sc, rays)
+    sc2 = scenes.instanced_scene(n_inst=40, motion_frac=0.25, freq=10, extent=8.0, seed=33)
+    sc2.obj_kind[:] = sc.obj_kind
+    # keep the kinds of the first scene; regenerate transforms/keys where needed
+    rng = np.random.Generator(np.random.PCG64(34))
+    key_off = np.zeros(41, np.uint32)
+    kt, km = [], []
+    for o in range(40):
+        M = scenes.random_trs(rng, 8.0)
+        sc2.obj_mat[o] = M
+        if sc.obj_kind[o] == 2:
+            M1 = M.copy()
+            M1[:3, 3] += 1.0
+            kt += [0.0, 1.0]
+            km += [np.linalg.inv(M.astype(np.float64)), np.linalg.inv(M1.astype(np.float64))]
+            key_off[o + 1] = key_off[o] + 2
+        else:
+            key_off[o + 1] = key_off[o]
+    sc2.key_off = key_off
+    sc2.key_times = np.asarray(kt, np.float32)
+    sc2.key_mats = np.asarray(km, np.float32).reshape(-1, 4, 4)
+    sc2.meshes = sc.meshes
+    dev_sc.update(sc2)
+    _check(oracle.OracleMBVH(sc2), gpu_device, dev_sc, rays)
+
+
+def test_translation_and_scale_motion_bit_exact(gpu_device):
+    """Motion without rotation samples with lerps only (no acosf/sinf), so
+    the device must match the oracle exactly, with no motion band."""
+    _case("motion_no_rotation")
+    base = scenes.displaced_sphere(freq=16, seed=3)
+    rng = np.random.Generator(np.random.PCG64(41))
+    n = 50
+    key_off = np.arange(0, 2 * n + 1, 2, dtype=np.uint32)
+    kt, km = [], []
+    for o in range(n):
+        M0 = scenes.random_trs(rng, 6.0)
+        M1 = M0.copy()
+        M1[:3, 3] += rng.uniform(-1, 1, 3).astype(np.float32)
+        if o % 2:
+            M1[:3, :3] *= np.float32(1.25)
+        kt += [0.0, 1.0]
+        km += [np.linalg.inv(M0.astype(np.float64)), np.linalg.inv(M1.astype(np.float64))]
+    sc = scenes.InstancedScene([base], np.zeros(n, np.uint32), np.full(n, 2, np.uint32),
+                               np.zeros((n, 4, 4), np.float32), key_off, np.asarray(kt, np.float32),
+                               np.asarray(km, np.float32).reshape(-1, 4, 4))
+    rays = scenes.instanced_rays(200000, extent=7.0, seed=42)
+    ob = oracle.OracleMBVH(sc)
+    st = _check(ob, gpu_device, gpu_device.scene_mbvh(sc), rays)
+    assert st["hits"] > 20000 and st["mismatch"] == st["ties"] + st["edge_band"]
+
+
+def test_empty_two_level(gpu_device):
+    _case("empty_mbvh")
+    sc = scenes.InstancedScene([scenes.geodesic_sphere(4)], np.zeros(0, np.uint32), np.zeros(0, np.uint32),
+                               np.zeros((0, 4, 4), np.float32), np.zeros(1, np.uint32), np.zeros(0, np.float32),
+                               np.zeros((0, 4, 4), np.float32))
+    rays = scenes.instanced_rays(1000, extent=2.0, seed=1)
+    got = gpu_device.trace_numpy(gpu_device.scene_mbvh(sc), rays)
+    assert np.all(got["meshIndex"] == NULL_INDEX) and np.array_equal(got["t"], rays["maxt"])
