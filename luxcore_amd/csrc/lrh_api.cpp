@@ -790,9 +790,9 @@ LaunchShape chooseShape(lrh_scene *s, unsigned count, bool any, const lrh_trace_
 	LaunchShape L;
 	L.block = (o && o->blockSize) ? o->blockSize : 256;
 	if (L.block != 64 && L.block != 128 && L.block != 256) throw Error(LRH_EINVAL, "blockSize must be 64, 128 or 256");
-	L.lds = (o && o->ldsStack) ? o->ldsStack : 16;
+	L.lds = (o && o->ldsStack) ? o->ldsStack : 8;
 	if (L.lds < 4 || L.lds > 64 || pow2floor(L.lds) != L.lds) throw Error(LRH_EINVAL, "ldsStack must be a power of two in 4..64");
-	L.refill = (o && o->refillThreshold) ? std::min(64u, o->refillThreshold) : 40;
+	L.refill = (o && o->refillThreshold) ? std::min(64u, o->refillThreshold) : 12;
 	const size_t ldsBytes = (size_t)L.lds * L.block * sizeof(uint2);
 	int perCU = (o && o->blocksPerCU) ? (int)o->blocksPerCU : 0;
 	if (!perCU) {
