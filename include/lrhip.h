@@ -60,6 +60,7 @@ extern "C" {
 typedef struct lrh_device lrh_device;
 typedef struct lrh_scene lrh_scene;
 typedef struct lrh_event lrh_event;
+typedef struct lrh_batch lrh_batch;
 
 /* One triangle mesh (TriangleMesh, include/luxrays/core/trianglemesh.h:79-134):
  * verts = 3*nVerts floats, tris = 3*nTris vertex indices local to the mesh. */
@@ -181,9 +182,24 @@ int lrh_trace_ex(lrh_scene *scene, const void *rays, void *hits, unsigned int co
 int lrh_trace_stats(lrh_scene *scene, const void *rays, void *hits, unsigned int count,
 		unsigned int query, const lrh_trace_opts *opts, void *stream, unsigned long long out[3]);
 /* Many ray buffers over one or more resident scenes of the same device in one
- * persistent launch; rays are handed out in chunks across all buffers. */
+ * persistent launch: the rays of all items form one ticket space, every lane
+ * takes the next ticket, finds its item and traces in that item's scene with
+ * that item's query.  All scenes of a batch share one node layout; at most
+ * 4,096 items and 2^31-1 rays.  (No reference counterpart: the reference
+ * launches one kernel per buffer, bvhaccelhw.cpp:259-268.) */
 int lrh_trace_batch(lrh_device *dev, const lrh_batch_item *items, unsigned int nItems,
 		const lrh_trace_opts *opts, void *stream);
+/* A resident batch: descriptor table and scratch uploaded once, then launched
+ * many times, directly or as a captured hipGraph (one kernel node).  Replays
+ * of one batch must not overlap; the kernel rewinds its own ticket counter, so
+ * the graph holds no memset node. */
+int lrh_batch_create(lrh_device *dev, const lrh_batch_item *items, unsigned int nItems,
+		const lrh_trace_opts *opts, lrh_batch **out);
+int lrh_batch_launch(lrh_batch *batch, void *stream);
+int lrh_batch_capture(lrh_batch *batch, void *stream);
+int lrh_batch_replay(lrh_batch *batch, void *stream);
+int lrh_batch_info(lrh_batch *batch, unsigned long long *rays, unsigned int *blocks, unsigned int *blockSize);
+int lrh_batch_destroy(lrh_batch *batch);
 /* Data-parallel ray count traced so far (IntersectionDevice stats). */
 int lrh_device_stats(lrh_device *dev, unsigned long long *rays, double *seconds);
 

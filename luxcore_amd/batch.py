@@ -1,0 +1,134 @@
+"""Batched tracing: many ray buffers over several resident scenes in one
+persistent launch (optionally a captured hipGraph), and the launch-shape
+autotuner.
+
+The autotuner times candidate launch shapes (block size x LDS stack depth x
+refill threshold) of the batched kernel on the actual batch and keeps the
+fastest; results are cached on disk per (device, workload signature) so a
+serving process tunes once.  Node layout and leaf size are build-time choices:
+``tune_build`` rebuilds the scenes for each candidate and times the best
+launch shape on each.
+"""
+import hashlib
+import itertools
+import json
+import os
+import time
+
+import numpy as np
+
+from . import device as D
+
+DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "lrhip", "autotune.json")
+
+SHAPES = [(b, l, r) for b, l, r in itertools.product((128, 256), (4, 8, 16), (4, 8, 12, 16, 24))]
+
+
+def signature(dev, scenes_info, counts, extra=""):
+    h = hashlib.sha1()
+    h.update(dev.name.encode())
+    h.update(json.dumps(scenes_info, sort_keys=True).encode())
+    h.update(np.asarray(counts, np.int64).tobytes())
+    h.update(extra.encode())
+    return h.hexdigest()[:16]
+
+
+def _load_cache(path):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store_cache(path, cache):
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(cache, f, indent=1)
+    os.replace(tmp, path)
+
+
+def time_batch(dev, items, opts, reps=5):
+    """Median ms of one batched launch (HIP events on the device stream)."""
+    b = dev.batch(items, opts)
+    for _ in range(2):
+        b.launch()
+    dev.sync()
+    evs = [(dev.event(), dev.event()) for _ in range(reps)]
+    for e0, e1 in evs:
+        e0.record()
+        b.launch()
+        e1.record()
+    dev.sync()
+    ms = float(np.median([e0.elapsed_ms(e1) for e0, e1 in evs]))
+    b.close()
+    return ms
+
+
+def autotune(dev, items, shapes=SHAPES, reps=3, cache_path=DEFAULT_CACHE, key=None, log=None):
+    """Best lrh_trace_opts for this batch; (opts, table of results)."""
+    cache = _load_cache(cache_path) if cache_path else {}
+    if key and key in cache:
+        c = cache[key]
+        return D.trace_opts(c["block"], c["lds"], 0, c["refill"]), c.get("table", [])
+    table = []
+    for block, lds, refill in shapes:
+        opts = D.trace_opts(block, lds, 0, refill)
+        try:
+            ms = time_batch(dev, items, opts, reps)
+        except D.LrhError as e:  # a shape the device refuses (LDS limit) is skipped
+            if log:
+                log(f"autotune: block {block} lds {lds} refill {refill}: {e}")
+            continue
+        table.append({"block": block, "lds": lds, "refill": refill, "ms": ms})
+        if log:
+            log(f"autotune: block {block} lds {lds} refill {refill}: {ms:.3f} ms")
+    best = min(table, key=lambda r: r["ms"])
+    if key and cache_path:
+        cache[key] = dict(best, table=table, when=time.time())
+        _store_cache(cache_path, cache)
+    return D.trace_opts(best["block"], best["lds"], 0, best["refill"]), table
+
+
+def tune_build(dev, scene_meshes, make_items, builds=((0, 4), (0, 2), (1, 4)), shapes=None, reps=3, log=None):
+    """Times each (layout, max_leaf) build with its best launch shape;
+    make_items(scenes) -> batch items.  Returns (best build, results)."""
+    shapes = shapes or [(256, 8, 8), (256, 8, 12), (256, 4, 12), (128, 8, 12)]
+    res = []
+    for layout, leaf in builds:
+        sc = [dev.scene_bvh(m, layout=layout, max_leaf=leaf) for m in scene_meshes]
+        items = make_items(sc)
+        opts, table = autotune(dev, items, shapes=shapes, reps=reps, cache_path=None)
+        best = min(table, key=lambda r: r["ms"])
+        res.append({"layout": layout, "max_leaf": leaf, **best})
+        if log:
+            log(f"build layout {layout} leaf {leaf}: {best}")
+        for s in sc:
+            s.close()
+    return min(res, key=lambda r: r["ms"]), res
+
+
+class BatchedWorkload:
+    """Resident scenes + ray buffers of a batched run (device memory)."""
+
+    def __init__(self, dev, scene_meshes, buffers, extent=100.0, seed=300, layout=0, max_leaf=4):
+        from . import scenes as S
+        self.dev = dev
+        self.scenes = [dev.scene_bvh(m, layout=layout, max_leaf=max_leaf) for m in scene_meshes]
+        self.buffers = buffers
+        self.rays = []
+        self.ray_bufs = []
+        self.hit_bufs = []
+        for i, (si, count, query) in enumerate(buffers):
+            r = S.buffer_rays(count, extent, seed + i)
+            self.rays.append(r)
+            self.ray_bufs.append(dev.upload(r))
+            self.hit_bufs.append(dev.alloc(count * 20))
+
+    def items(self):
+        return [(self.scenes[si], self.ray_bufs[i], self.hit_bufs[i], count, query)
+                for i, (si, count, query) in enumerate(self.buffers)]
+
+    def total_rays(self):
+        return sum(c for _, c, _ in self.buffers)

@@ -83,6 +83,8 @@ struct lrh_device {
 		unsigned long long *stats = nullptr;
 		uint2 *spill = nullptr;
 		size_t spillBytes = 0;
+		uint8_t *table = nullptr;         // batched-launch descriptor table
+		size_t tableBytes = 0;
 	};
 	std::map<hipStream_t, Workspace> ws;
 	std::atomic<size_t> used{0};
@@ -419,6 +421,7 @@ int lrh_device_close(lrh_device *dev) {
 		dev->devFree(w.second.spill, w.second.spillBytes);
 		dev->devFree(w.second.counter, 2 * sizeof(uint32_t));
 		dev->devFree(w.second.stats, 4 * sizeof(unsigned long long));
+		dev->devFree(w.second.table, w.second.tableBytes);
 	}
 	dev->devFree(dev->dError, sizeof(uint32_t));
 	for (auto &a : dev->allocs) (void)hipFree((void *)a.first);
@@ -777,6 +780,7 @@ namespace {
 
 struct LaunchShape {
 	unsigned block, lds, blocks, refill, spillDepth;
+	size_t ldsBytes;
 };
 
 unsigned pow2floor(unsigned v) {
@@ -785,34 +789,38 @@ unsigned pow2floor(unsigned v) {
 	return p;
 }
 
-LaunchShape chooseShape(lrh_scene *s, unsigned count, bool any, const lrh_trace_opts *o) {
-	lrh_device *dev = s->dev;
+// launch shape of one persistent trace: block size, LDS stack depth, grid
+// from the occupancy of the kernel variant, and the spill depth the tree
+// depth needs (3 pushes per level plus the object sentinel)
+LaunchShape chooseShape(lrh_device *dev, unsigned layout, bool two, bool any, bool batch, unsigned count,
+		unsigned maxDepth, size_t extraLds, const lrh_trace_opts *o) {
 	LaunchShape L;
 	L.block = (o && o->blockSize) ? o->blockSize : 256;
 	if (L.block != 64 && L.block != 128 && L.block != 256) throw Error(LRH_EINVAL, "blockSize must be 64, 128 or 256");
 	L.lds = (o && o->ldsStack) ? o->ldsStack : 8;
-	if (L.lds < 4 || L.lds > 64 || pow2floor(L.lds) != L.lds) throw Error(LRH_EINVAL, "ldsStack must be a power of two in 4..64");
+	if (L.lds < 4 || L.lds > 64 || pow2floor(L.lds) != L.lds)
+		throw Error(LRH_EINVAL, "ldsStack must be a power of two in 4..64");
 	L.refill = (o && o->refillThreshold) ? std::min(64u, o->refillThreshold) : 12;
-	const size_t ldsBytes = (size_t)L.lds * L.block * sizeof(uint2);
+	L.ldsBytes = (size_t)L.lds * L.block * sizeof(uint2) + extraLds;
+	if (L.ldsBytes > 65536) throw Error(LRH_ERANGE, "LDS stack and item table exceed 64 KiB per block");
 	int perCU = (o && o->blocksPerCU) ? (int)o->blocksPerCU : 0;
 	if (!perCU) {
-		const uint64_t key = ((uint64_t)s->layout << 40) | ((uint64_t)s->twoLevel << 39) | ((uint64_t)any << 38) |
-				((uint64_t)L.block << 16) | L.lds;
+		const uint64_t key = ((uint64_t)layout << 40) | ((uint64_t)two << 39) | ((uint64_t)any << 38) |
+				((uint64_t)batch << 37) | ((uint64_t)L.ldsBytes << 8) | (L.block >> 6);
 		std::lock_guard<std::mutex> g(dev->mu);
 		auto it = dev->occ.find(key);
 		if (it == dev->occ.end()) {
 			int nb = 0;
-			HIPCHK(lrhOccupancy(s->layout, s->twoLevel, any, L.block, ldsBytes, &nb));
+			HIPCHK(lrhOccupancy(layout, two, any, batch, L.block, L.ldsBytes, &nb));
 			dev->occ[key] = nb > 0 ? nb : 1;
 			perCU = nb > 0 ? nb : 1;
 		} else
 			perCU = it->second;
 	}
 	const unsigned persistent = (unsigned)perCU * (unsigned)dev->cuCount;
-	const unsigned need = (count + L.block - 1) / L.block;
+	const unsigned need = (unsigned)(((uint64_t)count + L.block - 1) / L.block);
 	L.blocks = std::max(1u, std::min(persistent, need));
-	// deepest stack: 3 pushes per level plus the object sentinel
-	const unsigned worst = 3u * (s->maxDepth + 2u) + 4u;
+	const unsigned worst = 3u * (maxDepth + 2u) + 4u;
 	L.spillDepth = worst > L.lds ? worst - L.lds : 1u;
 	return L;
 }
@@ -837,11 +845,18 @@ lrh_device::Workspace &workspace(lrh_device *dev, hipStream_t st, size_t spillBy
 	return w;
 }
 
-}  // namespace
+void fillSceneParams(LrhTraceParams &P, const lrh_scene *s) {
+	P.nodes = s->nodes.p;
+	P.tris = (const lrh_tri *)s->tris.p;
+	P.insts = (const lrh_inst *)s->insts.p;
+	P.xforms = (const float *)s->xforms.p;
+	P.motions = (const lrh_motion *)s->motions.p;
+	P.segs = (const lrh_motion_seg *)s->segs.p;
+	P.times = (const float *)s->times.p;
+	P.root = s->root;
+}
 
-extern "C" {
-
-static int traceImpl(lrh_scene *s, const void *rays, void *hits, unsigned int count, unsigned int query,
+int traceImpl(lrh_scene *s, const void *rays, void *hits, unsigned int count, unsigned int query,
 		const lrh_trace_opts *opts, void *stream, unsigned long long *statsOut) {
 	if (!s) return fail(LRH_EINVAL, "null scene");
 	if (count && (!rays || !hits)) return fail(LRH_EINVAL, "null ray or hit buffer");
@@ -856,21 +871,15 @@ static int traceImpl(lrh_scene *s, const void *rays, void *hits, unsigned int co
 		dev->activate();
 		hipStream_t st = stream ? (hipStream_t)stream : dev->stream;
 		const bool any = query == LRH_QUERY_ANY;
-		const LaunchShape L = chooseShape(s, count, any, opts);
+		const LaunchShape L = chooseShape(dev, s->layout, s->twoLevel, any, false, count, s->maxDepth, 0, opts);
 		const unsigned nLanes = L.blocks * L.block;
 		lrh_device::Workspace &w = workspace(dev, st, (size_t)nLanes * L.spillDepth * sizeof(uint2));
 		LrhTraceParams P;
-		P.nodes = s->nodes.p;
-		P.tris = (const lrh_tri *)s->tris.p;
-		P.insts = (const lrh_inst *)s->insts.p;
-		P.xforms = (const float *)s->xforms.p;
-		P.motions = (const lrh_motion *)s->motions.p;
-		P.segs = (const lrh_motion_seg *)s->segs.p;
-		P.times = (const float *)s->times.p;
+		std::memset(&P, 0, sizeof(P));
+		fillSceneParams(P, s);
 		P.rays = (const lrh_ray_rec *)rays;
 		P.hits = (lrh_hit_rec *)hits;
 		P.nRays = count;
-		P.root = s->root;
 		P.counter = w.counter;
 		P.spill = w.spill;
 		P.spillDepth = L.spillDepth;
@@ -880,8 +889,8 @@ static int traceImpl(lrh_scene *s, const void *rays, void *hits, unsigned int co
 		P.error = dev->dError;
 		P.stats = w.stats;
 		if (statsOut) HIPCHK(hipMemsetAsync(w.stats, 0, 4 * sizeof(unsigned long long), st));
-		HIPCHK(lrhLaunchTrace(s->layout, s->twoLevel, any, statsOut != nullptr, dim3(L.blocks), dim3(L.block),
-				(size_t)L.lds * L.block * sizeof(uint2), st, P));
+		HIPCHK(lrhLaunchTrace(s->layout, s->twoLevel, any, statsOut != nullptr, false, dim3(L.blocks), dim3(L.block),
+				L.ldsBytes, st, P));
 		dev->rays += count;
 		if (statsOut) {
 			HIPCHK(hipMemcpyAsync(statsOut, w.stats, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
@@ -890,9 +899,77 @@ static int traceImpl(lrh_scene *s, const void *rays, void *hits, unsigned int co
 	});
 }
 
+// descriptor table of a batched launch, validated against its device
+struct BatchTable {
+	std::vector<LrhBatchDesc> descs;
+	std::vector<uint32_t> starts;
+	unsigned layout = 0;
+	bool two = false;
+	unsigned maxDepth = 0;
+	uint64_t total = 0;
+};
+
+BatchTable makeTable(lrh_device *dev, const lrh_batch_item *items, unsigned n) {
+	BatchTable T;
+	if (n == 0) throw Error(LRH_EINVAL, "empty batch");
+	if (n > 4096) throw Error(LRH_ERANGE, "at most 4096 items per batch");
+	T.descs.resize(n);
+	T.starts.resize(n + 1);
+	for (unsigned i = 0; i < n; ++i) {
+		const lrh_batch_item &it = items[i];
+		const lrh_scene *s = it.scene;
+		if (!s || s->dev != dev) throw Error(LRH_EINVAL, "batch item " + std::to_string(i) + ": scene of another device");
+		if (it.count && (!it.rays || !it.hits)) throw Error(LRH_EINVAL, "batch item " + std::to_string(i) + ": null buffer");
+		if (((uintptr_t)it.rays & 15u) || ((uintptr_t)it.hits & 3u))
+			throw Error(LRH_EINVAL, "batch item " + std::to_string(i) + ": misaligned buffer");
+		if (it.query > 1) throw Error(LRH_EINVAL, "batch item " + std::to_string(i) + ": bad query");
+		if (i == 0) T.layout = s->layout;
+		else if (s->layout != T.layout) throw Error(LRH_EINVAL, "all scenes of a batch must share the node layout");
+		T.two = T.two || s->twoLevel;
+		T.maxDepth = std::max(T.maxDepth, s->maxDepth);
+		LrhBatchDesc &d = T.descs[i];
+		std::memset(&d, 0, sizeof(d));
+		d.nodes = s->nodes.p;
+		d.tris = (const lrh_tri *)s->tris.p;
+		d.insts = (const lrh_inst *)s->insts.p;
+		d.xforms = (const float *)s->xforms.p;
+		d.motions = (const lrh_motion *)s->motions.p;
+		d.segs = (const lrh_motion_seg *)s->segs.p;
+		d.times = (const float *)s->times.p;
+		d.rays = (const lrh_ray_rec *)it.rays;
+		d.hits = (lrh_hit_rec *)it.hits;
+		d.root = s->root;
+		d.count = it.count;
+		d.flags = (it.query == LRH_QUERY_ANY ? LRH_ITEM_ANY : 0u) | (s->twoLevel ? LRH_ITEM_TWO : 0u);
+		T.starts[i] = (uint32_t)T.total;
+		T.total += it.count;
+		if (T.total > 0x7fffffffull) throw Error(LRH_ERANGE, "at most 2^31-1 rays per batch");
+	}
+	T.starts[n] = (uint32_t)T.total;
+	return T;
+}
+
+}  // namespace
+
+struct lrh_batch {
+	lrh_device *dev = nullptr;
+	BatchTable T;
+	LaunchShape L{};
+	DevBuf descs, starts, counter, spill;
+	LrhTraceParams P{};
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
 int lrh_trace_ex(lrh_scene *s, const void *rays, void *hits, unsigned int count, unsigned int query,
 		const lrh_trace_opts *opts, void *stream) {
 	return traceImpl(s, rays, hits, count, query, opts, stream, nullptr);
+}
+
+int lrh_trace(lrh_scene *s, const void *rays, void *hits, unsigned int count, unsigned int query, void *stream) {
+	return traceImpl(s, rays, hits, count, query, nullptr, stream, nullptr);
 }
 
 int lrh_trace_stats(lrh_scene *s, const void *rays, void *hits, unsigned int count, unsigned int query,
@@ -902,20 +979,155 @@ int lrh_trace_stats(lrh_scene *s, const void *rays, void *hits, unsigned int cou
 	return traceImpl(s, rays, hits, count, query, opts, stream, out);
 }
 
-int lrh_trace(lrh_scene *s, const void *rays, void *hits, unsigned int count, unsigned int query, void *stream) {
-	return lrh_trace_ex(s, rays, hits, count, query, nullptr, stream);
+int lrh_batch_create(lrh_device *dev, const lrh_batch_item *items, unsigned int nItems, const lrh_trace_opts *opts,
+		lrh_batch **out) {
+	if (!dev || !out || (nItems && !items)) return fail(LRH_EINVAL, "null argument");
+	*out = nullptr;
+	std::unique_ptr<lrh_batch> b(new lrh_batch());
+	const int rc = guarded([&]() {
+		dev->activate();
+		b->dev = dev;
+		b->T = makeTable(dev, items, nItems);
+		const size_t extra = ((nItems + 1) * sizeof(uint32_t) + 15) & ~(size_t)15;
+		b->L = chooseShape(dev, b->T.layout, b->T.two, false, true, (unsigned)std::max<uint64_t>(b->T.total, 1),
+				b->T.maxDepth, extra, opts);
+		const unsigned nLanes = b->L.blocks * b->L.block;
+		b->descs.alloc(dev, b->T.descs.size() * sizeof(LrhBatchDesc));
+		b->starts.alloc(dev, b->T.starts.size() * sizeof(uint32_t));
+		b->counter.alloc(dev, 2 * sizeof(uint32_t));
+		b->spill.alloc(dev, (size_t)nLanes * b->L.spillDepth * sizeof(uint2));
+		b->descs.upload(b->T.descs.data(), b->descs.bytes, dev->stream);
+		b->starts.upload(b->T.starts.data(), b->starts.bytes, dev->stream);
+		HIPCHK(hipMemsetAsync(b->counter.p, 0, 2 * sizeof(uint32_t), dev->stream));
+		HIPCHK(hipStreamSynchronize(dev->stream));
+		LrhTraceParams &P = b->P;
+		std::memset(&P, 0, sizeof(P));
+		P.nRays = (uint32_t)b->T.total;
+		P.root = LRH_CHILD_EMPTY;
+		P.counter = (uint32_t *)b->counter.p;
+		P.spill = (uint2 *)b->spill.p;
+		P.spillDepth = b->L.spillDepth;
+		P.nLanes = nLanes;
+		P.ldsDepth = b->L.lds;
+		P.refill = b->L.refill;
+		P.error = dev->dError;
+		P.items = (const LrhBatchDesc *)b->descs.p;
+		P.itemStart = (const uint32_t *)b->starts.p;
+		P.nItems = nItems;
+	});
+	if (rc != LRH_OK) return rc;
+	*out = b.release();
+	return LRH_OK;
+}
+
+int lrh_batch_launch(lrh_batch *b, void *stream) {
+	if (!b) return fail(LRH_EINVAL, "null batch");
+	return guarded([&]() {
+		b->dev->activate();
+		if (b->T.total == 0) return;
+		hipStream_t st = stream ? (hipStream_t)stream : b->dev->stream;
+		HIPCHK(lrhLaunchTrace(b->T.layout, b->T.two, false, false, true, dim3(b->L.blocks), dim3(b->L.block),
+				b->L.ldsBytes, st, b->P));
+		b->dev->rays += b->T.total;
+	});
+}
+
+int lrh_batch_capture(lrh_batch *b, void *stream) {
+	if (!b) return fail(LRH_EINVAL, "null batch");
+	return guarded([&]() {
+		b->dev->activate();
+		hipStream_t st = stream ? (hipStream_t)stream : b->dev->stream;
+		if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+		if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+		HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+		hipError_t le = hipSuccess;
+		if (b->T.total)
+			le = lrhLaunchTrace(b->T.layout, b->T.two, false, false, true, dim3(b->L.blocks), dim3(b->L.block),
+					b->L.ldsBytes, st, b->P);
+		const hipError_t ce = hipStreamEndCapture(st, &b->graph);
+		HIPCHK(le);
+		HIPCHK(ce);
+		HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+	});
+}
+
+int lrh_batch_replay(lrh_batch *b, void *stream) {
+	if (!b) return fail(LRH_EINVAL, "null batch");
+	if (!b->exec) return fail(LRH_EINVAL, "batch has no captured graph (lrh_batch_capture)");
+	return guarded([&]() {
+		b->dev->activate();
+		HIPCHK(hipGraphLaunch(b->exec, stream ? (hipStream_t)stream : b->dev->stream));
+		b->dev->rays += b->T.total;
+	});
+}
+
+int lrh_batch_info(lrh_batch *b, unsigned long long *rays, unsigned int *blocks, unsigned int *blockSize) {
+	if (!b) return fail(LRH_EINVAL, "null batch");
+	if (rays) *rays = b->T.total;
+	if (blocks) *blocks = b->L.blocks;
+	if (blockSize) *blockSize = b->L.block;
+	return LRH_OK;
+}
+
+int lrh_batch_destroy(lrh_batch *b) {
+	if (!b) return LRH_OK;
+	(void)hipSetDevice(b->dev->ordinal);
+	(void)hipDeviceSynchronize();
+	if (b->exec) (void)hipGraphExecDestroy(b->exec);
+	if (b->graph) (void)hipGraphDestroy(b->graph);
+	delete b;
+	return LRH_OK;
 }
 
 int lrh_trace_batch(lrh_device *dev, const lrh_batch_item *items, unsigned int nItems, const lrh_trace_opts *opts,
 		void *stream) {
 	if (!dev || (nItems && !items)) return fail(LRH_EINVAL, "null argument");
-	for (unsigned i = 0; i < nItems; ++i) {
-		if (!items[i].scene || items[i].scene->dev != dev) return fail(LRH_EINVAL, "batch item on another device");
-		const int rc = lrh_trace_ex(items[i].scene, items[i].rays, items[i].hits, items[i].count, items[i].query, opts,
-				stream);
-		if (rc != LRH_OK) return rc;
-	}
-	return LRH_OK;
+	if (nItems == 0) return LRH_OK;
+	return guarded([&]() {
+		dev->activate();
+		hipStream_t st = stream ? (hipStream_t)stream : dev->stream;
+		BatchTable T = makeTable(dev, items, nItems);
+		if (T.total == 0) return;
+		const size_t extra = ((nItems + 1) * sizeof(uint32_t) + 15) & ~(size_t)15;
+		const LaunchShape L = chooseShape(dev, T.layout, T.two, false, true, (unsigned)T.total, T.maxDepth, extra, opts);
+		const unsigned nLanes = L.blocks * L.block;
+		lrh_device::Workspace &w = workspace(dev, st, (size_t)nLanes * L.spillDepth * sizeof(uint2));
+		{
+			// descriptor table of this call; stream order keeps the previous
+			// call's kernel ahead of the copy that overwrites it
+			std::lock_guard<std::mutex> g(dev->mu);
+			const size_t need = T.descs.size() * sizeof(LrhBatchDesc) + T.starts.size() * sizeof(uint32_t);
+			if (w.tableBytes < need) {
+				HIPCHK(hipStreamSynchronize(st));
+				dev->devFree(w.table, w.tableBytes);
+				w.table = nullptr;
+				w.tableBytes = 0;
+				w.table = (uint8_t *)dev->devAlloc(need);
+				w.tableBytes = need;
+			}
+		}
+		LrhBatchDesc *dDesc = (LrhBatchDesc *)w.table;
+		uint32_t *dStart = (uint32_t *)(w.table + T.descs.size() * sizeof(LrhBatchDesc));
+		HIPCHK(hipMemcpyAsync(dDesc, T.descs.data(), T.descs.size() * sizeof(LrhBatchDesc), hipMemcpyHostToDevice, st));
+		HIPCHK(hipMemcpyAsync(dStart, T.starts.data(), T.starts.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+		LrhTraceParams P;
+		std::memset(&P, 0, sizeof(P));
+		P.nRays = (uint32_t)T.total;
+		P.root = LRH_CHILD_EMPTY;
+		P.counter = w.counter;
+		P.spill = w.spill;
+		P.spillDepth = L.spillDepth;
+		P.nLanes = nLanes;
+		P.ldsDepth = L.lds;
+		P.refill = L.refill;
+		P.error = dev->dError;
+		P.stats = w.stats;
+		P.items = dDesc;
+		P.itemStart = dStart;
+		P.nItems = nItems;
+		HIPCHK(lrhLaunchTrace(T.layout, T.two, false, false, true, dim3(L.blocks), dim3(L.block), L.ldsBytes, st, P));
+		dev->rays += T.total;
+	});
 }
 
 int lrh_device_stats(lrh_device *dev, unsigned long long *rays, double *seconds) {

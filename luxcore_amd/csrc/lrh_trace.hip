@@ -267,9 +267,25 @@ constexpr uint32_t kMaxSteps = 1u << 24;  // per-ray guard against corrupt trees
 
 }  // namespace
 
-template <int LAYOUT, bool TWO, bool ANY, bool COUNT>
+// Per-lane view of the scene and buffers a ray belongs to.  In a plain trace
+// every field comes from the launch parameters (uniform, kept in SGPRs); in a
+// batched trace each lane loads its item's descriptor when it takes a ray.
+struct SceneView {
+	const void *nodes;
+	const lrh_tri *tris;
+	const lrh_inst *insts;
+	const float *xforms;
+	const lrh_motion *motions;
+	const lrh_motion_seg *segs;
+	const float *times;
+	lrh_hit_rec *hits;
+	uint32_t root;
+	uint32_t flags;  // LRH_ITEM_ANY | LRH_ITEM_TWO
+};
+
+template <int LAYOUT, bool TWO, bool ANY, bool COUNT, bool BATCH>
 __global__ __launch_bounds__(256) void lrh_trace_kernel(LrhTraceParams P) {
-	extern __shared__ uint2 ldsStack[];
+	extern __shared__ uint2 ldsStack[];  // [ldsDepth][blockDim] stack; batched: then item starts
 	const uint32_t lane = __lane_id();
 	Stack st;
 	st.lds = ldsStack;
@@ -283,7 +299,26 @@ __global__ __launch_bounds__(256) void lrh_trace_kernel(LrhTraceParams P) {
 	st.spillDepth = P.spillDepth;
 	st.sp = 0;
 
-	uint32_t rid = kNoRay;
+	uint32_t *itemStart = nullptr;
+	if (BATCH) {
+		itemStart = reinterpret_cast<uint32_t *>(ldsStack + (size_t)P.ldsDepth * blockDim.x);
+		for (uint32_t i = threadIdx.x; i <= P.nItems; i += blockDim.x) itemStart[i] = P.itemStart[i];
+		__syncthreads();
+	}
+
+	SceneView v;
+	v.nodes = P.nodes;
+	v.tris = P.tris;
+	v.insts = P.insts;
+	v.xforms = P.xforms;
+	v.motions = P.motions;
+	v.segs = P.segs;
+	v.times = P.times;
+	v.hits = P.hits;
+	v.root = P.root;
+	v.flags = (ANY ? LRH_ITEM_ANY : 0u) | (TWO ? LRH_ITEM_TWO : 0u);
+
+	uint32_t rid = kNoRay;  // ray index within its buffer
 	LocalRay r;
 	float wo[3], wd[3];
 	float mint = 0.f, best = 0.f, time = 0.f, hb1 = 0.f, hb2 = 0.f;
@@ -293,7 +328,6 @@ __global__ __launch_bounds__(256) void lrh_trace_kernel(LrhTraceParams P) {
 	bool exhausted = false;
 	uint32_t cNode = 0, cPrim = 0, cRay = 0;  // COUNT variant only
 
-	// ticket source: one contiguous range per item (a single item for plain traces)
 	for (;;) {
 		if (!exhausted) {
 			const uint64_t idle = __ballot(rid == kNoRay);
@@ -305,12 +339,35 @@ __global__ __launch_bounds__(256) void lrh_trace_kernel(LrhTraceParams P) {
 				base = __shfl(base, leader);
 				if (base + nIdle >= P.nRays) exhausted = true;
 				if (rid == kNoRay) {
-					const uint32_t myr = base + __popcll(idle & ((1ull << lane) - 1ull));
-					if (myr < P.nRays) {
-						const uint4 *rp = reinterpret_cast<const uint4 *>(P.rays + myr);
+					const uint32_t ticket = base + __popcll(idle & ((1ull << lane) - 1ull));
+					if (ticket < P.nRays) {
+						uint32_t local = ticket;
+						const lrh_ray_rec *rays = P.rays;
+						if (BATCH) {
+							// item of this ticket: the last item whose start is <= ticket
+							uint32_t lo = 0, hi = P.nItems;
+							while (hi - lo > 1) {
+								const uint32_t mid = (lo + hi) >> 1;
+								if (itemStart[mid] <= ticket) lo = mid; else hi = mid;
+							}
+							const LrhBatchDesc &d = P.items[lo];
+							local = ticket - itemStart[lo];
+							rays = d.rays;
+							v.nodes = d.nodes;
+							v.tris = d.tris;
+							v.insts = d.insts;
+							v.xforms = d.xforms;
+							v.motions = d.motions;
+							v.segs = d.segs;
+							v.times = d.times;
+							v.hits = d.hits;
+							v.root = d.root;
+							v.flags = d.flags & (TWO ? (LRH_ITEM_ANY | LRH_ITEM_TWO) : LRH_ITEM_ANY);
+						}
+						const uint4 *rp = reinterpret_cast<const uint4 *>(rays + local);
 						const uint4 a = rp[0], b = rp[1], c = rp[2];
 						if (!(c.y & 1u)) {  // RAY_FLAGS_MASKED rays are skipped, hit untouched
-							rid = myr;
+							rid = local;
 							wo[0] = asf(a.x); wo[1] = asf(a.y); wo[2] = asf(a.z);
 							wd[0] = asf(a.w); wd[1] = asf(b.x); wd[2] = asf(b.y);
 							mint = asf(b.z);
@@ -321,7 +378,7 @@ __global__ __launch_bounds__(256) void lrh_trace_kernel(LrhTraceParams P) {
 							setupRay(r);
 							hb1 = 0.f; hb2 = 0.f;
 							hTri = LRH_NULL_INDEX; hMesh = LRH_NULL_INDEX;
-							cur = P.root;
+							cur = v.root;
 							curT = -INFINITY;
 							inInst = false;
 							steps = 0;
@@ -339,6 +396,8 @@ __global__ __launch_bounds__(256) void lrh_trace_kernel(LrhTraceParams P) {
 		const uint32_t minLive = exhausted ? 1u : P.refill;
 		do {
 			if (rid != kNoRay) {
+				const bool two = TWO && (v.flags & LRH_ITEM_TWO);
+				const bool anyq = (BATCH || ANY) && (v.flags & LRH_ITEM_ANY);
 				bool done = false;
 				if (cur == LRH_CHILD_EMPTY) {
 					if (st.sp == 0) {
@@ -368,14 +427,14 @@ __global__ __launch_bounds__(256) void lrh_trace_kernel(LrhTraceParams P) {
 						const uint32_t cnt = ((cur >> LRH_LEAF_START_BITS) & 7u) + 1u;
 						cur = LRH_CHILD_EMPTY;
 						if (COUNT) cPrim += cnt;
-						if (TWO && !inInst) {
+						if (two && !inInst) {
 							// root leaf: enter the object's tree in its local space
-							const lrh_inst in = P.insts[start];
+							const lrh_inst in = v.insts[start];
 							if (in.kind == 1u) {
-								xfRay(P.xforms + 16 * (size_t)in.index, wo, wd, r);
+								xfRay(v.xforms + 16 * (size_t)in.index, wo, wd, r);
 							} else if (in.kind == 2u) {
 								float M[16];
-								motionSample(P.motions + in.index, P.segs, P.times, time, M);
+								motionSample(v.motions + in.index, v.segs, v.times, time, M);
 								xfRay(M, wo, wd, r);
 							}
 							obj = in.object;
@@ -387,7 +446,7 @@ __global__ __launch_bounds__(256) void lrh_trace_kernel(LrhTraceParams P) {
 								done = true;
 						} else {
 							for (uint32_t k = 0; k < cnt; ++k) {
-								const uint4 *tp = reinterpret_cast<const uint4 *>(P.tris + start + k);
+								const uint4 *tp = reinterpret_cast<const uint4 *>(v.tris + start + k);
 								const uint4 a = tp[0], b = tp[1], c = tp[2];
 								// Triangle::Intersect, rounded as written
 								const float p0x = asf(a.x), p0y = asf(a.y), p0z = asf(a.z);
@@ -413,8 +472,8 @@ __global__ __launch_bounds__(256) void lrh_trace_kernel(LrhTraceParams P) {
 									hb1 = b1;
 									hb2 = b2;
 									hTri = c.y;
-									hMesh = TWO ? obj : c.z;
-									if (ANY) {
+									hMesh = two ? obj : c.z;
+									if (anyq) {
 										done = true;
 										break;
 									}
@@ -426,9 +485,9 @@ __global__ __launch_bounds__(256) void lrh_trace_kernel(LrhTraceParams P) {
 						uint32_t c4[4];
 						if (COUNT) ++cNode;
 						if (LAYOUT == 0)
-							testNodeQ(reinterpret_cast<const lrh_node4q *>(P.nodes), cur, r, mint, best, t4, c4);
+							testNodeQ(reinterpret_cast<const lrh_node4q *>(v.nodes), cur, r, mint, best, t4, c4);
 						else
-							testNodeF(reinterpret_cast<const lrh_node4f *>(P.nodes), cur, r, mint, best, t4, c4);
+							testNodeF(reinterpret_cast<const lrh_node4f *>(v.nodes), cur, r, mint, best, t4, c4);
 						cswap(t4[0], c4[0], t4[1], c4[1]);
 						cswap(t4[2], c4[2], t4[3], c4[3]);
 						cswap(t4[0], c4[0], t4[2], c4[2]);
@@ -447,7 +506,7 @@ __global__ __launch_bounds__(256) void lrh_trace_kernel(LrhTraceParams P) {
 					}
 				}
 				if (done) {
-					lrh_hit_rec *h = P.hits + rid;
+					lrh_hit_rec *h = v.hits + rid;
 					h->t = best;
 					h->b1 = hb1;
 					h->b2 = hb2;
@@ -494,18 +553,23 @@ __global__ void lrh_reset_kernel(uint32_t *p, uint32_t n) {
 // ---------------------------------------------------------------------------
 typedef void (*KernelFn)(LrhTraceParams);
 
-#define LRH_K(L, T, A, C) (const void *)&lrh_trace_kernel<L, T, A, C>
-static const void *kernelPtr(int layout, bool two, bool any, bool count) {
-	static const void *tab[2][2][2][2] = {
-		{{{LRH_K(0, false, false, false), LRH_K(0, false, false, true)},
-		  {LRH_K(0, false, true, false), LRH_K(0, false, true, true)}},
-		 {{LRH_K(0, true, false, false), LRH_K(0, true, false, true)},
-		  {LRH_K(0, true, true, false), LRH_K(0, true, true, true)}}},
-		{{{LRH_K(1, false, false, false), LRH_K(1, false, false, true)},
-		  {LRH_K(1, false, true, false), LRH_K(1, false, true, true)}},
-		 {{LRH_K(1, true, false, false), LRH_K(1, true, false, true)},
-		  {LRH_K(1, true, true, false), LRH_K(1, true, true, true)}}}};
-	return tab[layout ? 1 : 0][two ? 1 : 0][any ? 1 : 0][count ? 1 : 0];
+#define LRH_K(L, T, A, C, B) (const void *)&lrh_trace_kernel<L, T, A, C, B>
+static const void *kernelPtr(int layout, bool two, bool any, bool count, bool batch) {
+	static const void *plain[2][2][2][2] = {
+		{{{LRH_K(0, false, false, false, false), LRH_K(0, false, false, true, false)},
+		  {LRH_K(0, false, true, false, false), LRH_K(0, false, true, true, false)}},
+		 {{LRH_K(0, true, false, false, false), LRH_K(0, true, false, true, false)},
+		  {LRH_K(0, true, true, false, false), LRH_K(0, true, true, true, false)}}},
+		{{{LRH_K(1, false, false, false, false), LRH_K(1, false, false, true, false)},
+		  {LRH_K(1, false, true, false, false), LRH_K(1, false, true, true, false)}},
+		 {{LRH_K(1, true, false, false, false), LRH_K(1, true, false, true, false)},
+		  {LRH_K(1, true, true, false, false), LRH_K(1, true, true, true, false)}}}};
+	// batched: the query is per item, so ANY is unused; COUNT is not offered
+	static const void *batched[2][2] = {
+		{LRH_K(0, false, false, false, true), LRH_K(0, true, false, false, true)},
+		{LRH_K(1, false, false, false, true), LRH_K(1, true, false, false, true)}};
+	if (batch) return batched[layout ? 1 : 0][two ? 1 : 0];
+	return plain[layout ? 1 : 0][two ? 1 : 0][any ? 1 : 0][count ? 1 : 0];
 }
 
 hipError_t lrhLaunchReset(uint32_t *p, uint32_t n, hipStream_t s) {
@@ -513,12 +577,13 @@ hipError_t lrhLaunchReset(uint32_t *p, uint32_t n, hipStream_t s) {
 	return hipGetLastError();
 }
 
-hipError_t lrhOccupancy(int layout, bool two, bool any, int block, size_t lds, int *blocksPerCU) {
-	return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kernelPtr(layout, two, any, false), block, lds);
+hipError_t lrhOccupancy(int layout, bool two, bool any, bool batch, int block, size_t lds, int *blocksPerCU) {
+	return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kernelPtr(layout, two, any, false, batch), block,
+			lds);
 }
 
-hipError_t lrhLaunchTrace(int layout, bool two, bool any, bool count, dim3 grid, dim3 block, size_t lds,
+hipError_t lrhLaunchTrace(int layout, bool two, bool any, bool count, bool batch, dim3 grid, dim3 block, size_t lds,
 		hipStream_t s, const LrhTraceParams &P) {
 	void *args[] = {(void *)&P};
-	return hipLaunchKernel(kernelPtr(layout, two, any, count), grid, block, args, lds, s);
+	return hipLaunchKernel(kernelPtr(layout, two, any, count, batch), grid, block, args, lds, s);
 }

@@ -102,6 +102,14 @@ SIGNATURES = {
                          ctypes.POINTER(ctypes.c_ulonglong)], ctypes.c_int),
     "lrh_trace_batch": ([_vp, ctypes.POINTER(LrhBatchItem), ctypes.c_uint, ctypes.POINTER(LrhTraceOpts), _vp],
                         ctypes.c_int),
+    "lrh_batch_create": ([_vp, ctypes.POINTER(LrhBatchItem), ctypes.c_uint, ctypes.POINTER(LrhTraceOpts), _pp],
+                         ctypes.c_int),
+    "lrh_batch_launch": ([_vp, _vp], ctypes.c_int),
+    "lrh_batch_capture": ([_vp, _vp], ctypes.c_int),
+    "lrh_batch_replay": ([_vp, _vp], ctypes.c_int),
+    "lrh_batch_info": ([_vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_uint),
+                        ctypes.POINTER(ctypes.c_uint)], ctypes.c_int),
+    "lrh_batch_destroy": ([_vp], ctypes.c_int),
     "lrh_device_stats": ([_vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_double)],
                          ctypes.c_int),
 }
@@ -304,6 +312,58 @@ def host_scene_mbvh(inst_scene, layout=0, max_leaf=4, threads=0, trav_cost=1.0):
     return Scene(None, p.value, True)
 
 
+def _batch_items(items):
+    arr = (LrhBatchItem * max(1, len(items)))()
+    for i, (scene, rb, hb, count, query) in enumerate(items):
+        arr[i].scene = scene.h
+        arr[i].rays = rb.ptr if hasattr(rb, "ptr") else rb
+        arr[i].hits = hb.ptr if hasattr(hb, "ptr") else hb
+        arr[i].count = count
+        arr[i].query = query
+    return arr
+
+
+class Batch:
+    """A resident batched launch (lrh_batch_*): items are tuples
+    (scene, ray buffer, hit buffer, count, query)."""
+
+    def __init__(self, dev, items, opts=None):
+        self.dev = dev
+        self.items = list(items)  # keeps the buffers and scenes alive
+        arr = _batch_items(self.items)
+        p = ctypes.c_void_p()
+        check(lib().lrh_batch_create(dev.h, arr, len(self.items), ctypes.byref(opts) if opts else None,
+                                     ctypes.byref(p)))
+        self.h = p.value
+        r = ctypes.c_ulonglong()
+        b = ctypes.c_uint()
+        bs = ctypes.c_uint()
+        check(lib().lrh_batch_info(self.h, ctypes.byref(r), ctypes.byref(b), ctypes.byref(bs)))
+        self.rays, self.blocks, self.block_size = r.value, b.value, bs.value
+
+    def launch(self, stream=None):
+        check(lib().lrh_batch_launch(self.h, stream))
+
+    def capture(self, stream=None):
+        check(lib().lrh_batch_capture(self.h, stream))
+
+    def replay(self, stream=None):
+        check(lib().lrh_batch_replay(self.h, stream))
+
+    def close(self):
+        if self.h and self.dev.h:
+            check(lib().lrh_batch_destroy(self.h))
+        self.h = None
+
+    def __del__(self):
+        try:
+            if self.h and self.dev.h:
+                lib().lrh_batch_destroy(self.h)
+        except Exception:
+            pass
+        self.h = None
+
+
 class Device:
     """One GPU: buffers, streams, scenes and traces."""
 
@@ -397,14 +457,12 @@ class Device:
         return out[0], out[1], out[2]
 
     def trace_batch(self, items, opts=None, stream=None):
-        arr = (LrhBatchItem * max(1, len(items)))()
-        for i, (scene, rb, hb, count, query) in enumerate(items):
-            arr[i].scene = scene.h
-            arr[i].rays = rb.ptr
-            arr[i].hits = hb.ptr
-            arr[i].count = count
-            arr[i].query = query
+        """One persistent launch over items (scene, rays, hits, count, query)."""
+        arr = _batch_items(items)
         check(lib().lrh_trace_batch(self.h, arr, len(items), ctypes.byref(opts) if opts else None, stream))
+
+    def batch(self, items, opts=None):
+        return Batch(self, items, opts)
 
     def trace_numpy(self, scene, rays, query=0, opts=None, hits_init=None):
         """Upload rays, trace, download hits (synchronous convenience)."""

@@ -425,3 +425,65 @@ def instanced_rays(n=1 << 22, extent=100.0, seed=9):
     r = random_rays(n, (-extent,) * 3, (extent,) * 3, rng)
     r["time"] = rng.random(n, dtype=np.float32)
     return r
+
+
+# ---------------------------------------------------------------------------
+# batched workload: 8 resident scenes, 64 ray buffers
+# ---------------------------------------------------------------------------
+BATCH_SCENE_TRIS = (10_000, 50_000, 200_000, 500_000, 1_000_000, 2_000_000, 3_000_000, 5_000_000)
+
+
+def soup_scene(ntris, extent=100.0, seed=0, mean_edge=0.05):
+    """Random soup: centres uniform in [0, extent]^3, triangle size lognormal
+    (sigma 0.5) with mean edge length mean_edge * extent."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sigma = 0.5
+    mu = np.log(mean_edge * extent) - sigma * sigma / 2
+    L = rng.lognormal(mu, sigma, ntris).astype(np.float32)
+    c = (rng.random((ntris, 3), dtype=np.float32) * extent).astype(np.float32)
+    u = rng.normal(size=(ntris, 3, 3)).astype(np.float32)
+    u /= np.linalg.norm(u, axis=2, keepdims=True)
+    # vertices on a sphere of radius L/sqrt(3) around the centre: edges ~ L
+    v = c[:, None, :] + u * (L / np.sqrt(3.0))[:, None, None]
+    return Mesh(v.reshape(-1, 3).astype(np.float32), np.arange(3 * ntris, dtype=np.uint32).reshape(-1, 3))
+
+
+def sphere_grid_scene(ntris, extent=100.0, seed=0, freq=10):
+    """Displaced geodesic spheres (20*freq^2 tris each) on a jittered grid."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    per = 20 * freq * freq
+    n = max(1, ntris // per)
+    base = displaced_sphere(freq=freq, seed=seed + 1)
+    k = int(np.ceil(n ** (1.0 / 3.0)))
+    cell = extent / k
+    idx = np.arange(n)
+    grid = np.stack([idx % k, (idx // k) % k, idx // (k * k)], axis=1).astype(np.float32)
+    centres = (grid + 0.5 + rng.uniform(-0.2, 0.2, (n, 3))) * cell
+    radii = rng.uniform(0.2, 0.45, n) * cell
+    v = base.verts[None, :, :] * radii[:, None, None] + centres[:, None, :]
+    t = base.tris[None, :, :] + (np.arange(n, dtype=np.uint32) * base.verts.shape[0])[:, None, None]
+    return Mesh(v.reshape(-1, 3).astype(np.float32), t.reshape(-1, 3).astype(np.uint32))
+
+
+def batch_scenes(sizes=BATCH_SCENE_TRIS, extent=100.0, seed=100):
+    """Scene i has sizes[i] tris: even i a random soup, odd i a sphere grid."""
+    out = []
+    for i, n in enumerate(sizes):
+        m = soup_scene(n, extent, seed + i) if i % 2 == 0 else sphere_grid_scene(n, extent, seed + i)
+        out.append([m])
+    return out
+
+
+def batch_buffers(n_buffers=64, lo=4096, hi=4 << 20, n_scenes=8, extent=100.0, seed=200):
+    """Buffer specs (scene index, ray count, query): sizes log-uniform in
+    [lo, hi], scenes and closest/any queries uniform."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), n_buffers)).astype(np.int64)
+    scene = rng.integers(0, n_scenes, n_buffers)
+    query = rng.integers(0, 2, n_buffers)
+    return [(int(scene[i]), int(sizes[i]), int(query[i])) for i in range(n_buffers)]
+
+
+def buffer_rays(count, extent=100.0, seed=0):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return random_rays(count, (0, 0, 0), (extent, extent, extent), rng)
