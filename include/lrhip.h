@@ -200,6 +200,29 @@ int lrh_batch_capture(lrh_batch *batch, void *stream);
 int lrh_batch_replay(lrh_batch *batch, void *stream);
 int lrh_batch_info(lrh_batch *batch, unsigned long long *rays, unsigned int *blocks, unsigned int *blockSize);
 int lrh_batch_destroy(lrh_batch *batch);
+/* ---- ray generation around a trace (served path-tracing sessions) ----
+ * Camera rays of a pinhole camera (pixel centres, row-major, width*height
+ * rays) and, from a wavefront's rays + hits on a single-level scene, the next
+ * cosine-weighted bounce (Russian roulette from bounce rrStart on, survival
+ * rrSurvive; terminated paths become masked rays) and one shadow ray per hit
+ * toward a uniform point of a 1x1 light centred at lightCenter (masked where
+ * the path missed).  Random numbers hash (seed, pixelBase + i, bounce).  They
+ * stand for SLG's camera / next-vertex ray generation
+ * (pathoclbase_kernels_micro.cl:723-844, 1045-1075). */
+typedef struct {
+	float eye[3], look[3], up[3];
+	float vfovDeg;
+	float mint;
+	unsigned int width, height;
+} lrh_camera;
+int lrh_rays_camera(lrh_device *dev, const lrh_camera *cam, void *rays, void *stream);
+int lrh_rays_bounce(lrh_scene *scene, const void *rays, const void *hits, unsigned int count,
+		unsigned int bounce, unsigned long long seed, unsigned int pixelBase, unsigned int rrStart,
+		float rrSurvive, const float lightCenter[3], void *nextRays, void *shadowRays,
+		unsigned int *liveCount /* device counter or NULL: += live rays written */, void *stream);
+/* 1 if all work before the event has completed, 0 if not yet, < 0 on error. */
+int lrh_event_query(lrh_event *ev);
+
 /* Data-parallel ray count traced so far (IntersectionDevice stats). */
 int lrh_device_stats(lrh_device *dev, unsigned long long *rays, double *seconds);
 

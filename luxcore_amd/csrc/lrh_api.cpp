@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <chrono>
 #include <cstdio>
@@ -22,6 +23,7 @@
 #include "../../include/lrhip.h"
 #include "lrh_build.h"
 #include "lrh_kernels.h"
+#include "lrh_paths.h"
 #include "lrh_layout.h"
 
 using lrh::Box;
@@ -144,6 +146,8 @@ struct lrh_scene {
 	uint32_t root = LRH_CHILD_EMPTY;
 	uint32_t maxDepth = 0;
 	DevBuf nodes, tris, insts, xforms, motions, segs, times;
+	DevBuf invPerm, meshTriBase;  // single level: (mesh, triangle) -> record, for hit shading
+	unsigned nMeshes = 0;
 	lrh_scene_info info{};
 	// two-level state kept for lrh_scene_update
 	lrh::BuildParams params;
@@ -211,7 +215,8 @@ struct HostTree {
 
 // build a mesh's triangle tree; appends nodes and triangle records
 uint32_t buildMeshTree(const lrh_mesh *meshes, const std::vector<unsigned> &meshList, bool localMeshIndex,
-		const lrh::BuildParams &p, HostTree &ht, std::vector<lrh_tri> &tris, uint32_t *maxDepth) {
+		const lrh::BuildParams &p, HostTree &ht, std::vector<lrh_tri> &tris, uint32_t *maxDepth,
+		std::vector<uint32_t> *invPerm = nullptr) {
 	// global primitive -> (mesh, tri)
 	std::vector<size_t> off(meshList.size() + 1, 0);
 	for (size_t i = 0; i < meshList.size(); ++i) off[i + 1] = off[i] + meshes[meshList[i]].nTris;
@@ -257,6 +262,10 @@ uint32_t buildMeshTree(const lrh_mesh *meshes, const std::vector<unsigned> &mesh
 			r.pad = 0;
 		}
 	});
+	if (invPerm) {
+		invPerm->resize(n);
+		for (size_t i = 0; i < n; ++i) (*invPerm)[w.order[i]] = (uint32_t)(primBase + i);
+	}
 	if (w.maxDepth > *maxDepth) *maxDepth = w.maxDepth;
 	return w.root;
 }
@@ -350,7 +359,7 @@ void finishInfo(lrh_scene *s, size_t nNodes, size_t nPrims, size_t primBytes, do
 	if (!s->dev) primBytes = s->hostPrims.size();
 	s->info.primBytes = primBytes;
 	s->info.deviceBytes = s->nodes.bytes + s->tris.bytes + s->insts.bytes + s->xforms.bytes + s->motions.bytes +
-			s->segs.bytes + s->times.bytes;
+			s->segs.bytes + s->times.bytes + s->invPerm.bytes + s->meshTriBase.bytes;
 	s->info.maxDepth = s->maxDepth;
 	s->info.layout = s->layout;
 	s->info.buildSeconds = tb;
@@ -593,7 +602,11 @@ int lrh_scene_create_bvh(lrh_device *dev, const lrh_mesh *meshes, unsigned int n
 		std::vector<lrh_tri> tris;
 		std::vector<unsigned> all(nMeshes);
 		for (unsigned i = 0; i < nMeshes; ++i) all[i] = i;
-		s->root = buildMeshTree(meshes, all, false, s->params, ht, tris, &s->maxDepth);
+		std::vector<uint32_t> inv;
+		s->root = buildMeshTree(meshes, all, false, s->params, ht, tris, &s->maxDepth, &inv);
+		std::vector<uint32_t> base(nMeshes + 1, 0);
+		for (unsigned i = 0; i < nMeshes; ++i) base[i + 1] = base[i] + meshes[i].nTris;
+		s->nMeshes = nMeshes;
 		const double t1 = now();
 		if (!dev) {
 			keepHost(s.get(), ht, tris.data(), tris.size() * sizeof(lrh_tri), nullptr, 0);
@@ -605,6 +618,10 @@ int lrh_scene_create_bvh(lrh_device *dev, const lrh_mesh *meshes, unsigned int n
 		s->nodes.upload(ht.data(s->layout), ht.bytes(s->layout), dev->stream);
 		s->tris.alloc(dev, std::max<size_t>(tris.size() * sizeof(lrh_tri), 48));
 		s->tris.upload(tris.data(), tris.size() * sizeof(lrh_tri), dev->stream);
+		s->invPerm.alloc(dev, std::max<size_t>(inv.size() * 4, 4));
+		s->invPerm.upload(inv.data(), inv.size() * 4, dev->stream);
+		s->meshTriBase.alloc(dev, base.size() * 4);
+		s->meshTriBase.upload(base.data(), base.size() * 4, dev->stream);
 		HIPCHK(hipStreamSynchronize(dev->stream));
 		finishInfo(s.get(), ht.count(s->layout), tris.size(), tris.size() * sizeof(lrh_tri), t1 - t0, now() - t1);
 	});
@@ -1128,6 +1145,84 @@ int lrh_trace_batch(lrh_device *dev, const lrh_batch_item *items, unsigned int n
 		HIPCHK(lrhLaunchTrace(T.layout, T.two, false, false, true, dim3(L.blocks), dim3(L.block), L.ldsBytes, st, P));
 		dev->rays += T.total;
 	});
+}
+
+int lrh_rays_camera(lrh_device *dev, const lrh_camera *cam, void *rays, void *stream) {
+	if (!dev || !cam || !rays) return fail(LRH_EINVAL, "null argument");
+	if (!cam->width || !cam->height || (uint64_t)cam->width * cam->height > 0x7fffffffull)
+		return fail(LRH_ERANGE, "bad camera size");
+	if ((uintptr_t)rays & 15u) return fail(LRH_EINVAL, "ray buffer must be 16-byte aligned");
+	return guarded([&]() {
+		dev->activate();
+		LrhCameraParams C;
+		double f[3], r[3], u[3];
+		for (int k = 0; k < 3; ++k) f[k] = (double)cam->look[k] - cam->eye[k];
+		double fl = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+		if (!(fl > 0)) throw Error(LRH_EINVAL, "camera eye equals look");
+		for (int k = 0; k < 3; ++k) f[k] /= fl;
+		r[0] = f[1] * cam->up[2] - f[2] * cam->up[1];
+		r[1] = f[2] * cam->up[0] - f[0] * cam->up[2];
+		r[2] = f[0] * cam->up[1] - f[1] * cam->up[0];
+		double rl = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+		if (!(rl > 0)) throw Error(LRH_EINVAL, "camera up parallel to the view direction");
+		for (int k = 0; k < 3; ++k) r[k] /= rl;
+		u[0] = r[1] * f[2] - r[2] * f[1];
+		u[1] = r[2] * f[0] - r[0] * f[2];
+		u[2] = r[0] * f[1] - r[1] * f[0];
+		for (int k = 0; k < 3; ++k) {
+			C.eye[k] = cam->eye[k];
+			C.fwd[k] = (float)f[k];
+			C.right[k] = (float)r[k];
+			C.up[k] = (float)u[k];
+		}
+		C.th = (float)tan(cam->vfovDeg * 3.14159265358979 / 360.0);
+		C.tw = C.th * (float)cam->width / (float)cam->height;
+		C.mint = cam->mint;
+		C.width = cam->width;
+		C.height = cam->height;
+		HIPCHK(lrhLaunchCamera(C, (lrh_ray_rec *)rays, stream ? (hipStream_t)stream : dev->stream));
+	});
+}
+
+int lrh_rays_bounce(lrh_scene *s, const void *rays, const void *hits, unsigned int count, unsigned int bounce,
+		unsigned long long seed, unsigned int pixelBase, unsigned int rrStart, float rrSurvive,
+		const float lightCenter[3], void *nextRays, void *shadowRays, unsigned int *liveCount, void *stream) {
+	if (!s || !s->dev || (count && (!rays || !hits || !nextRays)) || !lightCenter)
+		return fail(LRH_EINVAL, "null argument");
+	if (s->twoLevel) return fail(LRH_EINVAL, "bounce generation needs a single-level scene");
+	if (((uintptr_t)rays & 15u) || ((uintptr_t)nextRays & 15u) || ((uintptr_t)shadowRays & 15u) ||
+			((uintptr_t)hits & 3u))
+		return fail(LRH_EINVAL, "misaligned buffer");
+	if (count == 0) return LRH_OK;
+	return guarded([&]() {
+		s->dev->activate();
+		LrhBounceParams P;
+		P.rays = (const lrh_ray_rec *)rays;
+		P.hits = (const lrh_hit_rec *)hits;
+		P.next = (lrh_ray_rec *)nextRays;
+		P.shadow = (lrh_ray_rec *)shadowRays;
+		P.tris = (const lrh_tri *)s->tris.p;
+		P.invPerm = (const uint32_t *)s->invPerm.p;
+		P.meshTriBase = (const uint32_t *)s->meshTriBase.p;
+		P.nMeshes = s->nMeshes;
+		P.count = count;
+		P.bounce = bounce;
+		P.rrStart = rrStart;
+		P.rrSurvive = rrSurvive;
+		for (int k = 0; k < 3; ++k) P.light[k] = lightCenter[k];
+		P.seed = seed;
+		P.pixelBase = pixelBase;
+		P.liveCount = liveCount;
+		HIPCHK(lrhLaunchBounce(P, stream ? (hipStream_t)stream : s->dev->stream));
+	});
+}
+
+int lrh_event_query(lrh_event *ev) {
+	if (!ev) return fail(LRH_EINVAL, "null event");
+	const hipError_t e = hipEventQuery(ev->ev);
+	if (e == hipSuccess) return 1;
+	if (e == hipErrorNotReady) return 0;
+	return fail(LRH_EHIP, std::string("hipEventQuery: ") + hipGetErrorString(e));
 }
 
 int lrh_device_stats(lrh_device *dev, unsigned long long *rays, double *seconds) {

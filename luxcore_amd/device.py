@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(_HERE, "liblrhip.so")
 
 LRH_OK = 0
 LRH_H2D, LRH_D2H, LRH_D2D = 1, 2, 3
+LRH_QUERY_CLOSEST, LRH_QUERY_ANY = 0, 1
 ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "ERANGE", -5: "ENODEV", -6: "EDEVICE"}
 
 
@@ -54,6 +55,12 @@ class LrhSceneInfo(ctypes.Structure):
 class LrhTraceOpts(ctypes.Structure):
     _fields_ = [("blockSize", ctypes.c_uint), ("ldsStack", ctypes.c_uint), ("blocksPerCU", ctypes.c_uint),
                 ("refillThreshold", ctypes.c_uint)]
+
+
+class LrhCamera(ctypes.Structure):
+    _fields_ = [("eye", ctypes.c_float * 3), ("look", ctypes.c_float * 3), ("up", ctypes.c_float * 3),
+                ("vfovDeg", ctypes.c_float), ("mint", ctypes.c_float), ("width", ctypes.c_uint),
+                ("height", ctypes.c_uint)]
 
 
 class LrhBatchItem(ctypes.Structure):
@@ -110,6 +117,11 @@ SIGNATURES = {
     "lrh_batch_info": ([_vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_uint),
                         ctypes.POINTER(ctypes.c_uint)], ctypes.c_int),
     "lrh_batch_destroy": ([_vp], ctypes.c_int),
+    "lrh_rays_camera": ([_vp, ctypes.POINTER(LrhCamera), _vp, _vp], ctypes.c_int),
+    "lrh_rays_bounce": ([_vp, _vp, _vp, ctypes.c_uint, ctypes.c_uint, ctypes.c_ulonglong, ctypes.c_uint,
+                         ctypes.c_uint, ctypes.c_float, ctypes.POINTER(ctypes.c_float), _vp, _vp, _vp, _vp],
+                        ctypes.c_int),
+    "lrh_event_query": ([_vp], ctypes.c_int),
     "lrh_device_stats": ([_vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_double)],
                          ctypes.c_int),
 }
@@ -185,6 +197,12 @@ class Event:
 
     def sync(self):
         check(lib().lrh_event_sync(self.h))
+
+    def query(self):
+        rc = lib().lrh_event_query(self.h)
+        if rc < 0:
+            check(rc)
+        return rc == 1
 
     def elapsed_ms(self, end):
         ms = ctypes.c_float()
@@ -477,6 +495,25 @@ class Device:
         self.trace(scene, rb, hb, n, query, opts)
         self.sync()
         return self.download(hb, HIT_DTYPE, n)
+
+    def camera_rays(self, rays_buf, eye, look, up=(0.0, 0.0, 1.0), vfov_deg=45.0, width=512, height=512,
+                    mint=1e-4, stream=None):
+        c = LrhCamera()
+        c.eye[:] = list(eye)
+        c.look[:] = list(look)
+        c.up[:] = list(up)
+        c.vfovDeg = vfov_deg
+        c.mint = mint
+        c.width, c.height = width, height
+        check(lib().lrh_rays_camera(self.h, ctypes.byref(c), rays_buf.ptr, stream))
+
+    def bounce_rays(self, scene, rays_buf, hits_buf, count, bounce, seed, next_buf, shadow_buf=None,
+                    pixel_base=0, rr_start=2, rr_survive=0.7, light=(50.0, 50.0, 120.0), live_counter=None,
+                    stream=None):
+        lc = (ctypes.c_float * 3)(*light)
+        check(lib().lrh_rays_bounce(scene.h, rays_buf.ptr, hits_buf.ptr, count, bounce, seed, pixel_base,
+                                    rr_start, rr_survive, lc, next_buf.ptr, shadow_buf.ptr if shadow_buf else None,
+                                    live_counter, stream))
 
     def stats(self):
         r = ctypes.c_ulonglong()
