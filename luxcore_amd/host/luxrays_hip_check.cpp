@@ -1,0 +1,226 @@
+/*
+ * luxrays_hip_check — drives the C++ device classes the way an application of
+ * the reference drives an intersection device (DataSet::Add for every mesh,
+ * SetDataSet/Start, AllocBufferRO/RW, EnqueueTraceRayBuffer, EnqueueReadBuffer,
+ * FinishQueue, TraceRay, Update after a transform edit) on a scene read from
+ * files, and writes the hit records.
+ *
+ *   luxrays_hip_check <dir> [--accel auto|bvh|mbvh] [--anyhit] [--serial N]
+ *
+ * <dir>/meshes.bin   u32 nMeshes; per mesh u32 nVerts, u32 nTris,
+ *                    f32 verts[3 nVerts], u32 tris[3 nTris]
+ * <dir>/objects.bin  optional; u32 nObjects; per object u32 mesh, u32 kind
+ *                    (0 mesh, 1 instance, 2 motion), f32 m[16], u32 nKeys,
+ *                    f32 times[nKeys], f32 mats[16 nKeys]
+ *                    Without it every mesh is one object of kind 0.
+ * <dir>/objects2.bin optional; same objects with new matrices / keys: traced
+ *                    again after the edit and Update()
+ * <dir>/rays.bin     Ray[n] (48-byte records)
+ * writes <dir>/hits.bin (RayHit[n]), hits2.bin after an update, and
+ * serial.bin (RayHit[N] of TraceRay on the first N rays).
+ */
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <iterator>
+
+#include "luxrays_hip.h"
+
+using namespace luxrays_hip;
+
+namespace {
+
+std::vector<char> readAll(const std::string &path, bool required) {
+	std::ifstream f(path, std::ios::binary);
+	if (!f) {
+		if (required) throw std::runtime_error("cannot open " + path);
+		return {};
+	}
+	return std::vector<char>((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+}
+
+void writeAll(const std::string &path, const void *p, size_t n) {
+	std::ofstream f(path, std::ios::binary);
+	f.write(static_cast<const char *>(p), (std::streamsize)n);
+	if (!f) throw std::runtime_error("cannot write " + path);
+}
+
+struct Reader {
+	const std::vector<char> &b;
+	size_t at = 0;
+	explicit Reader(const std::vector<char> &buf) : b(buf) {}
+	template <typename T>
+	void get(T *out, size_t n) {
+		if (at + n * sizeof(T) > b.size()) throw std::runtime_error("truncated input file");
+		std::memcpy(out, b.data() + at, n * sizeof(T));
+		at += n * sizeof(T);
+	}
+	u_int u() {
+		u_int v;
+		get(&v, 1);
+		return v;
+	}
+};
+
+struct ObjDesc {
+	u_int mesh, kind;
+	float m[16];
+	std::vector<float> times, mats;
+};
+
+std::vector<ObjDesc> readObjects(const std::vector<char> &buf) {
+	Reader r(buf);
+	std::vector<ObjDesc> out(r.u());
+	for (ObjDesc &o : out) {
+		o.mesh = r.u();
+		o.kind = r.u();
+		r.get(o.m, 16);
+		const u_int k = r.u();
+		o.times.resize(k);
+		o.mats.resize(16 * (size_t)k);
+		r.get(o.times.data(), k);
+		r.get(o.mats.data(), 16 * (size_t)k);
+	}
+	return out;
+}
+
+Matrix4x4 mat(const float *p) {
+	float a[4][4];
+	std::memcpy(a, p, 64);
+	return Matrix4x4(a);
+}
+
+MotionSystem motion(const ObjDesc &o) {
+	std::vector<Transform> keys;
+	for (size_t j = 0; j < o.times.size(); ++j) keys.push_back(Transform(mat(&o.mats[16 * j])));
+	return MotionSystem(o.times, keys);
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+	if (argc < 2) {
+		std::cerr << "usage: luxrays_hip_check <dir> [--accel auto|bvh|mbvh] [--anyhit] [--serial N]\n";
+		return 2;
+	}
+	const std::string dir = argv[1];
+	AcceleratorType accelType = ACCEL_AUTO;
+	bool anyHit = false;
+	u_int serial = 0;
+	for (int i = 2; i < argc; ++i) {
+		const std::string a = argv[i];
+		if (a == "--accel" && i + 1 < argc) {
+			const std::string v = argv[++i];
+			accelType = v == "bvh" ? ACCEL_BVH : v == "mbvh" ? ACCEL_MBVH : ACCEL_AUTO;
+		} else if (a == "--anyhit")
+			anyHit = true;
+		else if (a == "--serial" && i + 1 < argc)
+			serial = (u_int)std::stoul(argv[++i]);
+		else {
+			std::cerr << "unknown argument " << a << "\n";
+			return 2;
+		}
+	}
+	try {
+		// meshes
+		const std::vector<char> mb = readAll(dir + "/meshes.bin", true);
+		Reader r(mb);
+		const u_int nMeshes = r.u();
+		std::vector<std::vector<Point>> verts(nMeshes);
+		std::vector<std::vector<Triangle>> tris(nMeshes);
+		std::vector<std::unique_ptr<TriangleMesh>> base;
+		for (u_int i = 0; i < nMeshes; ++i) {
+			const u_int nv = r.u(), nt = r.u();
+			verts[i].resize(nv);
+			tris[i].resize(nt);
+			r.get(verts[i].data(), nv);
+			r.get(tris[i].data(), nt);
+			base.emplace_back(new TriangleMesh(nv, nt, verts[i].data(), tris[i].data()));
+		}
+		// objects
+		std::vector<std::unique_ptr<Mesh>> owned;
+		std::vector<Mesh *> objects;
+		const std::vector<char> ob = readAll(dir + "/objects.bin", false);
+		if (ob.empty()) {
+			for (auto &m : base) objects.push_back(m.get());
+		} else {
+			for (const ObjDesc &o : readObjects(ob)) {
+				if (o.mesh >= nMeshes) throw std::runtime_error("object mesh index out of range");
+				if (o.kind == 0)
+					objects.push_back(base[o.mesh].get());
+				else if (o.kind == 1) {
+					owned.emplace_back(new InstanceTriangleMesh(base[o.mesh].get(), Transform(mat(o.m))));
+					objects.push_back(owned.back().get());
+				} else {
+					owned.emplace_back(new MotionTriangleMesh(base[o.mesh].get(), motion(o)));
+					objects.push_back(owned.back().get());
+				}
+			}
+		}
+		DataSet ds;
+		for (Mesh *m : objects) ds.Add(m);
+		ds.Preprocess();
+
+		// rays
+		std::vector<char> rb = readAll(dir + "/rays.bin", true);
+		if (rb.size() % sizeof(Ray)) throw std::runtime_error("rays.bin is not a whole number of Ray records");
+		const u_int n = (u_int)(rb.size() / sizeof(Ray));
+		std::vector<RayHit> hits(n);
+
+		HIPIntersectionDevice device(0);
+		device.SetDataSet(&ds, accelType);
+		device.Start();
+		std::cout << "device " << device.GetName() << " accel "
+				  << (device.GetAccelerator()->GetType() == ACCEL_BVH ? "BVH" : "MBVH") << " rays " << n << std::endl;
+
+		HardwareDeviceBuffer *rayBuff = nullptr, *hitBuff = nullptr;
+		device.AllocBufferRO(&rayBuff, rb.data(), rb.size(), "Ray");
+		device.AllocBufferRW(&hitBuff, nullptr, (size_t)n * sizeof(RayHit), "RayHit");
+		auto traceAll = [&](const char *name) {
+			if (anyHit)
+				device.EnqueueTraceRayBufferAnyHit(rayBuff, hitBuff, n);
+			else
+				device.EnqueueTraceRayBuffer(rayBuff, hitBuff, n);
+			device.EnqueueReadBuffer(hitBuff, false, (size_t)n * sizeof(RayHit), hits.data());
+			device.FinishQueue();
+			writeAll(dir + "/" + name, hits.data(), (size_t)n * sizeof(RayHit));
+		};
+		traceAll("hits.bin");
+
+		if (serial) {
+			const u_int k = serial < n ? serial : n;
+			std::vector<RayHit> sh(k);
+			const Ray *rays = reinterpret_cast<const Ray *>(rb.data());
+			u_int nHit = 0;
+			for (u_int i = 0; i < k; ++i) nHit += device.TraceRay(&rays[i], &sh[i]) ? 1 : 0;
+			writeAll(dir + "/serial.bin", sh.data(), (size_t)k * sizeof(RayHit));
+			std::cout << "serial " << k << " hits " << nHit << std::endl;
+		}
+
+		// DataSet edit: new transforms / keys, then Update()
+		const std::vector<char> ob2 = readAll(dir + "/objects2.bin", false);
+		if (!ob2.empty()) {
+			const std::vector<ObjDesc> o2 = readObjects(ob2);
+			if (o2.size() != objects.size()) throw std::runtime_error("objects2.bin has another object count");
+			for (size_t i = 0; i < o2.size(); ++i) {
+				if (o2[i].kind == 1)
+					static_cast<InstanceTriangleMesh *>(objects[i])->SetTransformation(Transform(mat(o2[i].m)));
+				else if (o2[i].kind == 2)
+					static_cast<MotionTriangleMesh *>(objects[i])->SetMotionSystem(motion(o2[i]));
+			}
+			device.Update();
+			traceAll("hits2.bin");
+		}
+
+		std::cout << "data-parallel rays " << device.GetDataParallelRayCount() << " serial rays "
+				  << device.GetSerialRayCount() << " used memory " << device.GetUsedMemory() << std::endl;
+		device.FreeBuffer(&rayBuff);
+		device.FreeBuffer(&hitBuff);
+		device.Stop();
+	} catch (const std::exception &e) {
+		std::cerr << "error: " << e.what() << std::endl;
+		return 1;
+	}
+	return 0;
+}
