@@ -111,13 +111,18 @@ typedef struct {
 } lrh_scene_info;
 
 /* Tuning of the trace kernel; all zero = built-in defaults.
- * blockSize in {64,128,256}; ldsStack in {8,16,24,32} entries per lane;
- * blocksPerCU 0 = occupancy-derived persistent grid. */
+ * blockSize in {64,128,256}; ldsStack a power of two in 4..64 entries per
+ * lane; blocksPerCU 0 = occupancy-derived persistent grid. */
 typedef struct {
 	unsigned int blockSize;
 	unsigned int ldsStack;
 	unsigned int blocksPerCU;
 	unsigned int refillThreshold;  /* refill a wave when fewer than this many lanes are live; 0 = default */
+	unsigned int loop;             /* 0 default; 1 a lane tests a leaf as soon as it reaches it; 2 postponed
+	                                  leaves: single-level closest-hit lanes park one leaf and keep descending,
+	                                  parked leaves are tested together */
+	unsigned int leafBatch;        /* loop 2: test parked leaves once leafBatch/8 of the live lanes hold one
+	                                  (or no lane can descend); 0 = default 8 */
 } lrh_trace_opts;
 
 /* One ray buffer of a batched launch (lrh_trace_batch). */

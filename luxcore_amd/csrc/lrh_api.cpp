@@ -798,6 +798,8 @@ namespace {
 struct LaunchShape {
 	unsigned block, lds, blocks, refill, spillDepth;
 	size_t ldsBytes;
+	bool pl;  // postponed leaves
+	unsigned leafBatch;
 };
 
 unsigned pow2floor(unsigned v) {
@@ -818,17 +820,22 @@ LaunchShape chooseShape(lrh_device *dev, unsigned layout, bool two, bool any, bo
 	if (L.lds < 4 || L.lds > 64 || pow2floor(L.lds) != L.lds)
 		throw Error(LRH_EINVAL, "ldsStack must be a power of two in 4..64");
 	L.refill = (o && o->refillThreshold) ? std::min(64u, o->refillThreshold) : 12;
+	const unsigned loop = o ? o->loop : 0u;
+	if (loop > 2) throw Error(LRH_EINVAL, "loop must be 0 (default), 1 (leaves when reached) or 2 (postponed leaves)");
+	L.pl = loop == 2;
+	L.leafBatch = (o && o->leafBatch) ? o->leafBatch : 8u;
+	if (L.leafBatch > 8) throw Error(LRH_EINVAL, "leafBatch must be 0..8 (eighths of the live lanes)");
 	L.ldsBytes = (size_t)L.lds * L.block * sizeof(uint2) + extraLds;
 	if (L.ldsBytes > 65536) throw Error(LRH_ERANGE, "LDS stack and item table exceed 64 KiB per block");
 	int perCU = (o && o->blocksPerCU) ? (int)o->blocksPerCU : 0;
 	if (!perCU) {
 		const uint64_t key = ((uint64_t)layout << 40) | ((uint64_t)two << 39) | ((uint64_t)any << 38) |
-				((uint64_t)batch << 37) | ((uint64_t)L.ldsBytes << 8) | (L.block >> 6);
+				((uint64_t)batch << 37) | ((uint64_t)L.pl << 36) | ((uint64_t)L.ldsBytes << 8) | (L.block >> 6);
 		std::lock_guard<std::mutex> g(dev->mu);
 		auto it = dev->occ.find(key);
 		if (it == dev->occ.end()) {
 			int nb = 0;
-			HIPCHK(lrhOccupancy(layout, two, any, batch, L.block, L.ldsBytes, &nb));
+			HIPCHK(lrhOccupancy(layout, two, any, batch, L.pl, L.block, L.ldsBytes, &nb));
 			dev->occ[key] = nb > 0 ? nb : 1;
 			perCU = nb > 0 ? nb : 1;
 		} else
@@ -903,10 +910,11 @@ int traceImpl(lrh_scene *s, const void *rays, void *hits, unsigned int count, un
 		P.nLanes = nLanes;
 		P.ldsDepth = L.lds;
 		P.refill = L.refill;
+		P.leafBatch = L.leafBatch;
 		P.error = dev->dError;
 		P.stats = w.stats;
 		if (statsOut) HIPCHK(hipMemsetAsync(w.stats, 0, 4 * sizeof(unsigned long long), st));
-		HIPCHK(lrhLaunchTrace(s->layout, s->twoLevel, any, statsOut != nullptr, false, dim3(L.blocks), dim3(L.block),
+		HIPCHK(lrhLaunchTrace(s->layout, s->twoLevel, any, statsOut != nullptr, false, L.pl, dim3(L.blocks), dim3(L.block),
 				L.ldsBytes, st, P));
 		dev->rays += count;
 		if (statsOut) {
@@ -1027,6 +1035,7 @@ int lrh_batch_create(lrh_device *dev, const lrh_batch_item *items, unsigned int 
 		P.nLanes = nLanes;
 		P.ldsDepth = b->L.lds;
 		P.refill = b->L.refill;
+		P.leafBatch = b->L.leafBatch;
 		P.error = dev->dError;
 		P.items = (const LrhBatchDesc *)b->descs.p;
 		P.itemStart = (const uint32_t *)b->starts.p;
@@ -1043,7 +1052,7 @@ int lrh_batch_launch(lrh_batch *b, void *stream) {
 		b->dev->activate();
 		if (b->T.total == 0) return;
 		hipStream_t st = stream ? (hipStream_t)stream : b->dev->stream;
-		HIPCHK(lrhLaunchTrace(b->T.layout, b->T.two, false, false, true, dim3(b->L.blocks), dim3(b->L.block),
+		HIPCHK(lrhLaunchTrace(b->T.layout, b->T.two, false, false, true, b->L.pl, dim3(b->L.blocks), dim3(b->L.block),
 				b->L.ldsBytes, st, b->P));
 		b->dev->rays += b->T.total;
 	});
@@ -1059,7 +1068,7 @@ int lrh_batch_capture(lrh_batch *b, void *stream) {
 		HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
 		hipError_t le = hipSuccess;
 		if (b->T.total)
-			le = lrhLaunchTrace(b->T.layout, b->T.two, false, false, true, dim3(b->L.blocks), dim3(b->L.block),
+			le = lrhLaunchTrace(b->T.layout, b->T.two, false, false, true, b->L.pl, dim3(b->L.blocks), dim3(b->L.block),
 					b->L.ldsBytes, st, b->P);
 		const hipError_t ce = hipStreamEndCapture(st, &b->graph);
 		HIPCHK(le);
@@ -1137,12 +1146,13 @@ int lrh_trace_batch(lrh_device *dev, const lrh_batch_item *items, unsigned int n
 		P.nLanes = nLanes;
 		P.ldsDepth = L.lds;
 		P.refill = L.refill;
+		P.leafBatch = L.leafBatch;
 		P.error = dev->dError;
 		P.stats = w.stats;
 		P.items = dDesc;
 		P.itemStart = dStart;
 		P.nItems = nItems;
-		HIPCHK(lrhLaunchTrace(T.layout, T.two, false, false, true, dim3(L.blocks), dim3(L.block), L.ldsBytes, st, P));
+		HIPCHK(lrhLaunchTrace(T.layout, T.two, false, false, true, L.pl, dim3(L.blocks), dim3(L.block), L.ldsBytes, st, P));
 		dev->rays += T.total;
 	});
 }

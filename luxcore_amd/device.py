@@ -54,7 +54,8 @@ class LrhSceneInfo(ctypes.Structure):
 
 class LrhTraceOpts(ctypes.Structure):
     _fields_ = [("blockSize", ctypes.c_uint), ("ldsStack", ctypes.c_uint), ("blocksPerCU", ctypes.c_uint),
-                ("refillThreshold", ctypes.c_uint)]
+                ("refillThreshold", ctypes.c_uint), ("loop", ctypes.c_uint),
+                ("leafBatch", ctypes.c_uint)]
 
 
 class LrhCamera(ctypes.Structure):
@@ -264,9 +265,10 @@ def _build_opts(layout=0, max_leaf=4, threads=0, trav_cost=1.0):
     return o
 
 
-def trace_opts(block=0, lds=0, blocks_per_cu=0, refill=0):
+def trace_opts(block=0, lds=0, blocks_per_cu=0, refill=0, loop=0, leaf_batch=0):
     o = LrhTraceOpts()
-    o.blockSize, o.ldsStack, o.blocksPerCU, o.refillThreshold = block, lds, blocks_per_cu, refill
+    o.blockSize, o.ldsStack, o.blocksPerCU, o.refillThreshold, o.loop = block, lds, blocks_per_cu, refill, loop
+    o.leafBatch = leaf_batch
     return o
 
 
