@@ -118,11 +118,8 @@ typedef struct {
 	unsigned int ldsStack;
 	unsigned int blocksPerCU;
 	unsigned int refillThreshold;  /* refill a wave when fewer than this many lanes are live; 0 = default */
-	unsigned int loop;             /* 0 default; 1 a lane tests a leaf as soon as it reaches it; 2 postponed
-	                                  leaves: single-level closest-hit lanes park one leaf and keep descending,
-	                                  parked leaves are tested together */
-	unsigned int leafBatch;        /* loop 2: test parked leaves once leafBatch/8 of the live lanes hold one
-	                                  (or no lane can descend); 0 = default 8 */
+	unsigned int leafTris;         /* triangles a lane tests per traversal pass; the rest of its leaf waits
+	                                  for the next pass; 0 = default (the whole leaf) */
 } lrh_trace_opts;
 
 /* One ray buffer of a batched launch (lrh_trace_batch). */
@@ -182,10 +179,14 @@ int lrh_trace_ex(lrh_scene *scene, const void *rays, void *hits, unsigned int co
 		unsigned int query, const lrh_trace_opts *opts, void *stream);
 /* Same trace through a counting variant of the kernel, synchronous:
  * out[0] = inner-node visits, out[1] = primitive records tested (triangles,
- * plus instance records in two-level scenes), out[2] = rays traced.  Used for
- * the bytes-per-ray roofline accounting. */
+ * plus instance records in two-level scenes), out[2] = rays traced — the
+ * bytes-per-ray roofline accounting; and the SIMD-efficiency counters, summed
+ * over waves: out[3] = traversal passes, out[4] = live lanes summed over those
+ * passes, out[5] = passes with a node test, out[6] = passes with a leaf test,
+ * out[7] = refills. */
+#define LRH_TRACE_STATS 8
 int lrh_trace_stats(lrh_scene *scene, const void *rays, void *hits, unsigned int count,
-		unsigned int query, const lrh_trace_opts *opts, void *stream, unsigned long long out[3]);
+		unsigned int query, const lrh_trace_opts *opts, void *stream, unsigned long long out[LRH_TRACE_STATS]);
 /* Many ray buffers over one or more resident scenes of the same device in one
  * persistent launch: the rays of all items form one ticket space, every lane
  * takes the next ticket, finds its item and traces in that item's scene with

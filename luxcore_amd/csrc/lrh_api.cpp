@@ -429,7 +429,7 @@ int lrh_device_close(lrh_device *dev) {
 	for (auto &w : dev->ws) {
 		dev->devFree(w.second.spill, w.second.spillBytes);
 		dev->devFree(w.second.counter, 2 * sizeof(uint32_t));
-		dev->devFree(w.second.stats, 4 * sizeof(unsigned long long));
+		dev->devFree(w.second.stats, LRH_TRACE_STATS * sizeof(unsigned long long));
 		dev->devFree(w.second.table, w.second.tableBytes);
 	}
 	dev->devFree(dev->dError, sizeof(uint32_t));
@@ -796,10 +796,8 @@ int lrh_scene_download(lrh_scene *s, void *nodes, size_t nodeBytes, void *prims,
 namespace {
 
 struct LaunchShape {
-	unsigned block, lds, blocks, refill, spillDepth;
+	unsigned block, lds, blocks, refill, spillDepth, leafTris;
 	size_t ldsBytes;
-	bool pl;  // postponed leaves
-	unsigned leafBatch;
 };
 
 unsigned pow2floor(unsigned v) {
@@ -820,22 +818,19 @@ LaunchShape chooseShape(lrh_device *dev, unsigned layout, bool two, bool any, bo
 	if (L.lds < 4 || L.lds > 64 || pow2floor(L.lds) != L.lds)
 		throw Error(LRH_EINVAL, "ldsStack must be a power of two in 4..64");
 	L.refill = (o && o->refillThreshold) ? std::min(64u, o->refillThreshold) : 12;
-	const unsigned loop = o ? o->loop : 0u;
-	if (loop > 2) throw Error(LRH_EINVAL, "loop must be 0 (default), 1 (leaves when reached) or 2 (postponed leaves)");
-	L.pl = loop == 2;
-	L.leafBatch = (o && o->leafBatch) ? o->leafBatch : 8u;
-	if (L.leafBatch > 8) throw Error(LRH_EINVAL, "leafBatch must be 0..8 (eighths of the live lanes)");
+	L.leafTris = (o && o->leafTris) ? o->leafTris : 8u;
+	if (L.leafTris > 8) throw Error(LRH_EINVAL, "leafTris must be 0..8");
 	L.ldsBytes = (size_t)L.lds * L.block * sizeof(uint2) + extraLds;
 	if (L.ldsBytes > 65536) throw Error(LRH_ERANGE, "LDS stack and item table exceed 64 KiB per block");
 	int perCU = (o && o->blocksPerCU) ? (int)o->blocksPerCU : 0;
 	if (!perCU) {
 		const uint64_t key = ((uint64_t)layout << 40) | ((uint64_t)two << 39) | ((uint64_t)any << 38) |
-				((uint64_t)batch << 37) | ((uint64_t)L.pl << 36) | ((uint64_t)L.ldsBytes << 8) | (L.block >> 6);
+				((uint64_t)batch << 37) | ((uint64_t)L.ldsBytes << 8) | (L.block >> 6);
 		std::lock_guard<std::mutex> g(dev->mu);
 		auto it = dev->occ.find(key);
 		if (it == dev->occ.end()) {
 			int nb = 0;
-			HIPCHK(lrhOccupancy(layout, two, any, batch, L.pl, L.block, L.ldsBytes, &nb));
+			HIPCHK(lrhOccupancy(layout, two, any, batch, L.block, L.ldsBytes, &nb));
 			dev->occ[key] = nb > 0 ? nb : 1;
 			perCU = nb > 0 ? nb : 1;
 		} else
@@ -854,7 +849,7 @@ lrh_device::Workspace &workspace(lrh_device *dev, hipStream_t st, size_t spillBy
 	lrh_device::Workspace &w = dev->ws[st];
 	if (!w.counter) {
 		w.counter = (uint32_t *)dev->devAlloc(2 * sizeof(uint32_t));
-		w.stats = (unsigned long long *)dev->devAlloc(4 * sizeof(unsigned long long));
+		w.stats = (unsigned long long *)dev->devAlloc(LRH_TRACE_STATS * sizeof(unsigned long long));
 		HIPCHK(hipMemsetAsync(w.counter, 0, 2 * sizeof(uint32_t), st));
 	}
 	if (w.spillBytes < spillBytes) {
@@ -910,15 +905,15 @@ int traceImpl(lrh_scene *s, const void *rays, void *hits, unsigned int count, un
 		P.nLanes = nLanes;
 		P.ldsDepth = L.lds;
 		P.refill = L.refill;
-		P.leafBatch = L.leafBatch;
+		P.leafTris = L.leafTris;
 		P.error = dev->dError;
 		P.stats = w.stats;
-		if (statsOut) HIPCHK(hipMemsetAsync(w.stats, 0, 4 * sizeof(unsigned long long), st));
-		HIPCHK(lrhLaunchTrace(s->layout, s->twoLevel, any, statsOut != nullptr, false, L.pl, dim3(L.blocks), dim3(L.block),
+		if (statsOut) HIPCHK(hipMemsetAsync(w.stats, 0, LRH_TRACE_STATS * sizeof(unsigned long long), st));
+		HIPCHK(lrhLaunchTrace(s->layout, s->twoLevel, any, statsOut != nullptr, false, dim3(L.blocks), dim3(L.block),
 				L.ldsBytes, st, P));
 		dev->rays += count;
 		if (statsOut) {
-			HIPCHK(hipMemcpyAsync(statsOut, w.stats, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+			HIPCHK(hipMemcpyAsync(statsOut, w.stats, LRH_TRACE_STATS * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
 			HIPCHK(hipStreamSynchronize(st));
 		}
 	});
@@ -998,9 +993,9 @@ int lrh_trace(lrh_scene *s, const void *rays, void *hits, unsigned int count, un
 }
 
 int lrh_trace_stats(lrh_scene *s, const void *rays, void *hits, unsigned int count, unsigned int query,
-		const lrh_trace_opts *opts, void *stream, unsigned long long out[3]) {
+		const lrh_trace_opts *opts, void *stream, unsigned long long out[LRH_TRACE_STATS]) {
 	if (!out) return fail(LRH_EINVAL, "null stats output");
-	out[0] = out[1] = out[2] = 0;
+	for (int i = 0; i < LRH_TRACE_STATS; ++i) out[i] = 0;
 	return traceImpl(s, rays, hits, count, query, opts, stream, out);
 }
 
@@ -1035,7 +1030,7 @@ int lrh_batch_create(lrh_device *dev, const lrh_batch_item *items, unsigned int 
 		P.nLanes = nLanes;
 		P.ldsDepth = b->L.lds;
 		P.refill = b->L.refill;
-		P.leafBatch = b->L.leafBatch;
+		P.leafTris = b->L.leafTris;
 		P.error = dev->dError;
 		P.items = (const LrhBatchDesc *)b->descs.p;
 		P.itemStart = (const uint32_t *)b->starts.p;
@@ -1052,7 +1047,7 @@ int lrh_batch_launch(lrh_batch *b, void *stream) {
 		b->dev->activate();
 		if (b->T.total == 0) return;
 		hipStream_t st = stream ? (hipStream_t)stream : b->dev->stream;
-		HIPCHK(lrhLaunchTrace(b->T.layout, b->T.two, false, false, true, b->L.pl, dim3(b->L.blocks), dim3(b->L.block),
+		HIPCHK(lrhLaunchTrace(b->T.layout, b->T.two, false, false, true, dim3(b->L.blocks), dim3(b->L.block),
 				b->L.ldsBytes, st, b->P));
 		b->dev->rays += b->T.total;
 	});
@@ -1068,7 +1063,7 @@ int lrh_batch_capture(lrh_batch *b, void *stream) {
 		HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
 		hipError_t le = hipSuccess;
 		if (b->T.total)
-			le = lrhLaunchTrace(b->T.layout, b->T.two, false, false, true, b->L.pl, dim3(b->L.blocks), dim3(b->L.block),
+			le = lrhLaunchTrace(b->T.layout, b->T.two, false, false, true, dim3(b->L.blocks), dim3(b->L.block),
 					b->L.ldsBytes, st, b->P);
 		const hipError_t ce = hipStreamEndCapture(st, &b->graph);
 		HIPCHK(le);
@@ -1146,13 +1141,13 @@ int lrh_trace_batch(lrh_device *dev, const lrh_batch_item *items, unsigned int n
 		P.nLanes = nLanes;
 		P.ldsDepth = L.lds;
 		P.refill = L.refill;
-		P.leafBatch = L.leafBatch;
+		P.leafTris = L.leafTris;
 		P.error = dev->dError;
 		P.stats = w.stats;
 		P.items = dDesc;
 		P.itemStart = dStart;
 		P.nItems = nItems;
-		HIPCHK(lrhLaunchTrace(T.layout, T.two, false, false, true, L.pl, dim3(L.blocks), dim3(L.block), L.ldsBytes, st, P));
+		HIPCHK(lrhLaunchTrace(T.layout, T.two, false, false, true, dim3(L.blocks), dim3(L.block), L.ldsBytes, st, P));
 		dev->rays += T.total;
 	});
 }

@@ -688,7 +688,10 @@ void quantize(const WNode &w, lrh_node4q *q) {
 		const float s = expScale((uint32_t)e);
 		*ex[k] = (uint8_t)e;
 		for (uint32_t c = 0; c < 4; ++c) {
-			if (c >= w.n) { qlo[k][c] = 0; qhi[k][c] = 0; continue; }
+			// an empty slot gets an inverted box (lo at the top, hi at the
+			// bottom of the node): the slab test rejects it without a check of
+			// the child reference
+			if (c >= w.n) { qlo[k][c] = 255; qhi[k][c] = 0; continue; }
 			const float lo = w.cbox[c].lo[k], hi = w.cbox[c].hi[k];
 			long ql = (long)floorf((lo - o) / s);
 			if (ql < 0) ql = 0;

@@ -45,18 +45,18 @@ struct LrhTraceParams {
 	uint32_t nLanes;
 	uint32_t ldsDepth;              /* power of two */
 	uint32_t refill;                /* refill a wave when fewer lanes than this are live */
-	uint32_t leafBatch;             /* postponed leaves: test them once leafBatch/8 of the live lanes hold one */
+	uint32_t leafTris;              /* triangles a lane tests per pass (1..8); the rest of a leaf waits a pass */
 	uint32_t *error;                /* bit 0 stack overflow, bit 1 step guard */
-	unsigned long long *stats;      /* counting variant: node visits, prim tests, rays */
+	unsigned long long *stats;      /* counting variant: LRH_TRACE_STATS counters */
 	const LrhBatchDesc *items;      /* batched: item descriptors */
 	const uint32_t *itemStart;      /* batched: nItems+1 ticket offsets (prefix sums of counts) */
 	uint32_t nItems;
 };
 
 hipError_t lrhLaunchReset(uint32_t *p, uint32_t n, hipStream_t s);
-hipError_t lrhOccupancy(int layout, bool two, bool any, bool batch, bool pl, int block, size_t lds,
+hipError_t lrhOccupancy(int layout, bool two, bool any, bool batch, int block, size_t lds,
 		int *blocksPerCU);
-hipError_t lrhLaunchTrace(int layout, bool two, bool any, bool count, bool batch, bool pl, dim3 grid, dim3 block,
+hipError_t lrhLaunchTrace(int layout, bool two, bool any, bool count, bool batch, dim3 grid, dim3 block,
 		size_t lds, hipStream_t s, const LrhTraceParams &P);
 
 #endif

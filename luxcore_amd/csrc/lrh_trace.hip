@@ -190,8 +190,10 @@ DEVI void testNodeQ(const lrh_node4q *__restrict__ nodes, uint32_t idx, const Lo
 		const float tn = fmaxf(fmaxf(tnx, tny), tnz) * kLo;
 		const float tf = fminf(fminf(tfx, tfy), tfz) * kHi;
 		const float a = fmaxf(tn, mint), b = fminf(tf, best);
-		const bool hit = (a <= b) && (c[k] != LRH_CHILD_EMPTY);
-		t[k] = hit ? a : INFINITY;
+		// empty slots hold inverted boxes; one that still passes (a far node
+		// whose inverted extent is below the widening) is dropped when its
+		// LRH_CHILD_EMPTY reference is taken or popped
+		t[k] = (a <= b) ? a : INFINITY;
 	}
 }
 
@@ -213,8 +215,7 @@ DEVI void testNodeF(const lrh_node4f *__restrict__ nodes, uint32_t idx, const Lo
 		const float tn = fmaxf(fmaxf(tnx, tny), tnz) * kLo;
 		const float tf = fminf(fminf(tfx, tfy), tfz) * kHi;
 		const float a = fmaxf(tn, mint), b = fminf(tf, best);
-		const bool hit = (a <= b) && (c[k] != LRH_CHILD_EMPTY);
-		t[k] = hit ? a : INFINITY;
+		t[k] = (a <= b) ? a : INFINITY;  // empty slots: lo = +inf, hi = -inf
 	}
 }
 
@@ -283,7 +284,7 @@ struct SceneView {
 	uint32_t flags;  // LRH_ITEM_ANY | LRH_ITEM_TWO
 };
 
-template <int LAYOUT, bool TWO, bool ANY, bool COUNT, bool BATCH, bool PL>
+template <int LAYOUT, bool TWO, bool ANY, bool COUNT, bool BATCH>
 __global__ __launch_bounds__(256) void lrh_trace_kernel(LrhTraceParams P) {
 	extern __shared__ uint2 ldsStack[];  // [ldsDepth][blockDim] stack; batched: then item starts
 	const uint32_t lane = __lane_id();
@@ -323,245 +324,246 @@ __global__ __launch_bounds__(256) void lrh_trace_kernel(LrhTraceParams P) {
 	float wo[3], wd[3];
 	float mint = 0.f, best = 0.f, time = 0.f, hb1 = 0.f, hb2 = 0.f;
 	uint32_t hTri = LRH_NULL_INDEX, hMesh = LRH_NULL_INDEX, cur = LRH_CHILD_EMPTY, obj = 0, steps = 0;
-	uint32_t pend = LRH_CHILD_EMPTY;  // parked leaf (postponed-leaf mode)
 	float curT = 0.f;
 	bool inInst = false;
 	bool exhausted = false;
-	uint32_t cNode = 0, cPrim = 0, cRay = 0;  // COUNT variant only
+	// COUNT variant only: per lane node tests / primitive tests / rays; per
+	// wave passes, live lanes summed over passes, passes with a node test,
+	// passes with a leaf test, refills
+	uint32_t cNode = 0, cPrim = 0, cRay = 0, cPass = 0, cLive = 0, cNodePass = 0, cLeafPass = 0, cRefill = 0;
+
 
 	for (;;) {
+		// the ray record a lane takes this pass
+		uint4 ldA = make_uint4(0u, 0u, 0u, 0u), ldB = make_uint4(0u, 0u, 0u, 0u);
+		uint32_t ldTime = 0u, ldFlags = 0u;
+		bool loading = false;
+		// ---- refill: idle lanes take tickets when fewer than `refill` lanes are live ----
 		if (!exhausted) {
 			const uint64_t idle = __ballot(rid == kNoRay);
-			if (idle) {
-				const uint32_t nIdle = __popcll(idle);
+			const uint32_t nIdle = (uint32_t)__popcll(idle);
+			if (64u - nIdle < P.refill) {
+				if (COUNT) ++cRefill;
+				// one atomic per refill for exactly the idle lanes: tickets are
+				// taken on demand so the grid drains evenly at the end
 				const int leader = __ffsll((long long)idle) - 1;
 				uint32_t base = 0;
 				if ((int)lane == leader) base = atomicAdd(P.counter, nIdle);
 				base = __shfl(base, leader);
 				if (base + nIdle >= P.nRays) exhausted = true;
-				if (rid == kNoRay) {
-					const uint32_t ticket = base + __popcll(idle & ((1ull << lane) - 1ull));
-					if (ticket < P.nRays) {
-						uint32_t local = ticket;
-						const lrh_ray_rec *rays = P.rays;
-						if (BATCH) {
-							// item of this ticket: the last item whose start is <= ticket
-							uint32_t lo = 0, hi = P.nItems;
-							while (hi - lo > 1) {
-								const uint32_t mid = (lo + hi) >> 1;
-								if (itemStart[mid] <= ticket) lo = mid; else hi = mid;
-							}
-							const LrhBatchDesc &d = P.items[lo];
-							local = ticket - itemStart[lo];
-							rays = d.rays;
-							v.nodes = d.nodes;
-							v.tris = d.tris;
-							v.insts = d.insts;
-							v.xforms = d.xforms;
-							v.motions = d.motions;
-							v.segs = d.segs;
-							v.times = d.times;
-							v.hits = d.hits;
-							v.root = d.root;
-							v.flags = d.flags & (TWO ? (LRH_ITEM_ANY | LRH_ITEM_TWO) : LRH_ITEM_ANY);
+				const uint32_t ticket = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+				if (rid == kNoRay && ticket < P.nRays) {
+					uint32_t local = ticket;
+					const lrh_ray_rec *rays = P.rays;
+					if (BATCH) {
+						// item of this ticket: the last item whose start is <= ticket
+						uint32_t lo = 0, hi = P.nItems;
+						while (hi - lo > 1) {
+							const uint32_t mid = (lo + hi) >> 1;
+							if (itemStart[mid] <= ticket) lo = mid; else hi = mid;
 						}
-						const uint4 *rp = reinterpret_cast<const uint4 *>(rays + local);
-						const uint4 a = rp[0], b = rp[1], c = rp[2];
-						if (!(c.y & 1u)) {  // RAY_FLAGS_MASKED rays are skipped, hit untouched
-							rid = local;
-							wo[0] = asf(a.x); wo[1] = asf(a.y); wo[2] = asf(a.z);
-							wd[0] = asf(a.w); wd[1] = asf(b.x); wd[2] = asf(b.y);
-							mint = asf(b.z);
-							best = asf(b.w);
-							time = asf(c.x);
-							r.ox = wo[0]; r.oy = wo[1]; r.oz = wo[2];
-							r.dx = wd[0]; r.dy = wd[1]; r.dz = wd[2];
-							setupRay(r);
-							hb1 = 0.f; hb2 = 0.f;
-							hTri = LRH_NULL_INDEX; hMesh = LRH_NULL_INDEX;
-							cur = v.root;
-							curT = -INFINITY;
-							pend = LRH_CHILD_EMPTY;
-							inInst = false;
-							steps = 0;
-							st.sp = 0;
-							if (COUNT) ++cRay;
-						}
+						const LrhBatchDesc &d = P.items[lo];
+						local = ticket - itemStart[lo];
+						rays = d.rays;
+						v.nodes = d.nodes;
+						v.tris = d.tris;
+						v.insts = d.insts;
+						v.xforms = d.xforms;
+						v.motions = d.motions;
+						v.segs = d.segs;
+						v.times = d.times;
+						v.hits = d.hits;
+						v.root = d.root;
+						v.flags = d.flags & (TWO ? (LRH_ITEM_ANY | LRH_ITEM_TWO) : LRH_ITEM_ANY);
 					}
+					const uint4 *rp = reinterpret_cast<const uint4 *>(rays + local);
+					rid = local;
+					ldA = rp[0];
+					ldB = rp[1];
+					const uint2 c2 = reinterpret_cast<const uint2 *>(rp + 2)[0];
+					ldTime = c2.x;
+					ldFlags = c2.y;
+					loading = true;
+					cur = LRH_CHILD_EMPTY;
+					st.sp = 0;
 				}
+			}
+		}
+		// ---- set up the rays taken this pass ----
+		if (loading) {
+			loading = false;
+			if (ldFlags & 1u) {  // RAY_FLAGS_MASKED: skipped, hit record untouched
+				rid = kNoRay;
+			} else {
+				r.ox = asf(ldA.x); r.oy = asf(ldA.y); r.oz = asf(ldA.z);
+				r.dx = asf(ldA.w); r.dy = asf(ldB.x); r.dz = asf(ldB.y);
+				if (TWO) {
+					wo[0] = r.ox; wo[1] = r.oy; wo[2] = r.oz;
+					wd[0] = r.dx; wd[1] = r.dy; wd[2] = r.dz;
+				}
+				mint = asf(ldB.z);
+				best = asf(ldB.w);
+				time = asf(ldTime);
+				setupRay(r);
+				hb1 = 0.f; hb2 = 0.f;
+				hTri = LRH_NULL_INDEX; hMesh = LRH_NULL_INDEX;
+				cur = v.root;
+				curT = -INFINITY;
+				inInst = false;
+				steps = 0;
+				if (COUNT) ++cRay;
 			}
 		}
 		if (!__ballot(rid != kNoRay)) {
 			if (exhausted) break;
 			continue;
 		}
-		const uint32_t minLive = exhausted ? 1u : P.refill;
-		do {
-			// ---- node phase: at most one node test per lane and pass ----
-			// A lane in postponed-leaf mode (PL, single-level closest-hit rays)
-			// parks the first leaf it reaches and keeps descending; parked leaves
-			// are tested together once enough lanes hold one, so the triangle
-			// loop runs with most lanes active instead of once per pass.
-			bool done = false, park = false, want = false, blocked = false, live = false;
-			if (rid != kNoRay) {
-				const bool two = TWO && (v.flags & LRH_ITEM_TWO);
-				const bool anyq = (BATCH || ANY) && (v.flags & LRH_ITEM_ANY);
-				park = PL && !two && !anyq;
-				for (;;) {
-					if (cur == LRH_CHILD_EMPTY) {
-						if (st.sp == 0) break;
-						const uint2 e = pop(st);
-						cur = e.x;
-						curT = asf(e.y);
-						if (TWO && cur == LRH_SENTINEL) {
-							// leaving an object's tree: back to the world-space ray
-							r.ox = wo[0]; r.oy = wo[1]; r.oz = wo[2];
-							r.dx = wd[0]; r.dy = wd[1]; r.dz = wd[2];
-							setupRay(r);
-							inInst = false;
-							cur = LRH_CHILD_EMPTY;
-						} else if (!(curT < best)) {
-							cur = LRH_CHILD_EMPTY;  // entry not before the current hit
-						}
-						if (cur == LRH_CHILD_EMPTY) continue;
-					}
-					if (cur & LRH_CHILD_LEAF) {
-						if (park && pend == LRH_CHILD_EMPTY) {
-							pend = cur;
-							cur = LRH_CHILD_EMPTY;
-							continue;
-						}
-						break;
-					}
-					if (++steps > kMaxSteps) {
-						atomicOr(P.error, 2u);
-						done = true;
-						break;
-					}
-					float t4[4];
-					uint32_t c4[4];
-					if (COUNT) ++cNode;
-					if (LAYOUT == 0)
-						testNodeQ(reinterpret_cast<const lrh_node4q *>(v.nodes), cur, r, mint, best, t4, c4);
-					else
-						testNodeF(reinterpret_cast<const lrh_node4f *>(v.nodes), cur, r, mint, best, t4, c4);
-					cswap(t4[0], c4[0], t4[1], c4[1]);
-					cswap(t4[2], c4[2], t4[3], c4[3]);
-					cswap(t4[0], c4[0], t4[2], c4[2]);
-					cswap(t4[1], c4[1], t4[3], c4[3]);
-					cswap(t4[1], c4[1], t4[2], c4[2]);
-					bool ok = true;
-					if (t4[3] != INFINITY) ok = ok && push(st, c4[3], t4[3], P.error);
-					if (t4[2] != INFINITY) ok = ok && push(st, c4[2], t4[2], P.error);
-					if (t4[1] != INFINITY) ok = ok && push(st, c4[1], t4[1], P.error);
-					if (t4[0] != INFINITY) {
-						cur = c4[0];
-						curT = t4[0];
-					} else
+		if (COUNT) {
+			++cPass;
+			cLive += (uint32_t)__popcll(__ballot(rid != kNoRay));
+		}
+
+		// ---- node phase: one node test per lane and pass ----
+		bool done = false, tested = false;
+		if (rid != kNoRay) {
+			for (;;) {
+				if (cur == LRH_CHILD_EMPTY) {
+					if (st.sp == 0) break;
+					const uint2 e = pop(st);
+					cur = e.x;
+					curT = asf(e.y);
+					if (TWO && cur == LRH_SENTINEL) {
+						// leaving an object's tree: back to the world-space ray
+						r.ox = wo[0]; r.oy = wo[1]; r.oz = wo[2];
+						r.dx = wd[0]; r.dy = wd[1]; r.dz = wd[2];
+						setupRay(r);
+						inInst = false;
 						cur = LRH_CHILD_EMPTY;
-					if (!ok) done = true;
+					} else if (!(curT < best)) {
+						cur = LRH_CHILD_EMPTY;  // entry not before the current hit
+					}
+					if (cur == LRH_CHILD_EMPTY) continue;
+				}
+				if (cur & LRH_CHILD_LEAF) break;
+				if (++steps > kMaxSteps) {
+					atomicOr(P.error, 2u);
+					done = true;
 					break;
 				}
-				const bool atLeaf = cur != LRH_CHILD_EMPTY && (cur & LRH_CHILD_LEAF);
-				if (park) {
-					want = !done && pend != LRH_CHILD_EMPTY;
-					// blocked: nothing but the parked leaf left to do this pass
-					blocked = want && (atLeaf || (cur == LRH_CHILD_EMPTY && st.sp == 0));
-				} else {
-					want = !done && atLeaf;
-					blocked = want;
+				float t4[4];
+				uint32_t c4[4];
+				if (COUNT) {
+					++cNode;
+					tested = true;
 				}
-				live = !done;
-			}
-			// ---- leaf phase ----
-			bool runLeaves = true;
-			if (PL) {
-				const uint32_t nWant = (uint32_t)__popcll(__ballot(want));
-				const uint32_t nLive = (uint32_t)__popcll(__ballot(live));
-				const uint64_t nEager = __ballot(want && !park);
-				runLeaves = nEager != 0 || __ballot(blocked) == __ballot(live) || nWant * 8u >= nLive * P.leafBatch;
-			}
-			if (runLeaves && want) {
-				const bool two = TWO && (v.flags & LRH_ITEM_TWO);
-				const bool anyq = (BATCH || ANY) && (v.flags & LRH_ITEM_ANY);
-				uint32_t leaf;
-				if (park) {
-					leaf = pend;
-					pend = LRH_CHILD_EMPTY;
-				} else {
-					leaf = cur;
+				if (LAYOUT == 0)
+					testNodeQ(reinterpret_cast<const lrh_node4q *>(v.nodes), cur, r, mint, best, t4, c4);
+				else
+					testNodeF(reinterpret_cast<const lrh_node4f *>(v.nodes), cur, r, mint, best, t4, c4);
+				cswap(t4[0], c4[0], t4[1], c4[1]);
+				cswap(t4[2], c4[2], t4[3], c4[3]);
+				cswap(t4[0], c4[0], t4[2], c4[2]);
+				cswap(t4[1], c4[1], t4[3], c4[3]);
+				cswap(t4[1], c4[1], t4[2], c4[2]);
+				bool ok = true;
+				if (t4[3] != INFINITY) ok = ok && push(st, c4[3], t4[3], P.error);
+				if (t4[2] != INFINITY) ok = ok && push(st, c4[2], t4[2], P.error);
+				if (t4[1] != INFINITY) ok = ok && push(st, c4[1], t4[1], P.error);
+				if (t4[0] != INFINITY) {
+					cur = c4[0];
+					curT = t4[0];
+				} else
 					cur = LRH_CHILD_EMPTY;
+				if (!ok) done = true;
+				break;
+			}
+		}
+
+		// ---- leaf phase ----
+		const bool atLeaf = rid != kNoRay && !done && cur != LRH_CHILD_EMPTY &&
+				(cur & LRH_CHILD_LEAF);
+		if (COUNT) {
+			cNodePass += __ballot(tested) ? 1u : 0u;
+			cLeafPass += __ballot(atLeaf) ? 1u : 0u;
+		}
+		if (atLeaf) {
+			const bool two = TWO && (v.flags & LRH_ITEM_TWO);
+			const bool anyq = (BATCH || ANY) && (v.flags & LRH_ITEM_ANY);
+			const uint32_t start = cur & LRH_LEAF_START_MASK;
+			uint32_t cnt = ((cur >> LRH_LEAF_START_BITS) & 7u) + 1u;
+			cur = LRH_CHILD_EMPTY;
+			if (!two && cnt > P.leafTris) {
+				// test leafTris triangles now and keep the rest of the leaf for
+				// the next pass: a wave then never loops over the longest leaf
+				// of its lanes while the others wait
+				cur = LRH_CHILD_LEAF | ((cnt - P.leafTris - 1u) << LRH_LEAF_START_BITS) | (start + P.leafTris);
+				cnt = P.leafTris;
+			}
+			if (COUNT) cPrim += cnt;
+			if (two && !inInst) {
+				// root leaf: enter the object's tree in its local space
+				const lrh_inst in = v.insts[start];
+				if (in.kind == 1u) {
+					xfRay(v.xforms + 16 * (size_t)in.index, wo, wd, r);
+				} else if (in.kind == 2u) {
+					float M[16];
+					motionSample(v.motions + in.index, v.segs, v.times, time, M);
+					xfRay(M, wo, wd, r);
 				}
-				const uint32_t start = leaf & LRH_LEAF_START_MASK;
-				const uint32_t cnt = ((leaf >> LRH_LEAF_START_BITS) & 7u) + 1u;
-				if (COUNT) cPrim += cnt;
-				if (two && !inInst) {
-					// root leaf: enter the object's tree in its local space
-					const lrh_inst in = v.insts[start];
-					if (in.kind == 1u) {
-						xfRay(v.xforms + 16 * (size_t)in.index, wo, wd, r);
-					} else if (in.kind == 2u) {
-						float M[16];
-						motionSample(v.motions + in.index, v.segs, v.times, time, M);
-						xfRay(M, wo, wd, r);
-					}
-					obj = in.object;
-					inInst = true;
-					if (push(st, LRH_SENTINEL, -INFINITY, P.error)) {
-						cur = in.root;
-						curT = -INFINITY;
-					} else
-						done = true;
-				} else {
-					for (uint32_t k = 0; k < cnt; ++k) {
-						const uint4 *tp = reinterpret_cast<const uint4 *>(v.tris + start + k);
-						const uint4 a = tp[0], b = tp[1], c = tp[2];
-						// Triangle::Intersect, rounded as written
-						const float p0x = asf(a.x), p0y = asf(a.y), p0z = asf(a.z);
-						const float e1x = asf(a.w), e1y = asf(b.x), e1z = asf(b.y);
-						const float e2x = asf(b.z), e2y = asf(b.w), e2z = asf(c.x);
-						const float s1x = (r.dy * e2z) - (r.dz * e2y);
-						const float s1y = (r.dz * e2x) - (r.dx * e2z);
-						const float s1z = (r.dx * e2y) - (r.dy * e2x);
-						const float div = s1x * e1x + s1y * e1y + s1z * e1z;
-						const float inv = 1.f / div;
-						const float ddx = r.ox - p0x, ddy = r.oy - p0y, ddz = r.oz - p0z;
-						const float b1 = (ddx * s1x + ddy * s1y + ddz * s1z) * inv;
-						const float s2x = (ddy * e1z) - (ddz * e1y);
-						const float s2y = (ddz * e1x) - (ddx * e1z);
-						const float s2z = (ddx * e1y) - (ddy * e1x);
-						const float b2 = (r.dx * s2x + r.dy * s2y + r.dz * s2z) * inv;
-						const float b0 = 1.f - b1 - b2;
-						const float t = (e2x * s2x + e2y * s2y + e2z * s2z) * inv;
-						const bool ok = (div != 0.f) && !(b1 < 0.f) && !(b2 < 0.f) && !(b0 < 0.f) &&
-								!(t < mint) && (t < best);
-						if (ok) {
-							best = t;
-							hb1 = b1;
-							hb2 = b2;
-							hTri = c.y;
-							hMesh = two ? obj : c.z;
-							if (anyq) {
-								done = true;
-								break;
-							}
+				obj = in.object;
+				inInst = true;
+				if (push(st, LRH_SENTINEL, -INFINITY, P.error)) {
+					cur = in.root;
+					curT = -INFINITY;
+				} else
+					done = true;
+			} else {
+				for (uint32_t k = 0; k < cnt; ++k) {
+					const uint4 *tp = reinterpret_cast<const uint4 *>(v.tris + start + k);
+					const uint4 a = tp[0], b = tp[1], c = tp[2];
+					// Triangle::Intersect, rounded as written
+					const float p0x = asf(a.x), p0y = asf(a.y), p0z = asf(a.z);
+					const float e1x = asf(a.w), e1y = asf(b.x), e1z = asf(b.y);
+					const float e2x = asf(b.z), e2y = asf(b.w), e2z = asf(c.x);
+					const float s1x = (r.dy * e2z) - (r.dz * e2y);
+					const float s1y = (r.dz * e2x) - (r.dx * e2z);
+					const float s1z = (r.dx * e2y) - (r.dy * e2x);
+					const float div = s1x * e1x + s1y * e1y + s1z * e1z;
+					const float inv = 1.f / div;
+					const float ddx = r.ox - p0x, ddy = r.oy - p0y, ddz = r.oz - p0z;
+					const float b1 = (ddx * s1x + ddy * s1y + ddz * s1z) * inv;
+					const float s2x = (ddy * e1z) - (ddz * e1y);
+					const float s2y = (ddz * e1x) - (ddx * e1z);
+					const float s2z = (ddx * e1y) - (ddy * e1x);
+					const float b2 = (r.dx * s2x + r.dy * s2y + r.dz * s2z) * inv;
+					const float b0 = 1.f - b1 - b2;
+					const float t = (e2x * s2x + e2y * s2y + e2z * s2z) * inv;
+					const bool ok = (div != 0.f) && !(b1 < 0.f) && !(b2 < 0.f) && !(b0 < 0.f) &&
+							!(t < mint) && (t < best);
+					if (ok) {
+						best = t;
+						hb1 = b1;
+						hb2 = b2;
+						hTri = c.y;
+						hMesh = two ? obj : c.z;
+						if (anyq) {
+							done = true;
+							break;
 						}
 					}
 				}
 			}
-			if (rid != kNoRay &&
-					(done || (cur == LRH_CHILD_EMPTY && st.sp == 0 && pend == LRH_CHILD_EMPTY))) {
-				lrh_hit_rec *h = v.hits + rid;
-				h->t = best;
-				h->b1 = hb1;
-				h->b2 = hb2;
-				h->meshIndex = hMesh;
-				h->triangleIndex = hTri;
-				rid = kNoRay;
-				cur = LRH_CHILD_EMPTY;
-				pend = LRH_CHILD_EMPTY;
-			}
-		} while ((uint32_t)__popcll(__ballot(rid != kNoRay)) >= minLive);
+		}
+		if (rid != kNoRay && (done || (cur == LRH_CHILD_EMPTY && st.sp == 0))) {
+			lrh_hit_rec *h = v.hits + rid;
+			h->t = best;
+			h->b1 = hb1;
+			h->b2 = hb2;
+			h->meshIndex = hMesh;
+			h->triangleIndex = hTri;
+			rid = kNoRay;
+			cur = LRH_CHILD_EMPTY;
+		}
 	}
 	if (COUNT) {
 		unsigned long long a = cNode, b = cPrim, c = cRay;
@@ -571,9 +573,15 @@ __global__ __launch_bounds__(256) void lrh_trace_kernel(LrhTraceParams P) {
 			c += __shfl_xor(c, o);
 		}
 		if (lane == 0) {
+			// wave counters are uniform: lane 0 adds them once per wave
 			atomicAdd(P.stats + 0, a);
 			atomicAdd(P.stats + 1, b);
 			atomicAdd(P.stats + 2, c);
+			atomicAdd(P.stats + 3, (unsigned long long)cPass);
+			atomicAdd(P.stats + 4, (unsigned long long)cLive);
+			atomicAdd(P.stats + 5, (unsigned long long)cNodePass);
+			atomicAdd(P.stats + 6, (unsigned long long)cLeafPass);
+			atomicAdd(P.stats + 7, (unsigned long long)cRefill);
 		}
 	}
 	// the last block to finish rewinds the ticket counter for the next launch
@@ -598,24 +606,22 @@ __global__ void lrh_reset_kernel(uint32_t *p, uint32_t n) {
 // ---------------------------------------------------------------------------
 typedef void (*KernelFn)(LrhTraceParams);
 
-#define LRH_K(L, T, A, C, B, W) (const void *)&lrh_trace_kernel<L, T, A, C, B, W>
-#define LRH_KW(L, T, A, C, B) {LRH_K(L, T, A, C, B, false), LRH_K(L, T, A, C, B, true)}
-static const void *kernelPtr(int layout, bool two, bool any, bool count, bool batch, bool pl) {
-	static const void *plain[2][2][2][2][2] = {
-		{{{LRH_KW(0, false, false, false, false), LRH_KW(0, false, false, true, false)},
-		  {LRH_KW(0, false, true, false, false), LRH_KW(0, false, true, true, false)}},
-		 {{LRH_KW(0, true, false, false, false), LRH_KW(0, true, false, true, false)},
-		  {LRH_KW(0, true, true, false, false), LRH_KW(0, true, true, true, false)}}},
-		{{{LRH_KW(1, false, false, false, false), LRH_KW(1, false, false, true, false)},
-		  {LRH_KW(1, false, true, false, false), LRH_KW(1, false, true, true, false)}},
-		 {{LRH_KW(1, true, false, false, false), LRH_KW(1, true, false, true, false)},
-		  {LRH_KW(1, true, true, false, false), LRH_KW(1, true, true, true, false)}}}};
+#define LRH_K(L, T, A, C, B) (const void *)&lrh_trace_kernel<L, T, A, C, B>
+static const void *kernelPtr(int layout, bool two, bool any, bool count, bool batch) {
+	static const void *plain[2][2][2][2] = {
+		{{{LRH_K(0, false, false, false, false), LRH_K(0, false, false, true, false)},
+		  {LRH_K(0, false, true, false, false), LRH_K(0, false, true, true, false)}},
+		 {{LRH_K(0, true, false, false, false), LRH_K(0, true, false, true, false)},
+		  {LRH_K(0, true, true, false, false), LRH_K(0, true, true, true, false)}}},
+		{{{LRH_K(1, false, false, false, false), LRH_K(1, false, false, true, false)},
+		  {LRH_K(1, false, true, false, false), LRH_K(1, false, true, true, false)}},
+		 {{LRH_K(1, true, false, false, false), LRH_K(1, true, false, true, false)},
+		  {LRH_K(1, true, true, false, false), LRH_K(1, true, true, true, false)}}}};
 	// batched: the query is per item, so ANY is unused; COUNT is not offered
-	static const void *batched[2][2][2] = {
-		{LRH_KW(0, false, false, false, true), LRH_KW(0, true, false, false, true)},
-		{LRH_KW(1, false, false, false, true), LRH_KW(1, true, false, false, true)}};
-	if (batch) return batched[layout ? 1 : 0][two ? 1 : 0][pl ? 1 : 0];
-	return plain[layout ? 1 : 0][two ? 1 : 0][any ? 1 : 0][count ? 1 : 0][pl ? 1 : 0];
+	static const void *batched[2][2] = {{LRH_K(0, false, false, false, true), LRH_K(0, true, false, false, true)},
+			{LRH_K(1, false, false, false, true), LRH_K(1, true, false, false, true)}};
+	if (batch) return batched[layout ? 1 : 0][two ? 1 : 0];
+	return plain[layout ? 1 : 0][two ? 1 : 0][any ? 1 : 0][count ? 1 : 0];
 }
 
 hipError_t lrhLaunchReset(uint32_t *p, uint32_t n, hipStream_t s) {
@@ -623,14 +629,14 @@ hipError_t lrhLaunchReset(uint32_t *p, uint32_t n, hipStream_t s) {
 	return hipGetLastError();
 }
 
-hipError_t lrhOccupancy(int layout, bool two, bool any, bool batch, bool pl, int block, size_t lds,
+hipError_t lrhOccupancy(int layout, bool two, bool any, bool batch, int block, size_t lds,
 		int *blocksPerCU) {
-	return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kernelPtr(layout, two, any, false, batch, pl),
+	return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kernelPtr(layout, two, any, false, batch),
 			block, lds);
 }
 
-hipError_t lrhLaunchTrace(int layout, bool two, bool any, bool count, bool batch, bool pl, dim3 grid, dim3 block,
+hipError_t lrhLaunchTrace(int layout, bool two, bool any, bool count, bool batch, dim3 grid, dim3 block,
 		size_t lds, hipStream_t s, const LrhTraceParams &P) {
 	void *args[] = {(void *)&P};
-	return hipLaunchKernel(kernelPtr(layout, two, any, count, batch, pl), grid, block, args, lds, s);
+	return hipLaunchKernel(kernelPtr(layout, two, any, count, batch), grid, block, args, lds, s);
 }

@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(_HERE, "liblrhip.so")
 LRH_OK = 0
 LRH_H2D, LRH_D2H, LRH_D2D = 1, 2, 3
 LRH_QUERY_CLOSEST, LRH_QUERY_ANY = 0, 1
+TRACE_STATS = 8  # LRH_TRACE_STATS
 ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "ERANGE", -5: "ENODEV", -6: "EDEVICE"}
 
 
@@ -54,8 +55,8 @@ class LrhSceneInfo(ctypes.Structure):
 
 class LrhTraceOpts(ctypes.Structure):
     _fields_ = [("blockSize", ctypes.c_uint), ("ldsStack", ctypes.c_uint), ("blocksPerCU", ctypes.c_uint),
-                ("refillThreshold", ctypes.c_uint), ("loop", ctypes.c_uint),
-                ("leafBatch", ctypes.c_uint)]
+                ("refillThreshold", ctypes.c_uint),
+                ("leafTris", ctypes.c_uint)]
 
 
 class LrhCamera(ctypes.Structure):
@@ -265,10 +266,10 @@ def _build_opts(layout=0, max_leaf=4, threads=0, trav_cost=1.0):
     return o
 
 
-def trace_opts(block=0, lds=0, blocks_per_cu=0, refill=0, loop=0, leaf_batch=0):
+def trace_opts(block=0, lds=0, blocks_per_cu=0, refill=0, leaf_tris=0):
     o = LrhTraceOpts()
-    o.blockSize, o.ldsStack, o.blocksPerCU, o.refillThreshold, o.loop = block, lds, blocks_per_cu, refill, loop
-    o.leafBatch = leaf_batch
+    o.blockSize, o.ldsStack, o.blocksPerCU, o.refillThreshold = block, lds, blocks_per_cu, refill
+    o.leafTris = leaf_tris
     return o
 
 
@@ -471,10 +472,16 @@ class Device:
 
     def trace_stats(self, scene, rays_buf, hits_buf, count, query=0, opts=None, stream=None):
         """(node visits, primitive tests, rays) of one trace (counting kernel variant)."""
-        out = (ctypes.c_ulonglong * 3)()
+        c = self.trace_counters(scene, rays_buf, hits_buf, count, query, opts, stream)
+        return c["node_visits"], c["prim_tests"], c["rays"]
+
+    def trace_counters(self, scene, rays_buf, hits_buf, count, query=0, opts=None, stream=None):
+        """Every counter of lrh_trace_stats, by name."""
+        out = (ctypes.c_ulonglong * TRACE_STATS)()
         check(lib().lrh_trace_stats(scene.h, rays_buf.ptr, hits_buf.ptr, count, query,
                                     ctypes.byref(opts) if opts else None, stream, out))
-        return out[0], out[1], out[2]
+        return dict(zip(("node_visits", "prim_tests", "rays", "passes", "live_lane_passes", "node_passes",
+                         "leaf_passes", "refills"), (int(x) for x in out)))
 
     def trace_batch(self, items, opts=None, stream=None):
         """One persistent launch over items (scene, rays, hits, count, query)."""
