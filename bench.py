@@ -53,10 +53,50 @@ def build_workload(args, dev):
     return meshes, scene, info, bounce, ntris
 
 
+def host_info():
+    """CPU model, sockets, cores, threads and frequency governor of this host."""
+    info = {"threads": os.cpu_count()}
+    try:
+        with open("/proc/cpuinfo") as f:
+            txt = f.read()
+        models = [ln.split(":", 1)[1].strip() for ln in txt.splitlines() if ln.startswith("model name")]
+        phys = {ln.split(":", 1)[1].strip() for ln in txt.splitlines() if ln.startswith("physical id")}
+        cores = {ln.split(":", 1)[1].strip() for ln in txt.splitlines() if ln.startswith("core id")}
+        info.update(model=models[0] if models else None, sockets=len(phys) or None,
+                    cores_per_socket=len(cores) or None)
+    except OSError:
+        pass
+    try:
+        with open("/sys/devices/system/cpu/cpu0/cpufreq/scaling_governor") as f:
+            info["governor"] = f.read().strip()
+    except OSError:
+        info["governor"] = None
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    return info
+
+
+def _timed_passes(ob, sample, threads, target_s, min_passes=5):
+    ob.intersect(sample, threads=threads)  # warm-up
+    rates, spent = [], 0.0
+    while (spent < target_s and len(rates) < 50) or len(rates) < min_passes:
+        t0 = time.time()
+        ob.intersect(sample, threads=threads)
+        dt = time.time() - t0
+        spent += dt
+        rates.append(sample.shape[0] / dt / 1e6)
+    return float(np.median(rates)), len(rates), spent
+
+
 def cpu_baseline(meshes, rays, target_s):
     """Oracle BVHAccel::Intersect on this host's cores: a warm-up pass, then
-    repeated passes over a bounded sample until ~target_s seconds of work;
-    the median pass rate is reported."""
+    repeated passes over a bounded sample until ~target_s seconds of work
+    (at least 5); the median pass rate is reported.  The same sample is also
+    traced once with the oracle's counters, for the reference-layout bytes
+    per ray (32-byte BVHArrayNode visits, 36-byte triangle gathers), and
+    timed over a CLASSIC-built tree for comparison."""
     import oracle  # the checker, used here only as the timed CPU baseline
     threads = os.cpu_count() or 1
     t0 = time.time()
@@ -66,19 +106,31 @@ def cpu_baseline(meshes, rays, target_s):
     t0 = time.time()
     ob.intersect(probe, threads=threads)
     rate = probe.shape[0] / max(time.time() - t0, 1e-6)
-    n = int(min(rays.shape[0], max(1 << 15, rate * target_s / 5)))
+    n = int(min(rays.shape[0], max(1 << 15, rate * target_s / 6)))
     sample = rays[:n]
-    rates, spent = [], 0.0
-    while (spent < target_s and len(rates) < 50) or len(rates) < 3:
+    value, passes, spent = _timed_passes(ob, sample, threads, target_s)
+    _, (visits, tests) = ob.intersect(sample, threads=threads, counters=True)
+    # every visited BVHArrayNode (inner or leaf) is 32 bytes; a leaf adds its
+    # three 12-byte vertex gathers
+    ref_bytes = 32.0 * (visits + tests) / n + 36.0 * tests / n + RAY_BYTES + HIT_BYTES
+    del ob
+    classic = None
+    try:
         t0 = time.time()
-        ob.intersect(sample, threads=threads)
-        dt = time.time() - t0
-        spent += dt
-        rates.append(n / dt / 1e6)
-    return {"value": round(float(np.median(rates)), 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"first {n} of the {rays.shape[0]} bounce rays x {len(rates)} passes ({spent:.1f}s), "
-                      f"oracle BVHAccel::Intersect over a 4-ary binned-SAH tree in the 32-byte BVHArrayNode "
-                      f"layout, {threads} std::threads, median pass"}
+        oc = oracle.OracleBVH(meshes, builder="CLASSIC", tree_type=4, threads=threads)
+        log(f"cpu CLASSIC tree built in {time.time() - t0:.1f}s")
+        classic, _, _ = _timed_passes(oc, sample, threads, target_s / 3, min_passes=3)
+        del oc
+    except Exception as e:  # the comparison line is optional; the baseline above is not
+        log(f"CLASSIC comparison skipped: {e}")
+    return {"value": round(value, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} of the {rays.shape[0]} bounce rays x {passes} passes ({spent:.1f}s) after a "
+                      f"warm-up, oracle BVHAccel::Intersect over a 4-ary binned-SAH tree in the 32-byte "
+                      f"BVHArrayNode layout, {threads} std::threads, median pass",
+            "classic_tree_value": round(classic, 3) if classic else None,
+            "reference_layout_bytes_per_ray": round(ref_bytes, 1),
+            "reference_per_ray": {"inner_node_visits": round(visits / n, 3), "leaf_visits": round(tests / n, 3)},
+            "host": host_info()}
 
 
 def run_batched(args, dev):
@@ -255,7 +307,9 @@ def main():
                      "bytes_per_ray": round(b_ray, 1),
                      "per_ray": {"node_visits": round(nodes / counted, 3),
                                  "tri_tests": round(prims / counted, 3)},
-                     "kernel_ms": round(kernel_ms, 4)},
+                     "kernel_ms": round(kernel_ms, 4),
+                     "compulsory_bytes_per_ray": RAY_BYTES + HIT_BYTES,
+                     "compulsory_ceiling_mrays": round(HBM_PEAK_GBS * 1e9 / (RAY_BYTES + HIT_BYTES) / 1e6, 1)},
         "cpu_baseline": cpu,
     }
     if cpu:

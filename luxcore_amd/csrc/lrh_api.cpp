@@ -821,7 +821,7 @@ LaunchShape chooseShape(lrh_device *dev, unsigned layout, bool two, bool any, bo
 	L.leafTris = (o && o->leafTris) ? o->leafTris : 8u;
 	if (L.leafTris > 8) throw Error(LRH_EINVAL, "leafTris must be 0..8");
 	L.ldsBytes = (size_t)L.lds * L.block * sizeof(uint2) + extraLds;
-	if (L.ldsBytes > 65536) throw Error(LRH_ERANGE, "LDS stack and item table exceed 64 KiB per block");
+	if (L.ldsBytes > 65536) throw Error(LRH_ERANGE, "LDS stack, node cache and item table exceed 64 KiB per block");
 	int perCU = (o && o->blocksPerCU) ? (int)o->blocksPerCU : 0;
 	if (!perCU) {
 		const uint64_t key = ((uint64_t)layout << 40) | ((uint64_t)two << 39) | ((uint64_t)any << 38) |
@@ -890,7 +890,13 @@ int traceImpl(lrh_scene *s, const void *rays, void *hits, unsigned int count, un
 		dev->activate();
 		hipStream_t st = stream ? (hipStream_t)stream : dev->stream;
 		const bool any = query == LRH_QUERY_ANY;
-		const LaunchShape L = chooseShape(dev, s->layout, s->twoLevel, any, false, count, s->maxDepth, 0, opts);
+		// top nodes held in LDS (stored first by the builder, breadth first)
+		uint32_t cacheN = 21u;  // three full levels of a 4-wide tree (1.3 KiB)
+		if (opts && opts->nodeCache) cacheN = opts->nodeCache == LRH_NODE_CACHE_NONE ? 0u : opts->nodeCache;
+		const size_t nodeSz = s->layout == 0 ? sizeof(lrh_node4q) : sizeof(lrh_node4f);
+		cacheN = (uint32_t)std::min<size_t>(cacheN, s->nodes.bytes / nodeSz);  // within the node allocation
+		const size_t cacheBytes = (size_t)cacheN * nodeSz;
+		const LaunchShape L = chooseShape(dev, s->layout, s->twoLevel, any, false, count, s->maxDepth, cacheBytes, opts);
 		const unsigned nLanes = L.blocks * L.block;
 		lrh_device::Workspace &w = workspace(dev, st, (size_t)nLanes * L.spillDepth * sizeof(uint2));
 		LrhTraceParams P;
@@ -906,6 +912,7 @@ int traceImpl(lrh_scene *s, const void *rays, void *hits, unsigned int count, un
 		P.ldsDepth = L.lds;
 		P.refill = L.refill;
 		P.leafTris = L.leafTris;
+		P.cacheNodes = cacheN;
 		P.error = dev->dError;
 		P.stats = w.stats;
 		if (statsOut) HIPCHK(hipMemsetAsync(w.stats, 0, LRH_TRACE_STATS * sizeof(unsigned long long), st));

@@ -746,6 +746,30 @@ void buildWideBVH(const std::vector<Box> &boxes, const BuildParams &p, uint32_t 
 	wb.run();
 	const size_t nNodes = wb.count.load();
 	if ((uint64_t)nodeBase + nNodes >= 0x80000000ull) throw Error(-4, "too many nodes");
+	// top of the tree first, breadth first: node i < K is at depth <= that of
+	// node K, so a prefix of the array holds the most visited nodes (the
+	// trace kernel keeps such a prefix in LDS); the rest keep build order
+	std::vector<uint32_t> newIndex(nNodes, UINT32_MAX), oldOf;
+	oldOf.reserve(nNodes);
+	{
+		const size_t topMax = std::min<size_t>(nNodes, LRH_TOP_NODES);
+		oldOf.push_back(0);
+		newIndex[0] = 0;
+		for (size_t q = 0; q < oldOf.size() && oldOf.size() < topMax; ++q) {
+			const WNode &w = wb.nodes[oldOf[q]];
+			for (uint32_t c = 0; c < w.n && oldOf.size() < topMax; ++c) {
+				const uint32_t ch = w.child[c];
+				if (ch & LRH_CHILD_LEAF) continue;
+				newIndex[ch] = (uint32_t)oldOf.size();
+				oldOf.push_back(ch);
+			}
+		}
+		for (size_t i = 0; i < nNodes; ++i)
+			if (newIndex[i] == UINT32_MAX) {
+				newIndex[i] = (uint32_t)oldOf.size();
+				oldOf.push_back((uint32_t)i);
+			}
+	}
 	// relocate child references
 	auto reloc = [&](uint32_t c) -> uint32_t {
 		if (c == LRH_CHILD_EMPTY) return c;
@@ -754,7 +778,7 @@ void buildWideBVH(const std::vector<Box> &boxes, const BuildParams &p, uint32_t 
 			const uint32_t start = (c & LRH_LEAF_START_MASK) + primBase;
 			return LRH_CHILD_LEAF | (cntm1 << LRH_LEAF_START_BITS) | start;
 		}
-		return c + nodeBase;
+		return newIndex[c] + nodeBase;
 	};
 	const unsigned nt = p.nThreads ? p.nThreads : hardwareThreads();
 	for (size_t i = 0; i < nNodes; ++i)
@@ -763,10 +787,10 @@ void buildWideBVH(const std::vector<Box> &boxes, const BuildParams &p, uint32_t 
 	out->fnodes.clear();
 	if (p.layout == 0) {
 		out->qnodes.resize(nNodes);
-		parallelFor(nNodes, nt, [&](size_t i) { quantize(wb.nodes[i], &out->qnodes[i]); });
+		parallelFor(nNodes, nt, [&](size_t i) { quantize(wb.nodes[oldOf[i]], &out->qnodes[i]); });
 	} else {
 		out->fnodes.resize(nNodes);
-		parallelFor(nNodes, nt, [&](size_t i) { toFloatNode(wb.nodes[i], &out->fnodes[i]); });
+		parallelFor(nNodes, nt, [&](size_t i) { toFloatNode(wb.nodes[oldOf[i]], &out->fnodes[i]); });
 	}
 	out->order = std::move(wb.ids);
 	out->root = nodeBase;

@@ -26,6 +26,7 @@
 #define LRH_CHILD_EMPTY 0xffffffffu
 #define LRH_CHILD_LEAF 0x80000000u
 #define LRH_LEAF_MAX 8u
+#define LRH_TOP_NODES 4096u   /* the builder stores this many top nodes breadth first */
 #define LRH_LEAF_START_BITS 28u
 #define LRH_LEAF_START_MASK 0x0fffffffu
 #define LRH_MAX_LEAF_START LRH_LEAF_START_MASK
