@@ -90,12 +90,20 @@ typedef struct {
 } lrh_object;
 
 /* Build options (defaults when the pointer is NULL or a field is 0).
- * layout: 0 = 64-byte quantized 4-wide nodes (default), 1 = 128-byte fp32. */
+ * layout: 0 = 64-byte quantized 4-wide nodes (default), 1 = 128-byte fp32.
+ * builder: LRH_BUILD_HOST_SAH (default) builds on the host (binned SAH, the
+ * stand-in for the reference's EMBREE_BINNED_SAH, bvhaccel.cpp:144-156);
+ * LRH_BUILD_DEVICE_LBVH builds on the GPU (Morton order + radix tree,
+ * collapsed to the same 4-wide records; single-level scenes with a device);
+ * it is switched off in this build and returns LRH_EINVAL. */
+#define LRH_BUILD_HOST_SAH 0u
+#define LRH_BUILD_DEVICE_LBVH 1u
 typedef struct {
 	unsigned int layout;
 	unsigned int maxLeafSize;  /* 1..8, default 4 */
 	unsigned int nThreads;     /* host build threads, default = hardware threads */
 	float travCost;            /* SAH traversal cost relative to one triangle test, default 1.0 */
+	unsigned int builder;      /* LRH_BUILD_HOST_SAH or LRH_BUILD_DEVICE_LBVH */
 } lrh_build_opts;
 
 typedef struct {

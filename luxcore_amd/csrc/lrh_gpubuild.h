@@ -1,0 +1,48 @@
+/*
+ * Device-side BVH build (linear BVH): Morton order of the triangle boxes,
+ * binary radix tree, bottom-up boxes, then a breadth-first collapse into the
+ * same 4-wide node records the host builder writes.  It stands in for the
+ * host build of BVHAccel::Init (src/luxrays/accelerators/bvhaccel.cpp:72-168)
+ * when the scene is built for a device.
+ */
+#ifndef LRH_GPUBUILD_H
+#define LRH_GPUBUILD_H
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+struct LrhGpuAlloc {
+	void *(*alloc)(void *ctx, size_t bytes);  // throws on failure
+	void (*release)(void *ctx, void *p, size_t bytes);
+	void *ctx;
+};
+
+struct LrhGpuBuildIn {
+	const float *verts;          // device, all meshes' vertices concatenated (3 floats each)
+	const uint32_t *tris;        // device, all meshes' triangles (3 mesh-local vertex indices each)
+	const uint32_t *meshTriBase; // host, nMeshes + 1 prefix sums of triangle counts
+	const uint32_t *meshVertBase;// host, nMeshes + 1 prefix sums of vertex counts
+	uint32_t nMeshes;
+	uint32_t nTris;
+	uint32_t maxLeaf;            // 1..8
+	uint32_t layout;             // 0 quantized 64 B nodes, 1 fp32 128 B nodes
+};
+
+struct LrhGpuBuildOut {
+	void *nodes = nullptr;       // device node records, breadth first from the root
+	size_t nodeBytes = 0;
+	uint32_t nodeCount = 0;
+	void *tris = nullptr;        // device lrh_tri records in leaf order
+	size_t triBytes = 0;
+	uint32_t *invPerm = nullptr; // device, global triangle id -> record index
+	size_t invPermBytes = 0;
+	uint32_t root = 0xffffffffu;
+	uint32_t maxDepth = 0;
+	double kernelMs = 0.0;
+};
+
+// Builds on `stream` and waits for it.  Throws std::runtime_error on HIP errors.
+void lrhGpuBuildBVH(const LrhGpuBuildIn &in, const LrhGpuAlloc &a, hipStream_t stream, LrhGpuBuildOut *out);
+
+#endif

@@ -43,7 +43,10 @@ class LrhObject(ctypes.Structure):
 
 class LrhBuildOpts(ctypes.Structure):
     _fields_ = [("layout", ctypes.c_uint), ("maxLeafSize", ctypes.c_uint), ("nThreads", ctypes.c_uint),
-                ("travCost", ctypes.c_float)]
+                ("travCost", ctypes.c_float), ("builder", ctypes.c_uint)]
+
+
+BUILD_HOST_SAH, BUILD_DEVICE_LBVH = 0, 1
 
 
 class LrhSceneInfo(ctypes.Structure):
@@ -257,8 +260,9 @@ def _object_structs(scene):
     return arr, keep
 
 
-def _build_opts(layout=0, max_leaf=4, threads=0, trav_cost=1.0):
+def _build_opts(layout=0, max_leaf=4, threads=0, trav_cost=1.0, builder=BUILD_HOST_SAH):
     o = LrhBuildOpts()
+    o.builder = builder
     o.layout = layout
     o.maxLeafSize = max_leaf
     o.nThreads = threads
@@ -451,9 +455,10 @@ class Device:
         return lib().lrh_used_memory(self.h)
 
     # -- scenes --
-    def scene_bvh(self, meshes, layout=0, max_leaf=4, threads=0, trav_cost=1.0):
+    def scene_bvh(self, meshes, layout=0, max_leaf=4, threads=0, trav_cost=1.0, builder=BUILD_HOST_SAH):
+        """lrh_scene_create_bvh; builder=BUILD_DEVICE_LBVH builds the tree on the GPU."""
         arr, keep = _mesh_structs(meshes)
-        opts = _build_opts(layout, max_leaf, threads, trav_cost)
+        opts = _build_opts(layout, max_leaf, threads, trav_cost, builder)
         p = ctypes.c_void_p()
         check(lib().lrh_scene_create_bvh(self.h, arr, len(meshes), ctypes.byref(opts), ctypes.byref(p)))
         return Scene(self, p.value, False)

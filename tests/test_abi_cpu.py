@@ -152,3 +152,14 @@ def test_host_builder_two_level():
         dec_ok = False
     assert dec_ok
     assert tris.shape[0] == ntri
+
+
+def test_device_builder_needs_a_device():
+    import ctypes
+    arr, keep = D._mesh_structs([scenes.box_mesh()])
+    opts = D._build_opts(builder=D.BUILD_DEVICE_LBVH)
+    p = ctypes.c_void_p()
+    assert D.lib().lrh_scene_create_bvh(None, arr, 1, ctypes.byref(opts), ctypes.byref(p)) == -1
+    assert b"needs a device" in D.lib().lrh_last_error()
+    opts.builder = 7
+    assert D.lib().lrh_scene_create_bvh(None, arr, 1, ctypes.byref(opts), ctypes.byref(p)) == -1

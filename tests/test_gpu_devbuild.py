@@ -1,0 +1,130 @@
+"""Device-built trees (LRH_BUILD_DEVICE_LBVH): the same 4-wide records as the
+host builder, a well-formed tree over every triangle, deterministic, and
+traces equal to the oracle to the parity bar (closest hit does not depend on
+the tree outside exact ties and the edge band)."""
+import numpy as np
+import pytest
+
+import oracle
+from luxcore_amd import device as D
+from luxcore_amd import scenes
+from luxcore_amd.raytypes import HIT_DTYPE, NULL_INDEX, machine_epsilon
+from tests.parity import assert_parity, compare
+from tests.test_abi_cpu import NODE_F, NODE_Q, TRI, check_tree
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skip(
+    reason="device builder switched off: its first GPU run went silent inside the build and the cause is not shown")]
+
+
+def _case(name, **kw):
+    print(f"[case] {name} {kw}", flush=True)
+
+
+def _leaf_boxes(meshes, tris):
+    lo = np.empty((tris.shape[0], 3), np.float32)
+    hi = np.empty_like(lo)
+    for mi, m in enumerate(meshes):
+        sel = tris["mesh"] == mi
+        if not sel.any():
+            continue
+        v = m.verts[m.tris[tris["tri"][sel]]]
+        lo[sel] = v.min(axis=1)
+        hi[sel] = v.max(axis=1)
+    e = np.maximum(machine_epsilon(lo).max(1), machine_epsilon(hi).max(1))
+    return (lo - e[:, None]).astype(np.float32), (hi + e[:, None]).astype(np.float32)
+
+
+def _records_sorted(tris):
+    order = np.lexsort((tris["tri"], tris["mesh"]))
+    return tris[order]
+
+
+@pytest.mark.parametrize("layout,max_leaf", [(0, 4), (0, 1), (0, 8), (1, 4)])
+def test_device_build_tree_and_records(gpu_device, layout, max_leaf):
+    _case("devbuild_tree", layout=layout, max_leaf=max_leaf)
+    meshes = scenes.plain_scene(seed=1, n_random=20000)
+    dsc = gpu_device.scene_bvh(meshes, layout=layout, max_leaf=max_leaf, builder=D.BUILD_DEVICE_LBVH)
+    info = dsc.info()
+    nb, pb = dsc.download()
+    nodes = nb.view(NODE_Q if layout == 0 else NODE_F)
+    tris = pb.view(TRI)
+    ntri = sum(m.ntris for m in meshes)
+    assert tris.shape[0] == ntri and nodes.shape[0] == info["nodeCount"]
+    # records: the host builder's records, in another order
+    hsc = D.host_scene_bvh(meshes, layout=layout, max_leaf=max_leaf)
+    _, hpb = hsc.download()
+    assert _records_sorted(tris).tobytes() == _records_sorted(hpb.view(TRI)).tobytes()
+    # every triangle once; every child box contains its triangles' epsilon-expanded boxes
+    lo, hi = _leaf_boxes(meshes, tris)
+
+    def boxes_of_child(s, c):
+        return lo[s:s + c].min(0), hi[s:s + c].max(0)
+    check_tree(nodes, boxes_of_child, 0, nodes.shape[0], ntri, layout)
+    # deterministic
+    dsc2 = gpu_device.scene_bvh(meshes, layout=layout, max_leaf=max_leaf, builder=D.BUILD_DEVICE_LBVH)
+    nb2, pb2 = dsc2.download()
+    assert nb.tobytes() == nb2.tobytes() and pb.tobytes() == pb2.tobytes()
+    dsc.close()
+    dsc2.close()
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_device_build_trace_parity(gpu_device, layout):
+    _case("devbuild_parity", layout=layout)
+    meshes = scenes.plain_scene(seed=1, n_random=20000)
+    rays = scenes.plain_rays(1 << 18, seed=21)
+    rays["flags"][::11] = 1
+    dsc = gpu_device.scene_bvh(meshes, layout=layout, builder=D.BUILD_DEVICE_LBVH)
+    init = np.zeros(rays.shape[0], HIT_DTYPE)
+    init["t"] = 3.25
+    got = gpu_device.trace_numpy(dsc, rays, hits_init=init)
+    live = (rays["flags"] & 1) == 0
+    assert np.all(got["t"][~live] == 3.25)
+    ref = oracle.OracleBVH(meshes).intersect(rays[live])
+    st = compare(ref, got[live])
+    print(st, flush=True)
+    assert_parity(st)
+    anyh = gpu_device.trace_numpy(dsc, rays[live], query=D.LRH_QUERY_ANY)
+    assert np.array_equal(anyh["meshIndex"] != NULL_INDEX, ref["meshIndex"] != NULL_INDEX)
+    dsc.close()
+
+
+def test_device_build_small_and_ragged(gpu_device):
+    _case("devbuild_small")
+    tri = scenes.Mesh(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), np.array([[0, 1, 2]], np.uint32))
+    empty = scenes.Mesh(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32))
+    box = scenes.box_mesh()
+    rng = np.random.default_rng(5)
+    for meshes in ([tri], [empty, tri], [box], [empty, box, empty, tri], [scenes.random_triangles(9, (0, 0, 0), (10, 10, 10), rng)]):
+        for max_leaf in (1, 4, 8):
+            dsc = gpu_device.scene_bvh(meshes, max_leaf=max_leaf, builder=D.BUILD_DEVICE_LBVH)
+            rays = scenes.random_rays(4096, (-1, -1, -1), (11, 11, 11), rng)
+            got = gpu_device.trace_numpy(dsc, rays)
+            ref = oracle.bruteforce(meshes, rays)
+            assert_parity(compare(ref, got))
+            dsc.close()
+    dsc = gpu_device.scene_bvh([empty], builder=D.BUILD_DEVICE_LBVH)
+    got = gpu_device.trace_numpy(dsc, scenes.plain_rays(256, seed=3))
+    assert np.all(got["meshIndex"] == NULL_INDEX)
+    dsc.close()
+
+
+def test_device_build_heightfield_equals_host_tree(gpu_device):
+    """Full 26M-triangle scene: the device-built tree's closest hits equal the
+    host-built tree's (same rays) outside exact ties and the edge band."""
+    _case("devbuild_heightfield")
+    meshes = scenes.heightfield_scene()
+    hsc = gpu_device.scene_bvh(meshes)
+    dsc = gpu_device.scene_bvh(meshes, builder=D.BUILD_DEVICE_LBVH)
+    print("host build", hsc.info()["buildSeconds"], "device build", dsc.info()["buildSeconds"],
+          "upload", dsc.info()["uploadSeconds"], flush=True)
+    prim = scenes.pinhole_rays(960, 540)
+    h = gpu_device.trace_numpy(hsc, prim)
+    bounce = scenes.bounce_rays(meshes, prim, h)
+    a = gpu_device.trace_numpy(hsc, bounce)
+    b = gpu_device.trace_numpy(dsc, bounce)
+    st = compare(a, b)
+    print(st, flush=True)
+    assert_parity(st)
+    hsc.close()
+    dsc.close()
