@@ -128,10 +128,7 @@ typedef struct {
 	unsigned int refillThreshold;  /* refill a wave when fewer than this many lanes are live; 0 = default */
 	unsigned int leafTris;         /* triangles a lane tests per traversal pass; the rest of its leaf waits
 	                                  for the next pass; 0 = default (the whole leaf) */
-	unsigned int nodeCache;        /* top tree nodes each block keeps in LDS (lrh_trace / lrh_trace_ex);
-	                                  0 = default 21, LRH_NODE_CACHE_NONE = none */
 } lrh_trace_opts;
-#define LRH_NODE_CACHE_NONE 0xffffffffu
 
 /* One ray buffer of a batched launch (lrh_trace_batch). */
 typedef struct {

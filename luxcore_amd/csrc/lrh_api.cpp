@@ -911,7 +911,7 @@ LaunchShape chooseShape(lrh_device *dev, unsigned layout, bool two, bool any, bo
 	L.lds = (o && o->ldsStack) ? o->ldsStack : 8;
 	if (L.lds < 4 || L.lds > 64 || pow2floor(L.lds) != L.lds)
 		throw Error(LRH_EINVAL, "ldsStack must be a power of two in 4..64");
-	L.refill = (o && o->refillThreshold) ? std::min(64u, o->refillThreshold) : 12;
+	L.refill = (o && o->refillThreshold) ? std::min(64u, o->refillThreshold) : 8;
 	L.leafTris = (o && o->leafTris) ? o->leafTris : 8u;
 	if (L.leafTris > 8) throw Error(LRH_EINVAL, "leafTris must be 0..8");
 	L.ldsBytes = (size_t)L.lds * L.block * sizeof(uint2) + extraLds;
@@ -984,13 +984,7 @@ int traceImpl(lrh_scene *s, const void *rays, void *hits, unsigned int count, un
 		dev->activate();
 		hipStream_t st = stream ? (hipStream_t)stream : dev->stream;
 		const bool any = query == LRH_QUERY_ANY;
-		// top nodes held in LDS (stored first by the builder, breadth first)
-		uint32_t cacheN = 21u;  // three full levels of a 4-wide tree (1.3 KiB)
-		if (opts && opts->nodeCache) cacheN = opts->nodeCache == LRH_NODE_CACHE_NONE ? 0u : opts->nodeCache;
-		const size_t nodeSz = s->layout == 0 ? sizeof(lrh_node4q) : sizeof(lrh_node4f);
-		cacheN = (uint32_t)std::min<size_t>(cacheN, s->nodes.bytes / nodeSz);  // within the node allocation
-		const size_t cacheBytes = (size_t)cacheN * nodeSz;
-		const LaunchShape L = chooseShape(dev, s->layout, s->twoLevel, any, false, count, s->maxDepth, cacheBytes, opts);
+		const LaunchShape L = chooseShape(dev, s->layout, s->twoLevel, any, false, count, s->maxDepth, 0, opts);
 		const unsigned nLanes = L.blocks * L.block;
 		lrh_device::Workspace &w = workspace(dev, st, (size_t)nLanes * L.spillDepth * sizeof(uint2));
 		LrhTraceParams P;
@@ -1006,7 +1000,6 @@ int traceImpl(lrh_scene *s, const void *rays, void *hits, unsigned int count, un
 		P.ldsDepth = L.lds;
 		P.refill = L.refill;
 		P.leafTris = L.leafTris;
-		P.cacheNodes = cacheN;
 		P.error = dev->dError;
 		P.stats = w.stats;
 		if (statsOut) HIPCHK(hipMemsetAsync(w.stats, 0, LRH_TRACE_STATS * sizeof(unsigned long long), st));

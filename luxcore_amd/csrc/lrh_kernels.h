@@ -46,7 +46,6 @@ struct LrhTraceParams {
 	uint32_t ldsDepth;              /* power of two */
 	uint32_t refill;                /* refill a wave when fewer lanes than this are live */
 	uint32_t leafTris;              /* triangles a lane tests per pass (1..8); the rest of a leaf waits a pass */
-	uint32_t cacheNodes;            /* top nodes copied to LDS after the stack (plain traces only) */
 	uint32_t *error;                /* bit 0 stack overflow, bit 1 step guard */
 	unsigned long long *stats;      /* counting variant: LRH_TRACE_STATS counters */
 	const LrhBatchDesc *items;      /* batched: item descriptors */

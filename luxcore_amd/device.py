@@ -59,7 +59,7 @@ class LrhSceneInfo(ctypes.Structure):
 class LrhTraceOpts(ctypes.Structure):
     _fields_ = [("blockSize", ctypes.c_uint), ("ldsStack", ctypes.c_uint), ("blocksPerCU", ctypes.c_uint),
                 ("refillThreshold", ctypes.c_uint),
-                ("leafTris", ctypes.c_uint), ("nodeCache", ctypes.c_uint)]
+                ("leafTris", ctypes.c_uint)]
 
 
 class LrhCamera(ctypes.Structure):
@@ -270,14 +270,11 @@ def _build_opts(layout=0, max_leaf=4, threads=0, trav_cost=1.0, builder=BUILD_HO
     return o
 
 
-NODE_CACHE_NONE = 0xFFFFFFFF
-
-
-def trace_opts(block=0, lds=0, blocks_per_cu=0, refill=0, leaf_tris=0, node_cache=0):
-    """lrh_trace_opts; zeros select the defaults (node_cache=NODE_CACHE_NONE: no LDS node cache)."""
+def trace_opts(block=0, lds=0, blocks_per_cu=0, refill=0, leaf_tris=0):
+    """lrh_trace_opts; zeros select the defaults."""
     o = LrhTraceOpts()
     o.blockSize, o.ldsStack, o.blocksPerCU, o.refillThreshold = block, lds, blocks_per_cu, refill
-    o.leafTris, o.nodeCache = leaf_tris, node_cache
+    o.leafTris = leaf_tris
     return o
 
 
