@@ -407,7 +407,12 @@ __device__ __forceinline__ void traceBody(const LrhTraceParams &P) {
 				setupRay(r);
 				hb1 = 0.f; hb2 = 0.f;
 				hTri = LRH_NULL_INDEX; hMesh = LRH_NULL_INDEX;
-				cur = v.root;
+				// a NaN origin or direction component makes every triangle test
+				// fail (the divisor or the distance is NaN), while every box
+				// would be entered: the ray misses without a traversal
+				const bool nanRay = isnan(r.ox) || isnan(r.oy) || isnan(r.oz) || isnan(r.dx) || isnan(r.dy) ||
+						isnan(r.dz);
+				cur = nanRay ? LRH_CHILD_EMPTY : v.root;
 				curT = -INFINITY;
 				inInst = false;
 				steps = 0;

@@ -94,7 +94,8 @@ def compare(ref, got, tie_check=None, rel_tol=REL_TOL, motion_objects=None, mint
         db = np.maximum(np.abs(ref["b1"][mboth].astype(np.float64) - got["b1"][mboth]),
                         np.abs(ref["b2"][mboth].astype(np.float64) - got["b2"][mboth]))
         motion_excess = float(max((dt - lim).max(), (db - MOTION_B_ABS).max(), 0.0))
-    miss_t_ok = bool(np.all(got["t"][~rhit & ~ghit] == ref["t"][~rhit & ~ghit]))
+    mt_r, mt_g = ref["t"][~rhit & ~ghit], got["t"][~rhit & ~ghit]
+    miss_t_ok = bool(np.all((mt_g == mt_r) | (np.isnan(mt_g) & np.isnan(mt_r))))  # t = maxt, NaN included
     return {"rays": int(ref.shape[0]), "hits": int(rhit.sum()), "mismatch": int(mism.size),
             "edge_band": n_edge, "ties": n_tie, "motion_band": n_motion, "bad": n_bad, "bad_examples": bad,
             "worst_rel": worst, "motion_hits": int(mboth.size), "motion_excess": motion_excess,
