@@ -7,6 +7,7 @@
  * may cull (src/luxrays/accelerators/mbvhaccel.cpp:187-202).
  */
 #include "lrh_build.h"
+#include "lrh_math.h"
 
 #include <algorithm>
 #include <atomic>
@@ -81,58 +82,9 @@ M4 identity() {
 	return r;
 }
 
-bool invert(const M4 &in, M4 *out) {
-	int indxc[4], indxr[4], ipiv[4] = {0, 0, 0, 0};
-	float a[4][4];
-	std::memcpy(a, in.m, sizeof(a));
-	for (int i = 0; i < 4; ++i) {
-		int irow = -1, icol = -1;
-		float big = 0.f;
-		for (int j = 0; j < 4; ++j) {
-			if (ipiv[j] == 1) continue;
-			for (int k = 0; k < 4; ++k) {
-				if (ipiv[k] == 0) {
-					if (fabsf(a[j][k]) >= big) { big = fabsf(a[j][k]); irow = j; icol = k; }
-				} else if (ipiv[k] > 1)
-					return false;
-			}
-		}
-		if (icol < 0) return false;
-		++ipiv[icol];
-		if (irow != icol)
-			for (int k = 0; k < 4; ++k) std::swap(a[irow][k], a[icol][k]);
-		indxr[i] = irow;
-		indxc[i] = icol;
-		if (a[icol][icol] == 0.f) return false;
-		const float pivinv = 1.f / a[icol][icol];
-		a[icol][icol] = 1.f;
-		for (int j = 0; j < 4; ++j) a[icol][j] *= pivinv;
-		for (int j = 0; j < 4; ++j) {
-			if (j == icol) continue;
-			const float save = a[j][icol];
-			a[j][icol] = 0;
-			for (int k = 0; k < 4; ++k) a[j][k] -= a[icol][k] * save;
-		}
-	}
-	for (int j = 3; j >= 0; --j)
-		if (indxr[j] != indxc[j])
-			for (int k = 0; k < 4; ++k) std::swap(a[k][indxr[j]], a[k][indxc[j]]);
-	std::memcpy(out->m, a, sizeof(a));
-	return true;
-}
+bool invert(const M4 &in, M4 *out) { return invert4(in.m, out->m); }
 
-// Matrix * Point with the projective divide (matrix4x4op.h:31-41)
-static void xfPoint(const M4 &M, const float p[3], float r[3]) {
-	const float x = p[0], y = p[1], z = p[2];
-	r[0] = M.m[0][0] * x + M.m[0][1] * y + M.m[0][2] * z + M.m[0][3];
-	r[1] = M.m[1][0] * x + M.m[1][1] * y + M.m[1][2] * z + M.m[1][3];
-	r[2] = M.m[2][0] * x + M.m[2][1] * y + M.m[2][2] * z + M.m[2][3];
-	const float w = M.m[3][0] * x + M.m[3][1] * y + M.m[3][2] * z + M.m[3][3];
-	if (w != 1.f) {
-		const float inv = 1.f / w;
-		r[0] = inv * r[0]; r[1] = inv * r[1]; r[2] = inv * r[2];
-	}
-}
+static void xfPoint(const M4 &M, const float p[3], float r[3]) { xfPoint4(M.m, p, r); }
 
 Box transformBox(const M4 &m, const Box &b) {
 	// corner order of transform.h:271-280: (pMin,pMax) first, then six more

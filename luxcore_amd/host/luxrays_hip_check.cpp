@@ -5,8 +5,11 @@
  * FinishQueue, TraceRay, Update after a transform edit) on a scene read from
  * files, and writes the hit records.
  *
- *   luxrays_hip_check <dir> [--accel auto|bvh|mbvh] [--anyhit] [--serial N]
+ *   luxrays_hip_check <dir> [--accel auto|bvh|mbvh] [--anyhit] [--serial N] [--scene FILE]
  *
+ * --scene FILE       meshes and objects from a LuxCore scene file (.scn/.cfg,
+ *                    PLY meshes) through libluxrays_ingest instead of the two
+ *                    files below
  * <dir>/meshes.bin   u32 nMeshes; per mesh u32 nVerts, u32 nTris,
  *                    f32 verts[3 nVerts], u32 tris[3 nTris]
  * <dir>/objects.bin  optional; u32 nObjects; per object u32 mesh, u32 kind
@@ -26,6 +29,7 @@
 #include <iterator>
 
 #include "luxrays_hip.h"
+#include "../../include/lrh_ingest.h"
 
 using namespace luxrays_hip;
 
@@ -101,13 +105,14 @@ MotionSystem motion(const ObjDesc &o) {
 
 int main(int argc, char **argv) {
 	if (argc < 2) {
-		std::cerr << "usage: luxrays_hip_check <dir> [--accel auto|bvh|mbvh] [--anyhit] [--serial N]\n";
+		std::cerr << "usage: luxrays_hip_check <dir> [--accel auto|bvh|mbvh] [--anyhit] [--serial N] [--scene FILE]\n";
 		return 2;
 	}
 	const std::string dir = argv[1];
 	AcceleratorType accelType = ACCEL_AUTO;
 	bool anyHit = false;
 	u_int serial = 0;
+	std::string sceneFile;
 	for (int i = 2; i < argc; ++i) {
 		const std::string a = argv[i];
 		if (a == "--accel" && i + 1 < argc) {
@@ -117,35 +122,82 @@ int main(int argc, char **argv) {
 			anyHit = true;
 		else if (a == "--serial" && i + 1 < argc)
 			serial = (u_int)std::stoul(argv[++i]);
+		else if (a == "--scene" && i + 1 < argc)
+			sceneFile = argv[++i];
 		else {
 			std::cerr << "unknown argument " << a << "\n";
 			return 2;
 		}
 	}
 	try {
-		// meshes
-		const std::vector<char> mb = readAll(dir + "/meshes.bin", true);
-		Reader r(mb);
-		const u_int nMeshes = r.u();
-		std::vector<std::vector<Point>> verts(nMeshes);
-		std::vector<std::vector<Triangle>> tris(nMeshes);
-		std::vector<std::unique_ptr<TriangleMesh>> base;
-		for (u_int i = 0; i < nMeshes; ++i) {
-			const u_int nv = r.u(), nt = r.u();
-			verts[i].resize(nv);
-			tris[i].resize(nt);
-			r.get(verts[i].data(), nv);
-			r.get(tris[i].data(), nt);
-			base.emplace_back(new TriangleMesh(nv, nt, verts[i].data(), tris[i].data()));
+		// meshes and object descriptions, from the scene file or the binary files
+		std::vector<std::vector<Point>> verts;
+		std::vector<std::vector<Triangle>> tris;
+		std::vector<ObjDesc> descs;
+		bool haveObjects = false;
+		if (!sceneFile.empty()) {
+			lrh_scenefile *sf = nullptr;
+			if (lrh_scenefile_load(sceneFile.c_str(), nullptr, 0, &sf) != LRH_OK)
+				throw std::runtime_error(lrh_ingest_last_error());
+			std::unique_ptr<lrh_scenefile, int (*)(lrh_scenefile *)> hold(sf, lrh_scenefile_destroy);
+			lrh_scenefile_info info;
+			if (lrh_scenefile_get_info(sf, &info) != LRH_OK) throw std::runtime_error(lrh_ingest_last_error());
+			std::vector<lrh_mesh> ms(info.nMeshes);
+			std::vector<lrh_object> os(info.nObjects);
+			if (lrh_scenefile_meshes(sf, ms.data(), info.nMeshes) != LRH_OK ||
+			    lrh_scenefile_objects(sf, os.data(), info.nObjects) != LRH_OK)
+				throw std::runtime_error(lrh_ingest_last_error());
+			for (const lrh_mesh &m : ms) {
+				verts.emplace_back(m.nVerts);
+				tris.emplace_back(m.nTris);
+				for (u_int v = 0; v < m.nVerts; ++v)
+					verts.back()[v] = Point(m.verts[3 * v], m.verts[3 * v + 1], m.verts[3 * v + 2]);
+				std::memcpy(tris.back().data(), m.tris, (size_t)m.nTris * sizeof(Triangle));
+			}
+			for (const lrh_object &o : os) {
+				ObjDesc d;
+				d.mesh = o.mesh;
+				d.kind = o.kind;
+				for (int k = 0; k < 16; ++k) d.m[k] = (k % 5 == 0) ? 1.f : 0.f;
+				if (o.kind == 1) std::memcpy(d.m, o.m, 64);
+				if (o.kind == 2) {
+					d.times.assign(o.keyTimes, o.keyTimes + o.nKeys);
+					d.mats.assign(o.keyMats, o.keyMats + 16 * (size_t)o.nKeys);
+				}
+				descs.push_back(d);
+			}
+			haveObjects = true;
+			std::cout << "scene " << sceneFile << " meshes " << info.nMeshes << " objects " << info.nObjects
+			          << " world triangles " << info.nWorldTris << std::endl;
+		} else {
+			const std::vector<char> mb = readAll(dir + "/meshes.bin", true);
+			Reader r(mb);
+			const u_int nMeshes = r.u();
+			for (u_int i = 0; i < nMeshes; ++i) {
+				const u_int nv = r.u(), nt = r.u();
+				verts.emplace_back(nv);
+				tris.emplace_back(nt);
+				r.get(verts.back().data(), nv);
+				r.get(tris.back().data(), nt);
+			}
+			const std::vector<char> ob = readAll(dir + "/objects.bin", false);
+			if (!ob.empty()) {
+				descs = readObjects(ob);
+				haveObjects = true;
+			}
 		}
+		const u_int nMeshes = (u_int)verts.size();
+		std::vector<std::unique_ptr<TriangleMesh>> base;
+		for (u_int i = 0; i < nMeshes; ++i)
+			base.emplace_back(new TriangleMesh((u_int)verts[i].size(), (u_int)tris[i].size(), verts[i].data(),
+			                                   tris[i].data()));
 		// objects
 		std::vector<std::unique_ptr<Mesh>> owned;
 		std::vector<Mesh *> objects;
-		const std::vector<char> ob = readAll(dir + "/objects.bin", false);
-		if (ob.empty()) {
+		if (!haveObjects) {
 			for (auto &m : base) objects.push_back(m.get());
 		} else {
-			for (const ObjDesc &o : readObjects(ob)) {
+			for (const ObjDesc &o : descs) {
 				if (o.mesh >= nMeshes) throw std::runtime_error("object mesh index out of range");
 				if (o.kind == 0)
 					objects.push_back(base[o.mesh].get());

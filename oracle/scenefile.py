@@ -1,0 +1,457 @@
+"""Scene-file ingest restated on the CPU: PLY meshes and the geometry part of
+LuxCore scene files.
+
+TEST INFRASTRUCTURE ONLY: the checker for the product's scene reader
+(libluxrays_ingest.so).  Only tests/ may import this module.  It shares no
+code with the product: a separate reading of the same reference rules.
+
+Reference rules restated (LuxCore v2.6 tree):
+  PLY values       rply value readers, src/luxrays/utils/ply/rply.cpp:1303-1423
+                   (ASCII ints/floats range-checked; binary float32 NaN/Inf -> 0)
+  PLY mesh         ExtTriangleMesh::LoadPly, src/luxrays/core/exttrianglemeshfile.cpp:259-452
+  face rule        FaceCB, exttrianglemeshfile.cpp:207-239 (3 -> tri, 4 -> (0,1,2),(0,2,3), else skipped)
+  properties       Properties::SetFromStream / Property::FromString,
+                   src/luxrays/utils/properties.cpp:652-749, 843-875, 1099-1113
+  matrices         Property::Get<Matrix4x4>, properties.cpp:606-614 (column by column)
+  shapes           Scene::CreateShape, src/slg/scene/parseshapes.cpp:141-400
+  objects          Scene::ParseObjects / CreateObject, src/slg/scene/parseobjects.cpp:30-200
+  file names       FileNameResolver::ResolveFile, src/slg/utils/filenameresolver.cpp:48-90
+
+What pins it (tests/test_ingest_cpu.py): PLY files written independently in
+all three encodings round-trip to the same arrays; the reference's own PLY
+files agree with numpy's text parser and with closed forms (a cube's six
+faces); scene files reproduce closed-form placements (the translation column
+maps the origin to the stated point, a key's global->local matrix maps it
+back).
+"""
+import math
+import os
+import struct
+
+import numpy as np
+
+from luxcore_amd.scenes import InstancedScene, Mesh
+
+_FLT_MAX = 3.4028234663852886e38
+_DBL_MAX = 1.7976931348623157e308
+
+# name -> (struct code, integer?, ascii range)
+_TYPES = {
+    "char": ("b", True, (-128, 127)), "int8": ("b", True, (-128, 127)),
+    "uchar": ("B", True, (0, 255)), "uint8": ("B", True, (0, 255)),
+    "short": ("h", True, (-32768, 32767)), "int16": ("h", True, (-32768, 32767)),
+    "ushort": ("H", True, (0, 65535)), "uint16": ("H", True, (0, 65535)),
+    "int": ("i", True, (-(1 << 63), (1 << 63) - 1)), "int32": ("i", True, (-(1 << 63), (1 << 63) - 1)),
+    "uint": ("I", True, (0, (1 << 63) - 1)), "uint32": ("I", True, (0, (1 << 63) - 1)),
+    "float": ("f", False, (-_FLT_MAX, _FLT_MAX)), "float32": ("f", False, (-_FLT_MAX, _FLT_MAX)),
+    "double": ("d", False, (-_DBL_MAX, _DBL_MAX)), "float64": ("d", False, (-_DBL_MAX, _DBL_MAX)),
+}
+
+
+class IngestError(Exception):
+    pass
+
+
+def _ascii_value(tok, tname):
+    code, is_int, (lo, hi) = _TYPES[tname]
+    try:
+        v = int(tok, 10) if is_int else float(tok)
+    except ValueError:
+        raise IngestError("parse")
+    if is_int:
+        if v < lo or v > hi:
+            raise IngestError("parse")
+        return float(v)
+    if not (math.isnan(v) or lo <= v <= hi):
+        raise IngestError("parse")
+    return v
+
+
+class _Binary:
+    def __init__(self, buf, pos, endian):
+        self.buf, self.pos, self.endian = buf, pos, endian
+
+    def value(self, tname):
+        code = _TYPES[tname][0]
+        n = struct.calcsize(code)
+        if self.pos + n > len(self.buf):
+            raise IngestError("parse")
+        (v,) = struct.unpack_from(self.endian + code, self.buf, self.pos)
+        self.pos += n
+        if code == "f" and (math.isnan(v) or math.isinf(v)):
+            v = 0.0
+        return float(v)
+
+
+class _Ascii:
+    def __init__(self, buf, pos):
+        self.toks = buf[pos:].split()
+        self.i = 0
+
+    def value(self, tname):
+        if self.i >= len(self.toks):
+            raise IngestError("parse")
+        t = self.toks[self.i].decode("latin-1")
+        self.i += 1
+        return _ascii_value(t, tname)
+
+
+def read_ply(path):
+    """(verts float32 (V,3), tris uint32 (T,3)) of one PLY file."""
+    with open(path, "rb") as f:
+        buf = f.read()
+    pos = 0
+    lines = []
+    while True:
+        nl = buf.find(b"\n", pos)
+        if nl < 0:
+            raise IngestError("header")
+        words = buf[pos:nl].decode("latin-1").split()
+        pos = nl + 1
+        lines.append(words)
+        if words == ["end_header"]:
+            break
+    if lines[0] != ["ply"]:
+        raise IngestError("header")
+    fmt, elems = None, []
+    for w in lines[1:-1]:
+        if not w:
+            raise IngestError("header")
+        if w[0] == "format":
+            if len(w) != 3 or w[2] != "1.0" or w[1] not in ("ascii", "binary_little_endian", "binary_big_endian"):
+                raise IngestError("header")
+            fmt = w[1]
+        elif w[0] in ("comment", "obj_info"):
+            continue
+        elif w[0] == "element":
+            if len(w) != 3 or not w[2].isdigit():
+                raise IngestError("header")
+            elems.append((w[1], int(w[2]), []))
+        elif w[0] == "property":
+            if not elems:
+                raise IngestError("header")
+            if len(w) == 5 and w[1] == "list" and w[2] in _TYPES and w[3] in _TYPES:
+                elems[-1][2].append((w[4], w[2], w[3]))
+            elif len(w) == 3 and w[1] in _TYPES:
+                elems[-1][2].append((w[2], None, w[1]))
+            else:
+                raise IngestError("header")
+        else:
+            raise IngestError("header")
+    if fmt is None:
+        raise IngestError("header")
+
+    def find(ename, pname):
+        for ei, (en, _, props) in enumerate(elems):
+            if en == ename:
+                for pi, p in enumerate(props):
+                    if p[0] == pname:
+                        return ei, pi
+                return None
+        return None
+
+    vx, vy, vz = find("vertex", "x"), find("vertex", "y"), find("vertex", "z")
+    if vx is None or elems[vx[0]][1] == 0:
+        raise IngestError("no vertices")
+    if vy is None or vz is None:
+        raise IngestError("no y/z")
+    fc = find("face", "vertex_indices")
+    if fc is None or elems[fc[0]][1] == 0:
+        raise IngestError("no faces")
+
+    rd = _Ascii(buf, pos) if fmt == "ascii" else _Binary(buf, pos, "<" if fmt == "binary_little_endian" else ">")
+    nv = elems[vx[0]][1]
+    verts = np.zeros((nv, 3), np.float32)
+    tris = []
+    for ei, (_, count, props) in enumerate(elems):
+        for inst in range(count):
+            for pi, (pname, ctype, vtype) in enumerate(props):
+                if ctype is None:
+                    v = rd.value(vtype)
+                    if ei == vx[0]:
+                        for axis, key in enumerate((vx, vy, vz)):
+                            if pi == key[1]:
+                                verts[inst, axis] = np.float32(v)
+                    continue
+                n = rd.value(ctype)
+                if not (n >= 0) or n != math.floor(n):
+                    raise IngestError("parse")
+                vals = [rd.value(vtype) for _ in range(int(n))]
+                if (ei, pi) != fc:
+                    continue
+                if len(vals) == 3:
+                    tris.append(vals)
+                elif len(vals) == 4:
+                    tris.append([vals[0], vals[1], vals[2]])
+                    tris.append([vals[0], vals[2], vals[3]])
+    t = np.asarray(tris, np.float64).reshape(-1, 3)
+    if t.size and (np.any(t < 0) or np.any(t >= nv) or np.any(t != np.floor(t))):
+        raise IngestError("index")
+    return verts, t.astype(np.uint32)
+
+
+# ---------------------------------------------------------------------------
+# properties
+# ---------------------------------------------------------------------------
+def _fields(name, value):
+    out, first, n = [], 0, len(value)
+
+    def blanks(i):
+        while i < n and value[i] in " \t":
+            i += 1
+        return i
+
+    while first < n:
+        if first + 5 < n and value[first] == "{" and value[first + 1] == "[":
+            last = first
+            while last < n - 1:
+                if value[last] == "]" or value[last + 1] == "}":
+                    out.append(value[first:last + 2])
+                    break
+                last += 1
+            else:
+                raise IngestError("Unterminated blob in property: " + name)
+            first = blanks(last + 2)
+        elif value[first] in "\"'":
+            last = first + 1
+            while last < n and value[last] not in "\"'":
+                last += 1
+            if last >= n:
+                raise IngestError("Unterminated quote in property: " + name)
+            out.append(value[first + 1:last])
+            first = blanks(last + 1)
+        else:
+            last = first
+            while last < n and value[last] not in " \t":
+                last += 1
+            out.append(value[first:last])
+            first = blanks(last)
+    return out
+
+
+def parse_properties(text):
+    """Ordered dict name -> list of string fields (first definition keeps its place)."""
+    props = {}
+    for ln, raw in enumerate(text.split("\n"), 1):
+        line = raw.strip()
+        if not line or line[0] == "#":
+            continue
+        if "=" not in line:
+            raise IngestError(f"Syntax error in a Properties at line {ln}")
+        k, v = line.split("=", 1)
+        k = k.strip()
+        props[k] = _fields(k, v.strip())
+    return props
+
+
+def unique_sub_names(props, prefix):
+    want = prefix.count(".") + 2
+    out = []
+    for name in props:
+        if not name.startswith(prefix):
+            continue
+        parts = name.split(".")
+        if len(parts) < want + 1:
+            continue
+        s = ".".join(parts[:want])
+        if s and s not in out:
+            out.append(s)
+    return out
+
+
+def _matrix(vals):
+    if len(vals) != 16:
+        raise IngestError("Wrong number of values")
+    v = np.asarray([np.float32(float(x)) for x in vals], np.float32)
+    return v.reshape(4, 4).T.copy()   # file order is column by column
+
+
+def transform_points(m, p):
+    """Transform * Point with the projective divide (transform.h:131-142), fp32 op by op."""
+    m = np.asarray(m, np.float32)
+    x, y, z = p[:, 0].astype(np.float32), p[:, 1].astype(np.float32), p[:, 2].astype(np.float32)
+    r = [((m[i, 0] * x + m[i, 1] * y) + m[i, 2] * z) + m[i, 3] for i in range(3)]
+    w = ((m[3, 0] * x + m[3, 1] * y) + m[3, 2] * z) + m[3, 3]
+    out = np.stack(r, axis=1).astype(np.float32)
+    proj = w != np.float32(1.0)
+    if np.any(proj):
+        inv = (np.float32(1.0) / w[proj]).astype(np.float32)
+        out[proj] = out[proj] * inv[:, None]
+    return out
+
+
+class SceneFile:
+    """meshes: list of Mesh (unique, definition order); objects: list of
+    dicts {name, mesh, kind, m (local->world), times, keys (global->local)};
+    camera: (orig, target, up, fov)."""
+
+    def __init__(self, meshes, mesh_names, objects, camera):
+        self.meshes, self.mesh_names, self.objects, self.camera = meshes, mesh_names, objects, camera
+
+    @property
+    def two_level(self):
+        return any(o["kind"] != 0 for o in self.objects)
+
+    def instanced(self):
+        """InstancedScene over the unique meshes (for OracleMBVH)."""
+        n = len(self.objects)
+        obj_mat = np.zeros((n, 4, 4), np.float32)
+        key_off = np.zeros(n + 1, np.uint32)
+        times, mats = [], []
+        for i, o in enumerate(self.objects):
+            if o["kind"] == 1:
+                obj_mat[i] = o["m"]
+            key_off[i + 1] = key_off[i] + len(o["times"])
+            times += list(o["times"])
+            mats += list(o["keys"])
+        km = np.asarray(mats, np.float32).reshape(-1, 4, 4) if mats else np.zeros((0, 4, 4), np.float32)
+        return InstancedScene(list(self.meshes), np.asarray([o["mesh"] for o in self.objects], np.uint32),
+                              np.asarray([o["kind"] for o in self.objects], np.uint32), obj_mat, key_off,
+                              np.asarray(times, np.float32), km)
+
+    def dataset_meshes(self):
+        """One mesh per object in object order (the single-level DataSet)."""
+        return [self.meshes[o["mesh"]] for o in self.objects]
+
+
+def _resolve(name, dirs):
+    if not name or os.path.isabs(name) or os.path.exists(name):
+        return name
+    for d in dirs:
+        p = os.path.join(d, name)
+        if os.path.exists(p):
+            return p
+    for d in dirs:
+        p = os.path.join(d, os.path.basename(name))
+        if os.path.exists(p):
+            return p
+    return name
+
+
+_KEEP_GEOMETRY = {"pointiness", "harlequin", "islandaov", "randomtriangleaov", "edgedetectoraov", "cameraprojuv"}
+
+
+def load_scene(path, search_dirs=()):
+    from oracle import matrix_inverse
+    dirs = [os.path.dirname(path)] + list(search_dirs)
+    with open(path, "r", encoding="latin-1") as f:
+        props = parse_properties(f.read())
+    if "scene.file" in props:
+        scn = _resolve(props["scene.file"][0], dirs)
+        if os.path.dirname(scn):
+            dirs.append(os.path.dirname(scn))
+        with open(scn, "r", encoding="latin-1") as f:
+            props = parse_properties(f.read())
+
+    def get(name, default=None):
+        return props[name] if name in props else default
+
+    # camera
+    z3 = [np.float32(0.0)] * 3
+    if "scene.camera.lookat" in props:
+        v = [np.float32(float(x)) for x in props["scene.camera.lookat"][:6]]
+        if len(v) < 6:
+            raise IngestError("Out of bound index in property: scene.camera.lookat")
+        orig, target, up = v[:3], v[3:6], [np.float32(0.0), np.float32(0.0), np.float32(1.0)]
+    elif "scene.camera.lookat.orig" in props:
+        orig = [np.float32(float(x)) for x in props["scene.camera.lookat.orig"]]
+        target = [np.float32(float(x)) for x in get("scene.camera.lookat.target", ["0", "0", "0"])]
+        up = [np.float32(float(x)) for x in get("scene.camera.up", ["0", "0", "1"])]
+    else:
+        orig, target, up = z3, z3, z3
+    fov = np.float32(float(get("scene.camera.fieldofview", ["45"])[0]))
+
+    materials = {k.split(".", 2)[2] for k in unique_sub_names(props, "scene.materials")}
+    meshes, names, index = [], [], {}
+
+    def define(name, verts, tris):
+        if name in index:
+            meshes[index[name]] = Mesh(verts, tris)
+        else:
+            index[name] = len(meshes)
+            meshes.append(Mesh(verts, tris))
+            names.append(name)
+
+    def inlined(key, shape_name):
+        v = get(key + ".vertices")
+        if v is None:
+            raise IngestError("Missing shape vertex list: " + shape_name)
+        if len(v) == 0 or len(v) % 3:
+            raise IngestError("Wrong shape vertex list length: " + shape_name)
+        f = get(key + ".faces")
+        if f is None:
+            raise IngestError("Missing shape face list: " + shape_name)
+        if len(f) == 0 or len(f) % 3:
+            raise IngestError("Wrong shape face list length: " + shape_name)
+        verts = np.asarray([np.float32(float(x)) for x in v], np.float32).reshape(-1, 3)
+        tris = np.asarray([int(x) for x in f], np.int64).reshape(-1, 3)
+        if np.any(tris < 0) or np.any(tris >= verts.shape[0]):
+            raise IngestError("index")
+        return verts, tris.astype(np.uint32)
+
+    for key in unique_sub_names(props, "scene.shapes"):
+        sname = key.split(".", 2)[2]
+        stype = get(key + ".type", ["mesh"])[0]
+        if stype in ("mesh", "inlinedmesh"):
+            if stype == "mesh":
+                verts, tris = read_ply(_resolve(get(key + ".ply", [""])[0], dirs))
+            else:
+                verts, tris = inlined(key, sname)
+            if key + ".transformation" in props:
+                verts = transform_points(_matrix(props[key + ".transformation"]), verts)
+        elif stype in _KEEP_GEOMETRY:
+            src = get(key + ".source", [""])[0]
+            if src not in index:
+                raise IngestError("Unknown shape name in a " + stype + " shape: " + sname)
+            m = meshes[index[src]]
+            verts, tris = m.verts, m.tris
+        else:
+            raise IngestError("unsupported shape type " + stype)
+        define(sname, verts, tris)
+
+    objects, obj_index = [], {}
+    for key in unique_sub_names(props, "scene.objects"):
+        oname = key.split(".", 2)[2]
+        mat = get(key + ".material", [""])
+        if not mat or not mat[0]:
+            raise IngestError("Syntax error in object material reference: " + oname)
+        if mat[0] not in materials:
+            raise IngestError("Unknown material: " + mat[0])
+        if key + ".ply" in props:
+            shape = props[key + ".ply"][0]
+            if shape not in index:
+                define(shape, *read_ply(_resolve(shape, dirs)))
+        elif key + ".vertices" in props:
+            shape = "InlinedMesh-" + oname
+            if shape not in index:
+                define(shape, *inlined(key, shape))
+        elif key + ".shape" in props:
+            shape = props[key + ".shape"][0]
+            if shape not in index:
+                raise IngestError("Unknown shape: " + shape)
+        else:
+            raise IngestError("Missing shape in object definition: " + oname)
+        o = {"name": oname, "mesh": index[shape], "kind": 0, "m": np.eye(4, dtype=np.float32),
+             "times": [], "keys": []}
+        if key + ".motion.0.time" in props:
+            o["kind"] = 2
+            i = 0
+            while key + f".motion.{i}.time" in props:
+                t = np.float32(float(props[key + f".motion.{i}.time"][0]))
+                if i > 0 and t <= o["times"][-1]:
+                    raise IngestError(oname + " motion time must be monotonic")
+                o["times"].append(t)
+                tk = key + f".motion.{i}.transformation"
+                l2w = _matrix(props[tk]) if tk in props else np.eye(4, dtype=np.float32)
+                o["keys"].append(matrix_inverse(l2w))
+                i += 1
+        elif key + ".transformation" in props:
+            o["kind"] = 1
+            o["m"] = _matrix(props[key + ".transformation"])
+            matrix_inverse(o["m"])   # singular -> ValueError, as Transform(mat) throws
+        if oname in obj_index:
+            objects[obj_index[oname]] = o
+        else:
+            obj_index[oname] = len(objects)
+            objects.append(o)
+    return SceneFile(meshes, names, objects, (orig, target, up, fov))
