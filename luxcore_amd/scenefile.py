@@ -147,6 +147,27 @@ class SceneFile:
                               np.asarray([o["kind"] for o in self.objects], np.uint32), obj_mat, key_off,
                               np.asarray(times, np.float32), km)
 
+    def world_bounds(self):
+        """Axis-aligned world box of every object (instances through their
+        matrix, motion objects at each key), in float64: for placing rays."""
+        lo, hi = np.full(3, np.inf), np.full(3, -np.inf)
+        for o in self.objects:
+            v = self.meshes[o["mesh"]].verts.astype(np.float64)
+            if v.size == 0:
+                continue
+            if o["kind"] == 0:
+                mats = [np.eye(4)]
+            elif o["kind"] == 1:
+                mats = [np.asarray(o["m"], np.float64)]
+            else:
+                mats = [np.linalg.inv(np.asarray(k, np.float64)) for k in o["keys"]]
+            for M in mats:
+                w = v @ M[:3, :3].T + M[:3, 3]
+                lo, hi = np.minimum(lo, w.min(0)), np.maximum(hi, w.max(0))
+        if not np.all(np.isfinite(lo)):
+            return np.zeros(3), np.ones(3)
+        return lo, hi
+
     def dataset_meshes(self):
         """One mesh per object, in object order: the single-level DataSet."""
         return [self.meshes[o["mesh"]] for o in self.objects]
