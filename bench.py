@@ -264,13 +264,68 @@ def run_scene(args, dev):
             "sets": res}
 
 
+def _moved(sc, rng, extent, step=5.0):
+    """The same objects with new instance matrices and motion keys."""
+    import copy
+    out = copy.deepcopy(sc)
+    for o in range(len(out.obj_kind)):
+        if out.obj_kind[o] == 1:
+            out.obj_mat[o] = scenes.random_trs(rng, extent)
+        elif out.obj_kind[o] == 2:
+            M = scenes.random_trs(rng, extent).astype(np.float64)
+            for k in range(int(out.key_off[o]), int(out.key_off[o + 1])):
+                Mk = M.copy()
+                Mk[:3, 3] += rng.uniform(-step, step, size=3)
+                out.key_mats[k] = np.linalg.inv(Mk).astype(np.float32)
+    return out
+
+
+def run_refit(args, dev):
+    """Edit latency of a two-level scene (1,000 instances of a 50,000-triangle
+    sphere, 200 with 2-key motion): every object moves, then either
+    lrh_scene_update (root rebuilt on the host, MotionSystem::Bound on the
+    host, upload) or lrh_scene_refit (same boxes computed on the GPU, root
+    refitted in place).  Also the trace rate after each, since a refitted
+    tree keeps the old grouping."""
+    rng = np.random.Generator(np.random.PCG64(77))
+    sc = scenes.instanced_scene()
+    rays = scenes.instanced_rays(1 << 22, seed=78)
+    edits = [_moved(sc, rng, 100.0) for _ in range(6)]
+    scene = dev.scene_mbvh(sc)
+    n = rays.shape[0]
+    rb = dev.upload(rays)
+    hb = dev.alloc(n * HIT_DTYPE.itemsize)
+    res = {}
+    for how in ("update", "refit"):
+        fn = scene.update if how == "update" else scene.refit
+        fn(edits[0])  # warm
+        times = []
+        for e in edits[1:]:
+            t0 = time.perf_counter()
+            fn(e)
+            times.append((time.perf_counter() - t0) * 1e3)
+        wall, lat = _event_timed(dev, lambda: dev.trace(scene, rb, hb, n, 0), args.steps, args.warmup)
+        res[how] = {"edit_ms_median": round(float(np.median(times)), 3), "edit_ms_min": round(float(min(times)), 3),
+                    "trace_mrays_after": round(n * args.steps / wall / 1e6, 1)}
+        log(f"{how}: {res[how]}")
+    return {"metric": "two-level edit latency (ms): host rebuild vs GPU refit",
+            "value": res["refit"]["edit_ms_median"], "unit": "ms", "n_gpus": 1, "steps": len(edits) - 1,
+            "warmup": 1, "ms_per_step": res["refit"]["edit_ms_median"], "higher_is_better": False,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "1,000 instances of a 50,000-tri displaced sphere, 200 with 2-key motion; "
+                                   "every object moves per edit", "rays_traced_after": n},
+            "update": res["update"], "refit": res["refit"],
+            "speedup": round(res["update"]["edit_ms_median"] / res["refit"]["edit_ms_median"], 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--mode", choices=("headline", "batched", "served", "scene"), default="headline",
+    ap.add_argument("--mode", choices=("headline", "batched", "served", "scene", "refit"), default="headline",
                     help="headline: incoherent closest-hit on the 26.2M-tri scene (the driver's line); "
                          "batched: 64 buffers over 8 resident scenes in one launch; served: continuous batching; "
-                         "scene: a LuxCore scene file (--scene, default the bundled lightinstances scene)")
+                         "scene: a LuxCore scene file (--scene, default the bundled lightinstances scene); "
+                         "refit: two-level edit latency, host rebuild (lrh_scene_update) vs GPU refit")
     ap.add_argument("--scene", default=None, help="scene file for --mode scene (.scn/.cfg)")
     ap.add_argument("--scene-rays", type=int, default=1 << 22, help="incoherent rays for --mode scene")
     ap.add_argument("--steps", type=int, default=200)
@@ -307,6 +362,9 @@ def main():
         return
     if args.mode == "scene":
         print(json.dumps(run_scene(args, dev)), flush=True)
+        return
+    if args.mode == "refit":
+        print(json.dumps(run_refit(args, dev)), flush=True)
         return
     meshes, scene, info, rays, ntris = build_workload(args, dev)
     n = rays.shape[0]

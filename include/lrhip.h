@@ -171,6 +171,19 @@ int lrh_scene_create_mbvh(lrh_device *dev, const lrh_mesh *meshes, unsigned int 
 /* MBVHAccel::Update: new instance matrices / motion keys for the same objects
  * (same count, kinds and meshes); root tree rebuilt, leaf trees kept. */
 int lrh_scene_update(lrh_scene *scene, const lrh_object *objects, unsigned int nObjects);
+/* The same edit without a rebuild (MBVHAccel::Update, mbvhaccel.cpp:237-250,
+ * rebuilds the root tree on the host; MBVHKernel::Update,
+ * mbvhaccelhw.cpp:442-466, re-uploads it): the root tree keeps its topology
+ * and the GPU recomputes its boxes -- motion objects' MotionSystem::Bound
+ * (1,025 samples per segment, motionsystem.cpp:75-87) and every root node,
+ * bottom-up.  The object boxes are the ones lrh_scene_update would compute;
+ * only the tree's grouping is the old one.  Synchronous on the device's
+ * stream; LRH_EINVAL if a motion sample is singular. */
+int lrh_scene_refit(lrh_scene *scene, const lrh_object *objects, unsigned int nObjects);
+/* The culling box of every object of a two-level scene (6 floats each: lo
+ * xyz, hi xyz; MachineEpsilon-expanded world boxes, mbvhaccel.cpp:187-202),
+ * in object order, as the last build, update or refit computed them. */
+int lrh_scene_object_boxes(lrh_scene *scene, float *boxes, unsigned int nObjects);
 int lrh_scene_destroy(lrh_scene *scene);
 int lrh_scene_get_info(lrh_scene *scene, lrh_scene_info *info);
 /* Copy of the device node / primitive arrays to host memory (for tests). */

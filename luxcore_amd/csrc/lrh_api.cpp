@@ -25,6 +25,7 @@
 #include "lrh_gpubuild.h"
 #include "lrh_kernels.h"
 #include "lrh_paths.h"
+#include "lrh_refit.h"
 #include "lrh_layout.h"
 
 using lrh::Box;
@@ -162,6 +163,12 @@ struct lrh_scene {
 	std::vector<Box> meshBox;
 	size_t rootReserve = 0;  // node slots reserved for the root tree
 	uint32_t leafDepth = 0;
+	// root topology kept for lrh_scene_refit: leaf slot -> object, node
+	// indices by level (deepest first), object boxes of the last host build
+	std::vector<uint32_t> rootOrder, levelNodes, levelStart;
+	std::vector<Box> objBoxes;
+	bool boxesOnDevice = false;  // a refit has replaced objBoxes on the device
+	DevBuf rSlotBoxes, rNodeBoxes, rLevelNodes, rMeshBoxes, rMotionObjs, rSampleBase, rKeys, rError, rMotions, rSegs;
 	// host-only scenes (created with dev == NULL) keep their arrays here so
 	// the builder can be inspected without a GPU; they cannot be traced
 	std::vector<uint8_t> hostNodes, hostPrims;
@@ -292,7 +299,7 @@ struct ObjectData {
 	std::vector<Box> boxes;                  // epsilon-expanded world boxes
 };
 
-void prepareObjects(lrh_scene *s, const lrh_object *objs, unsigned n, ObjectData &od) {
+void prepareObjects(lrh_scene *s, const lrh_object *objs, unsigned n, ObjectData &od, bool motionBounds = true) {
 	for (unsigned o = 0; o < n; ++o) {
 		const lrh_object &ob = objs[o];
 		if (ob.kind != s->objKind[o] || ob.mesh != s->objMesh[o])
@@ -325,6 +332,11 @@ void prepareObjects(lrh_scene *s, const lrh_object *objs, unsigned n, ObjectData
 			od.segs.insert(od.segs.end(), ms.segs.begin(), ms.segs.end());
 			in.index = (uint32_t)od.mot.size();
 			od.mot.push_back(md);
+			if (!motionBounds) {  // the device refit computes it
+				od.boxes.push_back(Box::Empty());
+				od.inst.push_back(in);
+				continue;
+			}
 			wb = ms.bound(wb);  // MotionTriangleMesh::GetBBox
 		} else if (ob.kind != 0)
 			throw Error(LRH_EINVAL, "object kind must be 0, 1 or 2");
@@ -348,6 +360,30 @@ void buildRoot(lrh_scene *s, ObjectData &od, HostTree &ht, std::vector<lrh_inst>
 	for (size_t i = 0; i < w.order.size(); ++i) instOut[i] = od.inst[w.order[i]];
 	*depth = w.maxDepth;
 	s->root = w.root;
+	// refit topology: slot order and the root nodes grouped by depth
+	s->rootOrder = w.order;
+	s->objBoxes = od.boxes;
+	s->boxesOnDevice = false;
+	const size_t nn = w.nodeCount();
+	std::vector<uint32_t> depthOf(nn, 0), bfs(1, w.root);
+	uint32_t maxD = 0;
+	for (size_t q = 0; q < bfs.size(); ++q) {
+		const uint32_t i = bfs[q];
+		for (int c = 0; c < 4; ++c) {
+			const uint32_t r = s->layout ? w.fnodes[i].child[c] : w.qnodes[i].child[c];
+			if (r == LRH_CHILD_EMPTY || (r & LRH_CHILD_LEAF) || r >= nn) continue;
+			depthOf[r] = depthOf[i] + 1;
+			maxD = std::max(maxD, depthOf[r]);
+			bfs.push_back(r);
+		}
+	}
+	s->levelNodes.clear();
+	s->levelStart.assign(1, 0);
+	for (int d = (int)maxD; d >= 0; --d) {
+		for (uint32_t i : bfs)
+			if (depthOf[i] == (uint32_t)d) s->levelNodes.push_back(i);
+		s->levelStart.push_back((uint32_t)s->levelNodes.size());
+	}
 }
 
 void keepHost(lrh_scene *s, const HostTree &ht, const void *tris, size_t triBytes, const void *insts,
@@ -846,6 +882,131 @@ int lrh_scene_update(lrh_scene *s, const lrh_object *objects, unsigned int nObje
 		HIPCHK(hipStreamSynchronize(dev->stream));
 		s->maxDepth = rootDepth + s->leafDepth + 1;
 		s->info.maxDepth = s->maxDepth;
+	});
+}
+
+int lrh_scene_refit(lrh_scene *s, const lrh_object *objects, unsigned int nObjects) {
+	if (!s || (nObjects && !objects)) return fail(LRH_EINVAL, "null argument");
+	if (!s->twoLevel) return fail(LRH_EINVAL, "only two-level scenes can be refitted");
+	if (nObjects != s->objMesh.size()) return fail(LRH_EINVAL, "object count differs from the scene");
+	if (!s->dev) return fail(LRH_EINVAL, "host-only scene");
+	return guarded([&]() {
+		if (!nObjects) return;
+		ObjectData od;
+		prepareObjects(s, objects, nObjects, od, false);
+		lrh_device *dev = s->dev;
+		dev->activate();
+		const hipStream_t st = dev->stream;
+		HIPCHK(hipStreamSynchronize(st));
+		// leaf-slot boxes of plain and instance objects, the list of motion objects
+		std::vector<float> slot(6 * (size_t)nObjects, 0.f);
+		std::vector<LrhRefitMotion> mo;
+		std::vector<uint32_t> sampleBase(1, 0);
+		for (size_t j = 0; j < s->rootOrder.size(); ++j) {
+			const uint32_t o = s->rootOrder[j];
+			if (s->objKind[o] == 2) {
+				const uint32_t mi = od.inst[o].index;
+				mo.push_back({(uint32_t)j, mi, s->objMesh[o], 0});
+				const uint64_t next = (uint64_t)sampleBase.back() + (uint64_t)(od.mot[mi].nTimes + 1) * 1025u;
+				if (next > 0xffffffffull) throw Error(LRH_ERANGE, "too many motion samples for one refit");
+				sampleBase.push_back((uint32_t)next);
+			} else {
+				for (int k = 0; k < 3; ++k) {
+					slot[6 * j + k] = od.boxes[o].lo[k];
+					slot[6 * j + 3 + k] = od.boxes[o].hi[k];
+				}
+			}
+		}
+		auto ensure = [&](DevBuf &b, size_t bytes) {
+			if (b.bytes < bytes) b.alloc(dev, bytes);
+		};
+		// refit inputs; the edit's motion systems go to staging buffers first,
+		// so a rejected edit (a singular motion sample) leaves the scene as it was
+		ensure(s->rSlotBoxes, slot.size() * 4);
+		s->rSlotBoxes.upload(slot.data(), slot.size() * 4, st);
+		ensure(s->rLevelNodes, std::max<size_t>(s->levelNodes.size(), 1) * 4);
+		s->rLevelNodes.upload(s->levelNodes.data(), s->levelNodes.size() * 4, st);
+		const size_t nRoot = s->levelNodes.empty() ? 1 : *std::max_element(s->levelNodes.begin(), s->levelNodes.end()) + 1;
+		ensure(s->rNodeBoxes, 6 * 4 * nRoot);
+		ensure(s->rError, 4);
+		LrhRefitArgs a;
+		if (!mo.empty()) {
+			std::vector<float> mb(6 * s->meshBox.size());
+			for (size_t m = 0; m < s->meshBox.size(); ++m)
+				for (int k = 0; k < 3; ++k) {
+					mb[6 * m + k] = s->meshBox[m].lo[k];
+					mb[6 * m + 3 + k] = s->meshBox[m].hi[k];
+				}
+			ensure(s->rMeshBoxes, mb.size() * 4);
+			s->rMeshBoxes.upload(mb.data(), mb.size() * 4, st);
+			ensure(s->rMotionObjs, mo.size() * sizeof(LrhRefitMotion));
+			s->rMotionObjs.upload(mo.data(), mo.size() * sizeof(LrhRefitMotion), st);
+			ensure(s->rSampleBase, sampleBase.size() * 4);
+			s->rSampleBase.upload(sampleBase.data(), sampleBase.size() * 4, st);
+			ensure(s->rKeys, 6 * 4 * mo.size());
+			ensure(s->rMotions, od.mot.size() * sizeof(lrh_motion));
+			s->rMotions.upload(od.mot.data(), od.mot.size() * sizeof(lrh_motion), st);
+			ensure(s->rSegs, od.segs.size() * sizeof(lrh_motion_seg));
+			s->rSegs.upload(od.segs.data(), od.segs.size() * sizeof(lrh_motion_seg), st);
+			a.motionObjs = (const LrhRefitMotion *)s->rMotionObjs.p;
+			a.nMotion = (uint32_t)mo.size();
+			a.sampleBase = (const uint32_t *)s->rSampleBase.p;
+			a.nSamples = sampleBase.back();
+			a.motions = (const lrh_motion *)s->rMotions.p;
+			a.segs = (const lrh_motion_seg *)s->rSegs.p;
+			a.meshBoxes = (const float *)s->rMeshBoxes.p;
+			a.keys = (uint32_t *)s->rKeys.p;
+			a.error = (uint32_t *)s->rError.p;
+		}
+		a.slotBoxes = (float *)s->rSlotBoxes.p;
+		a.nodeBoxes = (float *)s->rNodeBoxes.p;
+		a.levelNodes = (const uint32_t *)s->rLevelNodes.p;
+		a.levelStart = s->levelStart.data();
+		a.nLevels = (uint32_t)s->levelStart.size() - 1;
+		a.layout = s->layout;
+		a.nodes = s->nodes.p;
+		HIPCHK(lrhLaunchRefitMotion(a, st));
+		if (!mo.empty()) {
+			uint32_t err = 0;
+			HIPCHK(hipMemcpyAsync(&err, s->rError.p, 4, hipMemcpyDeviceToHost, st));
+			HIPCHK(hipStreamSynchronize(st));
+			if (err) throw Error(LRH_EINVAL, "singular motion sample");
+		}
+		// accepted: the edit's matrices and motion systems, as lrh_scene_update uploads them
+		if (od.xf.size() * 4 > s->xforms.bytes) s->xforms.alloc(dev, od.xf.size() * 4);
+		s->xforms.upload(od.xf.data(), od.xf.size() * 4, st);
+		if (od.mot.size() * sizeof(lrh_motion) > s->motions.bytes) s->motions.alloc(dev, od.mot.size() * sizeof(lrh_motion));
+		s->motions.upload(od.mot.data(), od.mot.size() * sizeof(lrh_motion), st);
+		if (od.segs.size() * sizeof(lrh_motion_seg) > s->segs.bytes)
+			s->segs.alloc(dev, od.segs.size() * sizeof(lrh_motion_seg));
+		s->segs.upload(od.segs.data(), od.segs.size() * sizeof(lrh_motion_seg), st);
+		if (od.times.size() * 4 > s->times.bytes) s->times.alloc(dev, od.times.size() * 4);
+		s->times.upload(od.times.data(), od.times.size() * 4, st);
+		HIPCHK(lrhLaunchRefitLevels(a, st));
+		HIPCHK(hipStreamSynchronize(st));
+		s->boxesOnDevice = true;
+	});
+}
+
+int lrh_scene_object_boxes(lrh_scene *s, float *boxes, unsigned int nObjects) {
+	if (!s || (nObjects && !boxes)) return fail(LRH_EINVAL, "null argument");
+	if (!s->twoLevel) return fail(LRH_EINVAL, "object boxes exist for two-level scenes only");
+	if (nObjects != s->objMesh.size()) return fail(LRH_EINVAL, "object count differs from the scene");
+	return guarded([&]() {
+		if (!s->boxesOnDevice) {
+			for (unsigned o = 0; o < nObjects; ++o)
+				for (int k = 0; k < 3; ++k) {
+					boxes[6 * o + k] = s->objBoxes[o].lo[k];
+					boxes[6 * o + 3 + k] = s->objBoxes[o].hi[k];
+				}
+			return;
+		}
+		std::vector<float> slot(6 * (size_t)nObjects);
+		s->dev->activate();
+		HIPCHK(hipMemcpyAsync(slot.data(), s->rSlotBoxes.p, slot.size() * 4, hipMemcpyDeviceToHost, s->dev->stream));
+		HIPCHK(hipStreamSynchronize(s->dev->stream));
+		for (size_t j = 0; j < s->rootOrder.size(); ++j)
+			std::memcpy(boxes + 6 * (size_t)s->rootOrder[j], slot.data() + 6 * j, 24);
 	});
 }
 

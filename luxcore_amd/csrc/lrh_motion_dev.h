@@ -1,0 +1,93 @@
+/*
+ * InterpolatedTransform::Sample on the device
+ * (src/luxrays/core/geometry/motionsystem.cpp:89-159), rounded as written:
+ * the files that include it build with -ffp-contract=off and correctly
+ * rounded division; slerp's acos/sin are evaluated in double and rounded
+ * once, so host and device produce the same bits.
+ */
+#ifndef LRH_MOTION_DEV_H
+#define LRH_MOTION_DEV_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lrh_layout.h"
+
+__device__ __forceinline__ float lrhLerp(float t, float a, float b) { return a + t * (b - a); }
+__device__ __forceinline__ float lrhQdot(const float *a, const float *b) {
+	return a[0] * b[0] + (a[1] * b[1] + a[2] * b[2] + a[3] * b[3]);
+}
+
+// one segment at `time` -> row-major M (global->local for scene objects)
+__device__ __forceinline__ void lrhSegSample(const lrh_motion_seg &g, float time, float M[16]) {
+	const uint32_t f = g.flags;
+	if (!(f & LRH_MF_ACTIVE) || time <= g.startTime) {
+		for (int i = 0; i < 16; ++i) M[i] = g.start[i];
+		return;
+	}
+	if (time >= g.endTime) {
+		for (int i = 0; i < 16; ++i) M[i] = g.end[i];
+		return;
+	}
+	const float w = g.endTime - g.startTime;
+	const float d = time - g.startTime;
+	const float le = d / w;
+	if ((f & LRH_MF_TRANS) && !(f & (LRH_MF_SCALE | LRH_MF_ROT))) {
+		for (int i = 0; i < 16; ++i) M[i] = g.start[i];
+		if (f & LRH_MF_TX) M[3] = lrhLerp(le, g.sT[0], g.eT[0]);
+		if (f & LRH_MF_TY) M[7] = lrhLerp(le, g.sT[1], g.eT[1]);
+		if (f & LRH_MF_TZ) M[11] = lrhLerp(le, g.sT[2], g.eT[2]);
+		return;
+	}
+	if (f & LRH_MF_ROT) {
+		// Slerp (src/luxrays/core/geometry/quaternion.cpp:143-164) + ToMatrix (:115-140)
+		float cos_phi = lrhQdot(g.startQ, g.endQ);
+		const float sign = (cos_phi > 0.f) ? 1.f : -1.f;
+		cos_phi *= sign;
+		float f1, f2;
+		if (1.f - cos_phi > 1e-6f) {
+			// correctly rounded fp32 acos/sin: evaluated in double, rounded once,
+			// so every platform gets the same bits
+			const float phi = (float)acos((double)cos_phi);
+			const float sin_phi = (float)sin((double)phi);
+			f1 = (float)sin((double)((1.f - le) * phi)) / sin_phi;
+			f2 = (float)sin((double)(le * phi)) / sin_phi;
+		} else {
+			f1 = 1.f - le;
+			f2 = le;
+		}
+		const float gg = sign * f2;
+		float q[4];
+		for (int i = 0; i < 4; ++i) q[i] = g.startQ[i] * f1 + g.endQ[i] * gg;
+		const float x = q[1], y = q[2], z = q[3], qw = q[0];
+		const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z;
+		const float xw = x * qw, yw = y * qw, zw = z * qw;
+		M[0] = 1.f - 2.f * (yy + zz);
+		M[4] = 2.f * (xy - zw);
+		M[8] = 2.f * (xz + yw);
+		M[1] = 2.f * (xy + zw);
+		M[5] = 1.f - 2.f * (xx + zz);
+		M[9] = 2.f * (yz - xw);
+		M[2] = 2.f * (xz - yw);
+		M[6] = 2.f * (yz + xw);
+		M[10] = 1.f - 2.f * (xx + yy);
+		M[3] = M[7] = M[11] = 0.f;
+		M[12] = M[13] = M[14] = 0.f;
+		M[15] = 1.f;
+	} else {
+		for (int i = 0; i < 16; ++i) M[i] = g.startR[i];
+	}
+	float S[3];
+	for (int k = 0; k < 3; ++k) S[k] = (f & LRH_MF_SCALE) ? lrhLerp(le, g.sS[k], g.eS[k]) : g.sS[k];
+	for (int j = 0; j < 3; ++j) {
+		M[0 + j] = S[0] * M[0 + j];
+		M[4 + j] = S[1] * M[4 + j];
+		M[8 + j] = S[2] * M[8 + j];
+	}
+	M[3] = (f & LRH_MF_TX) ? lrhLerp(le, g.sT[0], g.eT[0]) : g.sT[0];
+	M[7] = (f & LRH_MF_TY) ? lrhLerp(le, g.sT[1], g.eT[1]) : g.sT[1];
+	M[11] = (f & LRH_MF_TZ) ? lrhLerp(le, g.sT[2], g.eT[2]) : g.sT[2];
+}
+
+
+#endif

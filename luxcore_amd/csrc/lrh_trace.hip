@@ -28,6 +28,7 @@
 
 #include "lrh_layout.h"
 #include "lrh_kernels.h"
+#include "lrh_motion_dev.h"
 
 #define LRH_SENTINEL 0xfffffffeu
 #define DEVI __device__ __forceinline__
@@ -83,11 +84,9 @@ DEVI void xfRay(const float *M, const float wo[3], const float wd[3], LocalRay &
 	setupRay(r);
 }
 
-// ---- motion sampling: InterpolatedTransform::Sample / MotionSystem::Sample
-// (src/luxrays/core/geometry/motionsystem.cpp:89-159, 340-346) ----
-DEVI float lerpf(float t, float a, float b) { return a + t * (b - a); }
-DEVI float qdot(const float *a, const float *b) { return a[0] * b[0] + (a[1] * b[1] + a[2] * b[2] + a[3] * b[3]); }
-
+// ---- motion sampling: MotionSystem::Sample picks the segment
+// (src/luxrays/core/geometry/motionsystem.cpp:340-346), lrhSegSample
+// evaluates it (InterpolatedTransform::Sample, :89-159) ----
 __device__ __noinline__ void motionSample(const lrh_motion *mot, const lrh_motion_seg *segs, const float *times,
 		float time, float M[16]) {
 	// upper_bound(times, time), clamped to the last key
@@ -96,74 +95,7 @@ __device__ __noinline__ void motionSample(const lrh_motion *mot, const lrh_motio
 	uint32_t idx = 0;
 	while (idx < n && !(time < tk[idx])) ++idx;
 	if (idx > n - 1) idx = n - 1;
-	const lrh_motion_seg &g = segs[mot->firstSeg + idx];
-	const uint32_t f = g.flags;
-	if (!(f & LRH_MF_ACTIVE) || time <= g.startTime) {
-		for (int i = 0; i < 16; ++i) M[i] = g.start[i];
-		return;
-	}
-	if (time >= g.endTime) {
-		for (int i = 0; i < 16; ++i) M[i] = g.end[i];
-		return;
-	}
-	const float w = g.endTime - g.startTime;
-	const float d = time - g.startTime;
-	const float le = d / w;
-	if ((f & LRH_MF_TRANS) && !(f & (LRH_MF_SCALE | LRH_MF_ROT))) {
-		for (int i = 0; i < 16; ++i) M[i] = g.start[i];
-		if (f & LRH_MF_TX) M[3] = lerpf(le, g.sT[0], g.eT[0]);
-		if (f & LRH_MF_TY) M[7] = lerpf(le, g.sT[1], g.eT[1]);
-		if (f & LRH_MF_TZ) M[11] = lerpf(le, g.sT[2], g.eT[2]);
-		return;
-	}
-	if (f & LRH_MF_ROT) {
-		// Slerp (src/luxrays/core/geometry/quaternion.cpp:143-164) + ToMatrix (:115-140)
-		float cos_phi = qdot(g.startQ, g.endQ);
-		const float sign = (cos_phi > 0.f) ? 1.f : -1.f;
-		cos_phi *= sign;
-		float f1, f2;
-		if (1.f - cos_phi > 1e-6f) {
-			// correctly rounded fp32 acos/sin: evaluated in double, rounded once,
-			// so every platform gets the same bits
-			const float phi = (float)acos((double)cos_phi);
-			const float sin_phi = (float)sin((double)phi);
-			f1 = (float)sin((double)((1.f - le) * phi)) / sin_phi;
-			f2 = (float)sin((double)(le * phi)) / sin_phi;
-		} else {
-			f1 = 1.f - le;
-			f2 = le;
-		}
-		const float gg = sign * f2;
-		float q[4];
-		for (int i = 0; i < 4; ++i) q[i] = g.startQ[i] * f1 + g.endQ[i] * gg;
-		const float x = q[1], y = q[2], z = q[3], qw = q[0];
-		const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z;
-		const float xw = x * qw, yw = y * qw, zw = z * qw;
-		M[0] = 1.f - 2.f * (yy + zz);
-		M[4] = 2.f * (xy - zw);
-		M[8] = 2.f * (xz + yw);
-		M[1] = 2.f * (xy + zw);
-		M[5] = 1.f - 2.f * (xx + zz);
-		M[9] = 2.f * (yz - xw);
-		M[2] = 2.f * (xz - yw);
-		M[6] = 2.f * (yz + xw);
-		M[10] = 1.f - 2.f * (xx + yy);
-		M[3] = M[7] = M[11] = 0.f;
-		M[12] = M[13] = M[14] = 0.f;
-		M[15] = 1.f;
-	} else {
-		for (int i = 0; i < 16; ++i) M[i] = g.startR[i];
-	}
-	float S[3];
-	for (int k = 0; k < 3; ++k) S[k] = (f & LRH_MF_SCALE) ? lerpf(le, g.sS[k], g.eS[k]) : g.sS[k];
-	for (int j = 0; j < 3; ++j) {
-		M[0 + j] = S[0] * M[0 + j];
-		M[4 + j] = S[1] * M[4 + j];
-		M[8 + j] = S[2] * M[8 + j];
-	}
-	M[3] = (f & LRH_MF_TX) ? lerpf(le, g.sT[0], g.eT[0]) : g.sT[0];
-	M[7] = (f & LRH_MF_TY) ? lerpf(le, g.sT[1], g.eT[1]) : g.sT[1];
-	M[11] = (f & LRH_MF_TZ) ? lerpf(le, g.sT[2], g.eT[2]) : g.sT[2];
+	lrhSegSample(segs[mot->firstSeg + idx], time, M);
 }
 
 // ---- node tests: up to 4 children -> (entry distance, child ref) ----

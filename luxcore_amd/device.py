@@ -104,6 +104,8 @@ SIGNATURES = {
     "lrh_scene_create_mbvh": ([_vp, ctypes.POINTER(LrhMesh), ctypes.c_uint, ctypes.POINTER(LrhObject),
                                ctypes.c_uint, ctypes.POINTER(LrhBuildOpts), _pp], ctypes.c_int),
     "lrh_scene_update": ([_vp, ctypes.POINTER(LrhObject), ctypes.c_uint], ctypes.c_int),
+    "lrh_scene_refit": ([_vp, ctypes.POINTER(LrhObject), ctypes.c_uint], ctypes.c_int),
+    "lrh_scene_object_boxes": ([_vp, ctypes.POINTER(ctypes.c_float), ctypes.c_uint], ctypes.c_int),
     "lrh_scene_destroy": ([_vp], ctypes.c_int),
     "lrh_scene_get_info": ([_vp, ctypes.POINTER(LrhSceneInfo)], ctypes.c_int),
     "lrh_scene_download": ([_vp, _vp, ctypes.c_size_t, _vp, ctypes.c_size_t], ctypes.c_int),
@@ -297,6 +299,17 @@ class Scene:
     def update(self, inst_scene):
         arr, keep = _object_structs(inst_scene)
         check(lib().lrh_scene_update(self.h, arr, len(inst_scene.obj_mesh)))
+
+    def refit(self, inst_scene):
+        """lrh_scene_refit: new matrices / motion keys, root tree refitted on the GPU."""
+        arr, keep = _object_structs(inst_scene)
+        check(lib().lrh_scene_refit(self.h, arr, len(inst_scene.obj_mesh)))
+
+    def object_boxes(self, n_objects):
+        """(n, 2, 3) float32 culling boxes of the objects (lo, hi)."""
+        out = np.zeros((n_objects, 6), np.float32)
+        check(lib().lrh_scene_object_boxes(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n_objects))
+        return out.reshape(n_objects, 2, 3)
 
     def download(self):
         inf = self.info()

@@ -163,3 +163,17 @@ def test_device_builder_needs_a_device():
     assert b"needs a device" in D.lib().lrh_last_error()
     opts.builder = 7
     assert D.lib().lrh_scene_create_bvh(None, arr, 1, ctypes.byref(opts), ctypes.byref(p)) == -1
+
+
+def test_object_boxes_equal_oracle_leaf_boxes():
+    """The culling box of every object (instance: transformed mesh box;
+    motion: MotionSystem::Bound over 1,025 samples per segment; both
+    MachineEpsilon-expanded) equals the oracle MBVHAccel's root-leaf box."""
+    import oracle
+    sc = scenes.instanced_scene(n_inst=60, motion_frac=0.5, freq=6, extent=20.0, seed=61)
+    n = len(sc.obj_kind)
+    got = D.host_scene_mbvh(sc).object_boxes(n)
+    ob = oracle.OracleMBVH(sc)
+    for o in range(n):
+        lo, hi = ob.leaf_bbox(o)
+        assert np.array_equal(got[o, 0], lo) and np.array_equal(got[o, 1], hi), o
