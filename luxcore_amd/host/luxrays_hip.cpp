@@ -158,7 +158,10 @@ void Accelerator::Update() {
 	std::vector<lrh_object> objs;
 	std::vector<float> store;
 	fillObjects(objs, store);
-	CheckLrh(lrh_scene_update(scene, objs.data(), (unsigned int)objs.size()));
+	if (updateMode == UPDATE_REFIT)
+		CheckLrh(lrh_scene_refit(scene, objs.data(), (unsigned int)objs.size()));
+	else
+		CheckLrh(lrh_scene_update(scene, objs.data(), (unsigned int)objs.size()));
 }
 
 bool Accelerator::Intersect(const Ray *ray, RayHit *hit) const {

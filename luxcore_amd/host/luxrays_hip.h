@@ -195,6 +195,11 @@ public:
 	void Init(const std::deque<const Mesh *> &meshes, unsigned long long totalVertexCount,
 			unsigned long long totalTriangleCount);
 	bool DoesSupportUpdate() const { return type == ACCEL_MBVH; }
+	// Update() rebuilds the root tree on the host as MBVHAccel::Update does
+	// (UPDATE_REBUILD, the default) or refits it on the GPU (UPDATE_REFIT:
+	// the same object boxes, the old grouping; lrh_scene_refit)
+	enum UpdateMode { UPDATE_REBUILD, UPDATE_REFIT };
+	void SetUpdateMode(UpdateMode m) { updateMode = m; }
 	void Update();
 	bool Intersect(const Ray *ray, RayHit *hit) const;
 	HardwareIntersectionKernel *NewHardwareIntersectionKernel(HIPIntersectionDevice &device) const;
@@ -203,6 +208,7 @@ public:
 private:
 	void fillObjects(std::vector<lrh_object> &objs, std::vector<float> &store) const;
 	HIPIntersectionDevice &device;
+	UpdateMode updateMode = UPDATE_REBUILD;
 	AcceleratorType type;
 	std::deque<const Mesh *> meshes;
 	std::vector<const TriangleMesh *> baseMeshes;

@@ -5,8 +5,10 @@
  * FinishQueue, TraceRay, Update after a transform edit) on a scene read from
  * files, and writes the hit records.
  *
- *   luxrays_hip_check <dir> [--accel auto|bvh|mbvh] [--anyhit] [--serial N] [--scene FILE]
+ *   luxrays_hip_check <dir> [--accel auto|bvh|mbvh] [--anyhit] [--serial N] [--scene FILE] [--refit]
  *
+ * --refit            Update() refits the root tree on the GPU instead of
+ *                    rebuilding it on the host
  * --scene FILE       meshes and objects from a LuxCore scene file (.scn/.cfg,
  *                    PLY meshes) through libluxrays_ingest instead of the two
  *                    files below
@@ -105,7 +107,8 @@ MotionSystem motion(const ObjDesc &o) {
 
 int main(int argc, char **argv) {
 	if (argc < 2) {
-		std::cerr << "usage: luxrays_hip_check <dir> [--accel auto|bvh|mbvh] [--anyhit] [--serial N] [--scene FILE]\n";
+		std::cerr << "usage: luxrays_hip_check <dir> [--accel auto|bvh|mbvh] [--anyhit] [--serial N] [--scene FILE] "
+		             "[--refit]\n";
 		return 2;
 	}
 	const std::string dir = argv[1];
@@ -113,6 +116,7 @@ int main(int argc, char **argv) {
 	bool anyHit = false;
 	u_int serial = 0;
 	std::string sceneFile;
+	bool refit = false;
 	for (int i = 2; i < argc; ++i) {
 		const std::string a = argv[i];
 		if (a == "--accel" && i + 1 < argc) {
@@ -124,6 +128,8 @@ int main(int argc, char **argv) {
 			serial = (u_int)std::stoul(argv[++i]);
 		else if (a == "--scene" && i + 1 < argc)
 			sceneFile = argv[++i];
+		else if (a == "--refit")
+			refit = true;
 		else {
 			std::cerr << "unknown argument " << a << "\n";
 			return 2;
@@ -261,6 +267,8 @@ int main(int argc, char **argv) {
 				else if (o2[i].kind == 2)
 					static_cast<MotionTriangleMesh *>(objects[i])->SetMotionSystem(motion(o2[i]));
 			}
+			if (refit)
+				const_cast<Accelerator *>(device.GetAccelerator())->SetUpdateMode(Accelerator::UPDATE_REFIT);
 			device.Update();
 			traceAll("hits2.bin");
 		}

@@ -161,6 +161,11 @@ def test_cpp_device_two_level_and_update(gpu_device, tmp_path):
     assert np.array_equal(got2, gpu_device.trace_numpy(dsc2, rays))
     assert not np.array_equal(got, got2)
     assert np.array_equal(read_hits(tmp_path, "serial.bin"), got[:256])
+    # the same edit through a GPU refit: same hits as the refitted C-ABI scene
+    run_check(tmp_path, "--refit")
+    got3 = read_hits(tmp_path, "hits2.bin")
+    dsc.refit(sc2)
+    assert np.array_equal(got3, gpu_device.trace_numpy(dsc, rays))
     dsc.close()
     dsc2.close()
 
