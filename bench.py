@@ -318,14 +318,91 @@ def run_refit(args, dev):
             "speedup": round(res["update"]["edit_ms_median"] / res["refit"]["edit_ms_median"], 1)}
 
 
+def run_configs(args, dev):
+    """Closest-hit and any-hit rates of every workload the parity tests use,
+    HIP-event timed, with the oracle on this host's cores beside each
+    closest-hit set (bounded sample, median pass):
+      plain        box + level-6 icosphere + 20,000 random triangles (101,932),
+                   1,048,576 rays uniform in the box;
+      heightfield  the 26.2M-triangle headline scene: 1920x1080 primaries
+                   (coherent), one cosine bounce per hit (incoherent, the
+                   headline set) and one shadow ray per hit towards a 1x1 area
+                   light (any-hit);
+      instanced    1,000 instances of a 50,000-triangle sphere, 200 with 2-key
+                   motion, 4,194,304 rays with time ~ U[0,1]."""
+    import oracle  # the checker, timed here only as the CPU baseline
+    threads = os.cpu_count() or 1
+    out = {}
+
+    def rate(scene, rays, query):
+        n = rays.shape[0]
+        rb = dev.upload(rays)
+        hb = dev.alloc(n * HIT_DTYPE.itemsize)
+        wall, lat = _event_timed(dev, lambda: dev.trace(scene, rb, hb, n, query), args.steps, args.warmup)
+        hits = dev.download(hb, HIT_DTYPE, n)
+        rb.free()
+        hb.free()
+        return {"rays": int(n), "mrays": round(n * args.steps / wall / 1e6, 1),
+                "kernel_ms": round(float(lat.mean()), 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),
+                "hit_rate": round(float((hits["meshIndex"] != NULL_INDEX).mean()), 4)}
+
+    def cpu(ob, rays, seconds=4.0):
+        probe = rays[: 1 << 14]
+        t0 = time.time()
+        ob.intersect(probe, threads=threads)
+        r = probe.shape[0] / max(time.time() - t0, 1e-6)
+        sample = rays[: int(min(rays.shape[0], max(1 << 14, r * seconds / 6)))]
+        v, passes, spent = _timed_passes(ob, sample, threads, seconds, min_passes=3)
+        return {"mrays": round(v, 3), "sample_rays": int(sample.shape[0]), "passes": passes, "threads": threads}
+
+    # plain
+    meshes = scenes.plain_scene()
+    sc = dev.scene_bvh(meshes)
+    rays = scenes.plain_rays(1 << 20)
+    out["plain"] = {"triangles": sum(m.ntris for m in meshes), "closest": rate(sc, rays, 0),
+                    "any": rate(sc, rays, 1)}
+    if not args.no_cpu:
+        out["plain"]["cpu_closest"] = cpu(oracle.OracleBVH(meshes, threads=threads), rays)
+    sc.close()
+    log(f"plain: {out['plain']}")
+    # heightfield
+    meshes = scenes.heightfield_scene()
+    sc = dev.scene_bvh(meshes)
+    prim = scenes.pinhole_rays(1920, 1080)
+    h = dev.trace_numpy(sc, prim)
+    bounce = scenes.bounce_rays(meshes, prim, h, seed=5)
+    shadow = scenes.shadow_rays(meshes, prim, h, seed=6)
+    out["heightfield"] = {"triangles": sum(m.ntris for m in meshes), "camera_closest": rate(sc, prim, 0),
+                          "bounce_closest": rate(sc, bounce, 0), "shadow_any": rate(sc, shadow, 1),
+                          "shadow_closest": rate(sc, shadow, 0)}
+    sc.close()
+    log(f"heightfield: {out['heightfield']}")
+    # instanced
+    isc = scenes.instanced_scene()
+    sc = dev.scene_mbvh(isc)
+    rays = scenes.instanced_rays(1 << 22)
+    out["instanced"] = {"objects": int(len(isc.obj_kind)), "motion_objects": int((isc.obj_kind == 2).sum()),
+                        "closest": rate(sc, rays, 0), "any": rate(sc, rays, 1)}
+    if not args.no_cpu:
+        out["instanced"]["cpu_closest"] = cpu(oracle.OracleMBVH(isc, threads=threads), rays)
+    sc.close()
+    log(f"instanced: {out['instanced']}")
+    return {"metric": "Mrays/s per workload (closest / any hit)", "value": out["heightfield"]["bounce_closest"]["mrays"],
+            "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "plain 101,932 tris / heightfield 26.2M tris / 1,000 instances"}, "configs": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--mode", choices=("headline", "batched", "served", "scene", "refit"), default="headline",
+    ap.add_argument("--mode", choices=("headline", "batched", "served", "scene", "refit", "configs"),
+                    default="headline",
                     help="headline: incoherent closest-hit on the 26.2M-tri scene (the driver's line); "
                          "batched: 64 buffers over 8 resident scenes in one launch; served: continuous batching; "
                          "scene: a LuxCore scene file (--scene, default the bundled lightinstances scene); "
-                         "refit: two-level edit latency, host rebuild (lrh_scene_update) vs GPU refit")
+                         "refit: two-level edit latency, host rebuild (lrh_scene_update) vs GPU refit; "
+                         "configs: closest/any-hit rates of the plain, heightfield and instanced workloads")
     ap.add_argument("--scene", default=None, help="scene file for --mode scene (.scn/.cfg)")
     ap.add_argument("--scene-rays", type=int, default=1 << 22, help="incoherent rays for --mode scene")
     ap.add_argument("--steps", type=int, default=200)
@@ -365,6 +442,9 @@ def main():
         return
     if args.mode == "refit":
         print(json.dumps(run_refit(args, dev)), flush=True)
+        return
+    if args.mode == "configs":
+        print(json.dumps(run_configs(args, dev)), flush=True)
         return
     meshes, scene, info, rays, ntris = build_workload(args, dev)
     n = rays.shape[0]
