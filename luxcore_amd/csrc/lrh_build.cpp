@@ -272,24 +272,31 @@ void quatToMatrix(const float q[4], float m[4][4]) {  // quaternion.cpp:115-140
 	m[3][3] = 1.f;
 }
 
-void slerp(float t, const float q1[4], const float q2[4], float out[4]) {  // quaternion.cpp:143-164
-	float cos_phi = qdot(q1, q2);
-	const float sign = (cos_phi > 0.f) ? 1.f : -1.f;
-	cos_phi *= sign;
+void slerpSeg(float t, const lrh_motion_seg &g, float out[4]) {  // quaternion.cpp:143-164
 	float f1, f2;
-	if (1.f - cos_phi > 1e-6f) {
-		// correctly rounded fp32 acos/sin (double evaluation, one rounding),
-		// the same values the device computes
-		const float phi = (float)acos((double)cos_phi);
-		const float sin_phi = (float)sin((double)phi);
-		f1 = (float)sin((double)((1.f - t) * phi)) / sin_phi;
-		f2 = (float)sin((double)(t * phi)) / sin_phi;
+	if (g.slerpArc) {
+		// correctly rounded fp32 sin (double evaluation, one rounding), the
+		// same values the device computes; phi and sin(phi) come with the segment
+		f1 = (float)sin((double)((1.f - t) * g.phi)) / g.sinPhi;
+		f2 = (float)sin((double)(t * g.phi)) / g.sinPhi;
 	} else {
 		f1 = 1.f - t;
 		f2 = t;
 	}
-	const float g = sign * f2;
-	for (int i = 0; i < 4; ++i) out[i] = q1[i] * f1 + q2[i] * g;
+	const float gg = g.qsign * f2;
+	for (int i = 0; i < 4; ++i) out[i] = g.startQ[i] * f1 + g.endQ[i] * gg;
+}
+
+// the time-independent part of Slerp: the shorter-arc sign, phi, sin(phi)
+void slerpConstants(lrh_motion_seg &g) {
+	float cos_phi = qdot(g.startQ, g.endQ);
+	g.qsign = (cos_phi > 0.f) ? 1.f : -1.f;
+	cos_phi *= g.qsign;
+	g.slerpArc = (1.f - cos_phi > 1e-6f) ? 1u : 0u;
+	if (g.slerpArc) {
+		g.phi = (float)acos((double)cos_phi);
+		g.sinPhi = (float)sin((double)g.phi);
+	}
 }
 
 lrh_motion_seg makeSeg(float st, float et, const M4 &s, const M4 &e) {
@@ -327,6 +334,7 @@ lrh_motion_seg makeSeg(float st, float et, const M4 &s, const M4 &e) {
 	if (fabsf(qdot(g.startQ, g.endQ) - 1.f) >= 1e-6f) f |= LRH_MF_ROT;
 	if (f & (LRH_MF_TRANS | LRH_MF_SCALE | LRH_MF_ROT)) f |= LRH_MF_ACTIVE;
 	g.flags = f;
+	slerpConstants(g);
 	return g;
 }
 
@@ -346,7 +354,7 @@ M4 segSample(const lrh_motion_seg &g, float time) {  // InterpolatedTransform::S
 	}
 	if (g.flags & LRH_MF_ROT) {
 		float q[4];
-		slerp(le, g.startQ, g.endQ, q);
+		slerpSeg(le, g, q);
 		quatToMatrix(q, r.m);
 	} else
 		std::memcpy(r.m, g.startR, 64);

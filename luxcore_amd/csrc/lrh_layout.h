@@ -99,6 +99,12 @@ typedef struct __attribute__((aligned(16))) {
 	float endQ[4];
 	float sS[3], eS[3]; /* Sx Sy Sz of startT / endT */
 	float sT[3], eT[3]; /* Tx Ty Tz of startT / endT */
+	/* Slerp's per-segment constants (quaternion.cpp:143-164): the sign that
+	 * takes the shorter arc, and, when 1 - cos_phi > 1e-6 (slerpArc = 1),
+	 * phi = acos(cos_phi) and sin(phi), each evaluated in double and rounded
+	 * once to float; only sin((1-t) phi) and sin(t phi) depend on the time */
+	float qsign, phi, sinPhi;
+	uint32_t slerpArc;
 } lrh_motion_seg;
 
 /* A motion system: keys times[0..nTimes) and nTimes+1 segments. */

@@ -2,8 +2,10 @@
  * InterpolatedTransform::Sample on the device
  * (src/luxrays/core/geometry/motionsystem.cpp:89-159), rounded as written:
  * the files that include it build with -ffp-contract=off and correctly
- * rounded division; slerp's acos/sin are evaluated in double and rounded
- * once, so host and device produce the same bits.
+ * rounded division; slerp's sines are evaluated in double and rounded
+ * once, and its time-independent constants (sign, phi = acos(cos_phi),
+ * sin(phi)) come with the segment from the host, so host and device
+ * produce the same bits.
  */
 #ifndef LRH_MOTION_DEV_H
 #define LRH_MOTION_DEV_H
@@ -14,9 +16,6 @@
 #include "lrh_layout.h"
 
 __device__ __forceinline__ float lrhLerp(float t, float a, float b) { return a + t * (b - a); }
-__device__ __forceinline__ float lrhQdot(const float *a, const float *b) {
-	return a[0] * b[0] + (a[1] * b[1] + a[2] * b[2] + a[3] * b[3]);
-}
 
 // one segment at `time` -> row-major M (global->local for scene objects)
 __device__ __forceinline__ void lrhSegSample(const lrh_motion_seg &g, float time, float M[16]) {
@@ -40,23 +39,19 @@ __device__ __forceinline__ void lrhSegSample(const lrh_motion_seg &g, float time
 		return;
 	}
 	if (f & LRH_MF_ROT) {
-		// Slerp (src/luxrays/core/geometry/quaternion.cpp:143-164) + ToMatrix (:115-140)
-		float cos_phi = lrhQdot(g.startQ, g.endQ);
-		const float sign = (cos_phi > 0.f) ? 1.f : -1.f;
-		cos_phi *= sign;
+		// Slerp (src/luxrays/core/geometry/quaternion.cpp:143-164) + ToMatrix (:115-140);
+		// the segment holds the shorter-arc sign, phi and sin(phi)
 		float f1, f2;
-		if (1.f - cos_phi > 1e-6f) {
-			// correctly rounded fp32 acos/sin: evaluated in double, rounded once,
+		if (g.slerpArc) {
+			// correctly rounded fp32 sin: evaluated in double, rounded once,
 			// so every platform gets the same bits
-			const float phi = (float)acos((double)cos_phi);
-			const float sin_phi = (float)sin((double)phi);
-			f1 = (float)sin((double)((1.f - le) * phi)) / sin_phi;
-			f2 = (float)sin((double)(le * phi)) / sin_phi;
+			f1 = (float)sin((double)((1.f - le) * g.phi)) / g.sinPhi;
+			f2 = (float)sin((double)(le * g.phi)) / g.sinPhi;
 		} else {
 			f1 = 1.f - le;
 			f2 = le;
 		}
-		const float gg = sign * f2;
+		const float gg = g.qsign * f2;
 		float q[4];
 		for (int i = 0; i < 4; ++i) q[i] = g.startQ[i] * f1 + g.endQ[i] * gg;
 		const float x = q[1], y = q[2], z = q[3], qw = q[0];
