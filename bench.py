@@ -405,7 +405,9 @@ def main():
                          "configs: closest/any-hit rates of the plain, heightfield and instanced workloads")
     ap.add_argument("--scene", default=None, help="scene file for --mode scene (.scn/.cfg)")
     ap.add_argument("--scene-rays", type=int, default=1 << 22, help="incoherent rays for --mode scene")
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed traces (default 1,000 for the headline, whose p50/p99 wavefront latency is "
+                         "taken over them; 20 per set in the other modes)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cells", type=int, default=3240)
     ap.add_argument("--layout", type=int, default=0)
@@ -428,6 +430,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.steps is None:
+        args.steps = 1000 if args.mode == "headline" else 20
     dev = D.Device(0)
     log(f"device {dev.name}, {dev.cu_count} CUs")
     if args.mode == "batched":
