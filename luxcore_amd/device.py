@@ -18,6 +18,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liblrhip.so")
 
 LRH_OK = 0
+LRH_EINVAL, LRH_EHIP, LRH_ENOMEM, LRH_ERANGE, LRH_ENODEV, LRH_EDEVICE = -1, -2, -3, -4, -5, -6
 LRH_H2D, LRH_D2H, LRH_D2D = 1, 2, 3
 LRH_QUERY_CLOSEST, LRH_QUERY_ANY = 0, 1
 TRACE_STATS = 8  # LRH_TRACE_STATS

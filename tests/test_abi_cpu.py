@@ -1,6 +1,7 @@
 """The C-ABI library loads and exports every function include/lrhip.h
 declares; the host-side builder produces well-formed device records
 (CPU only: no device call is made)."""
+import ctypes
 import os
 import re
 
@@ -177,3 +178,28 @@ def test_object_boxes_equal_oracle_leaf_boxes():
     for o in range(n):
         lo, hi = ob.leaf_bbox(o)
         assert np.array_equal(got[o, 0], lo) and np.array_equal(got[o, 1], hi), o
+
+
+def test_documented_limits_fail_cleanly():
+    """Size limits are checked before anything touches a device: 2^31-1 rays
+    per trace, query 0/1, maxLeaf 1..8 (0: default), layouts 0/1; updates and refits
+    need a two-level device scene with the same objects."""
+    L = D.lib()
+    host = D.host_scene_bvh(scenes.plain_scene(n_random=10))
+    fake = ctypes.c_void_p(1 << 20)
+    assert L.lrh_trace(host.h, fake, fake, 0x80000000, 0, None) == D.LRH_ERANGE
+    assert b"2^31-1" in L.lrh_last_error()
+    assert L.lrh_trace(host.h, fake, fake, 16, 2, None) == -1
+    assert L.lrh_trace(host.h, fake, fake, 16, 0, None) == -1  # host-only scene
+    assert b"host-only" in L.lrh_last_error()
+    assert L.lrh_scene_refit(host.h, None, 0) == -1
+    assert L.lrh_scene_update(host.h, None, 0) == -1
+    for bad in (dict(max_leaf=9), dict(layout=2)):  # 0 selects the default
+        with pytest.raises(D.LrhError):
+            D.host_scene_bvh(scenes.plain_scene(n_random=10), **bad)
+    sc = scenes.instanced_scene(n_inst=6, motion_frac=0.5, freq=4, extent=5.0, seed=3)
+    two = D.host_scene_mbvh(sc)
+    with pytest.raises(D.LrhError):
+        two.object_boxes(5)  # wrong object count
+    arr, keep = D._object_structs(sc)
+    assert L.lrh_scene_refit(two.h, arr, len(sc.obj_kind)) == -1  # host-only
