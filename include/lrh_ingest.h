@@ -98,6 +98,15 @@ int lrh_scenefile_object_name(const lrh_scenefile *sf, unsigned int obj, const c
 
 int lrh_scenefile_get_camera(const lrh_scenefile *sf, lrh_scenefile_camera *cam);
 
+/* The object's material reduced to what the Lambertian shading step of
+ * lrh_shade uses (src/slg/scene/parsematerials.cpp:143-186, 555-563):
+ * kd = .kd (default 0.75 grey; one or three numbers, or a constfloat1 /
+ * constfloat3 texture), le = .emission x .emission.gain (0 when .emission is
+ * absent or black).  Material types other than matte keep their .kd (or the
+ * default) as a Lambertian stand-in; textures other than constants read as
+ * the default.  The emission power/efficiency normalisation is not applied. */
+int lrh_scenefile_object_material(const lrh_scenefile *sf, unsigned int obj, float kd[3], float le[3]);
+
 int lrh_scenefile_destroy(lrh_scenefile *sf);
 
 const char *lrh_ingest_last_error(void);

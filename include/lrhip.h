@@ -247,6 +247,51 @@ int lrh_rays_bounce(lrh_scene *scene, const void *rays, const void *hits, unsign
 		unsigned int bounce, unsigned long long seed, unsigned int pixelBase, unsigned int rrStart,
 		float rrSurvive, const float lightCenter[3], void *nextRays, void *shadowRays,
 		unsigned int *liveCount /* device counter or NULL: += live rays written */, void *stream);
+/* One shading step of a path-tracing wavefront: the part of SLG's
+ * AdvancePaths micro-kernels that consumes a hit (MK_HIT_OBJECT,
+ * pathoclbase_kernels_micro.cl:204-330), samples direct light (MK_DL_ILLUMINATE /
+ * MK_DL_SAMPLE_BSDF, :450-560) and generates the next vertex ray
+ * (MK_GENERATE_NEXT_VERTEX_RAY, :1045-1075), reduced to Lambertian surfaces
+ * (MatteMaterial kd) and triangle area lights with next-event estimation:
+ *   - a shadow ray of the previous step that missed adds its pending light;
+ *   - emission is added where the camera sees a light directly (depth 0);
+ *   - one light triangle is picked uniformly, a point on it uniformly by
+ *     area; its contribution kd/pi * Le * cosS * cosL / d^2 / pdf waits for
+ *     the shadow ray this step writes (masked when there is none);
+ *   - the next ray is cosine-weighted (throughput *= kd), with Russian
+ *     roulette from rrDepth on; paths end at maxDepth vertices.
+ * Random numbers hash (seed, pixelBase + i, depth, dimension).  Single-level
+ * scenes; materials are per DataSet mesh; all arrays are device memory. */
+typedef struct {
+	float kd[3];      /* diffuse reflectance */
+	float le[3];      /* emitted radiance (0: not a light) */
+} lrh_material;
+typedef struct {
+	float p0[3], e1[3], e2[3];   /* world-space triangle: p0, p1 - p0, p2 - p0 */
+	float le[3];                 /* emitted radiance */
+} lrh_light_tri;
+typedef struct {
+	float thr[3];     /* path throughput */
+	float rad[3];     /* radiance gathered so far (the pixel sample) */
+	float dl[3];      /* direct light waiting for its shadow ray */
+	unsigned int depth, alive, pad;
+} lrh_path;
+typedef struct {
+	unsigned int count;          /* paths in the wavefront */
+	unsigned int maxDepth;       /* path vertices */
+	unsigned int rrDepth;        /* Russian roulette from this vertex on */
+	float rrSurvive;             /* survival probability */
+	unsigned long long seed;
+	unsigned int pixelBase;
+	unsigned int pad;
+} lrh_shade_params;
+/* Fresh paths (throughput 1, nothing gathered). */
+int lrh_paths_init(lrh_device *dev, void *paths, unsigned int count, void *stream);
+int lrh_shade(lrh_scene *scene, const lrh_material *materials, unsigned int nMaterials,
+		const lrh_light_tri *lights, unsigned int nLights, const void *rays, const void *hits,
+		const void *shadowHits /* hits of the previous step's shadow rays, or NULL */, void *paths,
+		void *nextRays, void *shadowRays, const lrh_shade_params *params,
+		unsigned int *liveCount /* device counter or NULL: += live rays written */, void *stream);
 /* 1 if all work before the event has completed, 0 if not yet, < 0 on error. */
 int lrh_event_query(lrh_event *ev);
 

@@ -1477,6 +1477,64 @@ int lrh_rays_bounce(lrh_scene *s, const void *rays, const void *hits, unsigned i
 	});
 }
 
+static_assert(sizeof(lrh_material) == sizeof(LrhMaterial), "lrh_material layout");
+static_assert(sizeof(lrh_light_tri) == sizeof(LrhLightTri), "lrh_light_tri layout");
+static_assert(sizeof(lrh_path) == sizeof(LrhPath) && sizeof(lrh_path) == 48, "lrh_path layout");
+
+int lrh_paths_init(lrh_device *dev, void *paths, unsigned int count, void *stream) {
+	if (!dev || (count && !paths)) return fail(LRH_EINVAL, "null argument");
+	if ((uintptr_t)paths & 3u) return fail(LRH_EINVAL, "misaligned path buffer");
+	if (count == 0) return LRH_OK;
+	return guarded([&]() {
+		dev->activate();
+		HIPCHK(lrhLaunchPathsInit((LrhPath *)paths, count, stream ? (hipStream_t)stream : dev->stream));
+	});
+}
+
+int lrh_shade(lrh_scene *s, const lrh_material *materials, unsigned int nMaterials, const lrh_light_tri *lights,
+		unsigned int nLights, const void *rays, const void *hits, const void *shadowHits, void *paths,
+		void *nextRays, void *shadowRays, const lrh_shade_params *prm, unsigned int *liveCount, void *stream) {
+	if (!s || !s->dev || !prm) return fail(LRH_EINVAL, "null argument");
+	if (s->twoLevel) return fail(LRH_EINVAL, "shading needs a single-level scene");
+	const unsigned count = prm->count;
+	if (count && (!rays || !hits || !paths || !nextRays || !shadowRays))
+		return fail(LRH_EINVAL, "null ray, hit or path buffer");
+	if ((nMaterials && !materials) || (nLights && !lights)) return fail(LRH_EINVAL, "null material or light array");
+	if (((uintptr_t)rays & 15u) || ((uintptr_t)nextRays & 15u) || ((uintptr_t)shadowRays & 15u) ||
+			((uintptr_t)hits & 3u) || ((uintptr_t)shadowHits & 3u) || ((uintptr_t)paths & 3u) ||
+			((uintptr_t)materials & 3u) || ((uintptr_t)lights & 3u))
+		return fail(LRH_EINVAL, "misaligned buffer");
+	if (prm->maxDepth == 0) return fail(LRH_EINVAL, "maxDepth must be at least 1");
+	if (!(prm->rrSurvive > 0.f && prm->rrSurvive <= 1.f)) return fail(LRH_EINVAL, "rrSurvive must be in (0, 1]");
+	if (count == 0) return LRH_OK;
+	return guarded([&]() {
+		s->dev->activate();
+		LrhShadeParams P;
+		P.rays = (const lrh_ray_rec *)rays;
+		P.hits = (const lrh_hit_rec *)hits;
+		P.shadowHits = (const lrh_hit_rec *)shadowHits;
+		P.paths = (LrhPath *)paths;
+		P.next = (lrh_ray_rec *)nextRays;
+		P.shadow = (lrh_ray_rec *)shadowRays;
+		P.materials = (const LrhMaterial *)materials;
+		P.nMaterials = nMaterials;
+		P.lights = (const LrhLightTri *)lights;
+		P.nLights = nLights;
+		P.tris = (const lrh_tri *)s->tris.p;
+		P.invPerm = (const uint32_t *)s->invPerm.p;
+		P.meshTriBase = (const uint32_t *)s->meshTriBase.p;
+		P.nMeshes = s->nMeshes;
+		P.count = count;
+		P.maxDepth = prm->maxDepth;
+		P.rrDepth = prm->rrDepth;
+		P.rrSurvive = prm->rrSurvive;
+		P.seed = prm->seed;
+		P.pixelBase = prm->pixelBase;
+		P.liveCount = liveCount;
+		HIPCHK(lrhLaunchShade(P, stream ? (hipStream_t)stream : s->dev->stream));
+	});
+}
+
 int lrh_event_query(lrh_event *ev) {
 	if (!ev) return fail(LRH_EINVAL, "null event");
 	const hipError_t e = hipEventQuery(ev->ev);

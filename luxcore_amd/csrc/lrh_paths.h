@@ -33,7 +33,37 @@ struct LrhBounceParams {
 	uint32_t *liveCount;          /* may be null: += live next + shadow rays */
 };
 
+struct LrhMaterial { float kd[3], le[3]; };
+struct LrhLightTri { float p0[3], e1[3], e2[3], le[3]; };
+struct LrhPath { float thr[3], rad[3], dl[3]; uint32_t depth, alive, pad; };
+
+struct LrhShadeParams {
+	const lrh_ray_rec *rays;
+	const lrh_hit_rec *hits;
+	const lrh_hit_rec *shadowHits;  /* may be null */
+	LrhPath *paths;
+	lrh_ray_rec *next;
+	lrh_ray_rec *shadow;
+	const LrhMaterial *materials;
+	uint32_t nMaterials;
+	const LrhLightTri *lights;
+	uint32_t nLights;
+	const lrh_tri *tris;
+	const uint32_t *invPerm;
+	const uint32_t *meshTriBase;
+	uint32_t nMeshes;
+	uint32_t count;
+	uint32_t maxDepth;
+	uint32_t rrDepth;
+	float rrSurvive;
+	uint64_t seed;
+	uint32_t pixelBase;
+	uint32_t *liveCount;
+};
+
 hipError_t lrhLaunchCamera(const LrhCameraParams &C, lrh_ray_rec *rays, hipStream_t s);
+hipError_t lrhLaunchPathsInit(LrhPath *paths, uint32_t count, hipStream_t s);
+hipError_t lrhLaunchShade(const LrhShadeParams &P, hipStream_t s);
 hipError_t lrhLaunchBounce(const LrhBounceParams &P, hipStream_t s);
 
 #endif

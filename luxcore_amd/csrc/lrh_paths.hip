@@ -137,6 +137,128 @@ __device__ uint32_t bounceOne(const LrhBounceParams &P, uint32_t i) {
 	return live;
 }
 
+// ---- one shading step (see lrh_shade in lrhip.h) ----
+__device__ __forceinline__ void cosineDir(const float n[3], float u1, float u2, float d[3]) {
+	// cosine-weighted direction around n (Malley), branchless basis (Duff et al. 2017)
+	const float r = sqrtf(u1), phi = 6.2831853f * u2;
+	const float lx = r * cosf(phi), ly = r * sinf(phi), lz = sqrtf(fmaxf(0.f, 1.f - u1));
+	const float sg = n[2] >= 0.f ? 1.f : -1.f;
+	const float a = -1.f / (sg + n[2]);
+	const float b = n[0] * n[1] * a;
+	const float t0[3] = {1.f + sg * n[0] * n[0] * a, sg * b, -sg * n[0]};
+	const float t1[3] = {b, sg + n[1] * n[1] * a, -n[1]};
+	float v[3];
+	for (int k = 0; k < 3; ++k) v[k] = lx * t0[k] + ly * t1[k] + lz * n[k];
+	const float dl = 1.f / sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+	for (int k = 0; k < 3; ++k) d[k] = v[k] * dl;
+}
+
+__device__ uint32_t shadeOne(const LrhShadeParams &P, uint32_t i) {
+	LrhPath path = P.paths[i];
+	const lrh_ray_rec ray = P.rays[i];
+	// the previous step's shadow ray: its light counts if nothing blocked it
+	const bool pending = path.dl[0] != 0.f || path.dl[1] != 0.f || path.dl[2] != 0.f;
+	if (P.shadowHits && pending && P.shadowHits[i].meshIndex == 0xffffffffu)
+		for (int k = 0; k < 3; ++k) path.rad[k] = path.rad[k] + path.dl[k];
+	for (int k = 0; k < 3; ++k) path.dl[k] = 0.f;
+	lrh_ray_rec *next = P.next + i;
+	lrh_ray_rec *shadow = P.shadow + i;
+	const lrh_hit_rec hit = P.hits[i];
+	if (!path.alive || (ray.flags & 1u) || hit.meshIndex >= P.nMeshes ||
+			hit.triangleIndex >= P.meshTriBase[hit.meshIndex + 1] - P.meshTriBase[hit.meshIndex]) {
+		path.alive = 0u;
+		writeMasked(next);
+		writeMasked(shadow);
+		P.paths[i] = path;
+		return 0;
+	}
+	LrhMaterial m;
+	if (hit.meshIndex < P.nMaterials)
+		m = P.materials[hit.meshIndex];
+	else
+		for (int k = 0; k < 3; ++k) m.kd[k] = m.le[k] = 0.f;
+	const uint32_t rec = P.invPerm[P.meshTriBase[hit.meshIndex] + hit.triangleIndex];
+	const lrh_tri tr = P.tris[rec];
+	float n[3] = {tr.e1[1] * tr.e2[2] - tr.e1[2] * tr.e2[1], tr.e1[2] * tr.e2[0] - tr.e1[0] * tr.e2[2],
+			tr.e1[0] * tr.e2[1] - tr.e1[1] * tr.e2[0]};
+	const float nl = sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+	const float s = (n[0] * ray.dx + n[1] * ray.dy + n[2] * ray.dz > 0.f ? -1.f : 1.f) / fmaxf(nl, 1e-30f);
+	for (int k = 0; k < 3; ++k) n[k] *= s;
+	const float p[3] = {ray.ox + ray.dx * hit.t, ray.oy + ray.dy * hit.t, ray.oz + ray.dz * hit.t};
+	const float eps = fmaxf(epsE(p[0]), fmaxf(epsE(p[1]), epsE(p[2])));
+	const uint32_t pix = P.pixelBase + i;
+	const uint32_t depth = path.depth;
+	if (depth == 0)  // a light seen directly
+		for (int k = 0; k < 3; ++k) path.rad[k] = path.rad[k] + path.thr[k] * m.le[k];
+	const bool diffuse = m.kd[0] > 0.f || m.kd[1] > 0.f || m.kd[2] > 0.f;
+	uint32_t live = 0;
+	bool shadowWritten = false;
+	if (P.nLights && diffuse) {
+		const float u0 = rnd(P.seed, pix, depth, 3), u1 = rnd(P.seed, pix, depth, 4), u2 = rnd(P.seed, pix, depth, 5);
+		uint32_t j = (uint32_t)(u0 * (float)P.nLights);
+		if (j > P.nLights - 1) j = P.nLights - 1;
+		const LrhLightTri L = P.lights[j];
+		const float su = sqrtf(u1), a = su * (1.f - u2), b = su * u2;
+		float w[3];
+		for (int k = 0; k < 3; ++k) w[k] = (L.p0[k] + a * L.e1[k] + b * L.e2[k]) - p[k];
+		const float d2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+		const float dist = sqrtf(d2);
+		float wn[3];
+		for (int k = 0; k < 3; ++k) wn[k] = w[k] / dist;
+		const float ln[3] = {L.e1[1] * L.e2[2] - L.e1[2] * L.e2[1], L.e1[2] * L.e2[0] - L.e1[0] * L.e2[2],
+				L.e1[0] * L.e2[1] - L.e1[1] * L.e2[0]};
+		const float lnl = sqrtf(ln[0] * ln[0] + ln[1] * ln[1] + ln[2] * ln[2]);
+		const float cosS = n[0] * wn[0] + n[1] * wn[1] + n[2] * wn[2];
+		const float cosL = fabsf(ln[0] * wn[0] + ln[1] * wn[1] + ln[2] * wn[2]) / lnl;
+		if (cosS > 0.f && cosL > 0.f && dist > 4.f * eps) {
+			// uniform light pick (1/n) and point (1/area): 1/pdf = n * area
+			const float g = ((cosS * cosL) * (0.5f * lnl * (float)P.nLights)) / d2 * 0.31830988f;
+			for (int k = 0; k < 3; ++k) path.dl[k] = path.thr[k] * m.kd[k] * L.le[k] * g;
+			lrh_ray_rec o;
+			o.ox = p[0]; o.oy = p[1]; o.oz = p[2];
+			o.dx = wn[0]; o.dy = wn[1]; o.dz = wn[2];
+			o.mint = eps;
+			o.maxt = dist - 2.f * eps;
+			o.time = ray.time;
+			o.flags = 0u;
+			o.pad[0] = o.pad[1] = 0.f;
+			*shadow = o;
+			shadowWritten = true;
+			++live;
+		}
+	}
+	if (!shadowWritten) writeMasked(shadow);
+	const uint32_t depth1 = depth + 1;
+	bool alive = depth1 < P.maxDepth && diffuse;
+	if (alive && depth1 >= P.rrDepth) {
+		if (rnd(P.seed, pix, depth, 2) < P.rrSurvive)
+			for (int k = 0; k < 3; ++k) path.thr[k] = path.thr[k] / P.rrSurvive;
+		else
+			alive = false;
+	}
+	if (alive) {
+		float d[3];
+		cosineDir(n, rnd(P.seed, pix, depth, 0), rnd(P.seed, pix, depth, 1), d);
+		// Lambertian: f cos / pdf = kd for cosine-weighted directions
+		for (int k = 0; k < 3; ++k) path.thr[k] = path.thr[k] * m.kd[k];
+		lrh_ray_rec o;
+		o.ox = p[0]; o.oy = p[1]; o.oz = p[2];
+		o.dx = d[0]; o.dy = d[1]; o.dz = d[2];
+		o.mint = eps;
+		o.maxt = INFINITY;
+		o.time = ray.time;
+		o.flags = 0u;
+		o.pad[0] = o.pad[1] = 0.f;
+		*next = o;
+		++live;
+	} else
+		writeMasked(next);
+	path.depth = depth1;
+	path.alive = alive ? 1u : 0u;
+	P.paths[i] = path;
+	return live;
+}
+
 __device__ __forceinline__ void countLive(uint32_t *counter, uint32_t live) {
 	// one atomic per wave
 	unsigned long long v = live;
@@ -151,11 +273,43 @@ __global__ void lrh_bounce_kernel(LrhBounceParams P) {
 	countLive(P.liveCount, live);
 }
 
+__global__ void lrh_shade_kernel(LrhShadeParams P) {
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	uint32_t live = 0;
+	if (i < P.count) live = shadeOne(P, i);
+	countLive(P.liveCount, live);
+}
+
+__global__ void lrh_paths_init_kernel(LrhPath *paths, uint32_t count) {
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= count) return;
+	LrhPath p;
+	for (int k = 0; k < 3; ++k) {
+		p.thr[k] = 1.f;
+		p.rad[k] = 0.f;
+		p.dl[k] = 0.f;
+	}
+	p.depth = 0u;
+	p.alive = 1u;
+	p.pad = 0u;
+	paths[i] = p;
+}
+
 }  // namespace
 
 hipError_t lrhLaunchCamera(const LrhCameraParams &C, lrh_ray_rec *rays, hipStream_t s) {
 	const uint32_t n = C.width * C.height;
 	hipLaunchKernelGGL(lrh_camera_kernel, dim3((n + 255) / 256), dim3(256), 0, s, C, rays);
+	return hipGetLastError();
+}
+
+hipError_t lrhLaunchPathsInit(LrhPath *paths, uint32_t count, hipStream_t s) {
+	hipLaunchKernelGGL(lrh_paths_init_kernel, dim3((count + 255) / 256), dim3(256), 0, s, paths, count);
+	return hipGetLastError();
+}
+
+hipError_t lrhLaunchShade(const LrhShadeParams &P, hipStream_t s) {
+	hipLaunchKernelGGL(lrh_shade_kernel, dim3((P.count + 255) / 256), dim3(256), 0, s, P);
 	return hipGetLastError();
 }
 

@@ -69,6 +69,20 @@ class LrhCamera(ctypes.Structure):
                 ("height", ctypes.c_uint)]
 
 
+class LrhShadeParams(ctypes.Structure):
+    _fields_ = [("count", ctypes.c_uint), ("maxDepth", ctypes.c_uint), ("rrDepth", ctypes.c_uint),
+                ("rrSurvive", ctypes.c_float), ("seed", ctypes.c_ulonglong), ("pixelBase", ctypes.c_uint),
+                ("pad", ctypes.c_uint)]
+
+
+# lrh_material, lrh_light_tri, lrh_path records
+MATERIAL_DTYPE = np.dtype([("kd", "<f4", 3), ("le", "<f4", 3)])
+LIGHT_DTYPE = np.dtype([("p0", "<f4", 3), ("e1", "<f4", 3), ("e2", "<f4", 3), ("le", "<f4", 3)])
+PATH_DTYPE = np.dtype([("thr", "<f4", 3), ("rad", "<f4", 3), ("dl", "<f4", 3), ("depth", "<u4"),
+                       ("alive", "<u4"), ("pad", "<u4")])
+assert MATERIAL_DTYPE.itemsize == 24 and LIGHT_DTYPE.itemsize == 48 and PATH_DTYPE.itemsize == 48
+
+
 class LrhBatchItem(ctypes.Structure):
     _fields_ = [("scene", ctypes.c_void_p), ("rays", ctypes.c_void_p), ("hits", ctypes.c_void_p),
                 ("count", ctypes.c_uint), ("query", ctypes.c_uint)]
@@ -129,6 +143,9 @@ SIGNATURES = {
     "lrh_rays_bounce": ([_vp, _vp, _vp, ctypes.c_uint, ctypes.c_uint, ctypes.c_ulonglong, ctypes.c_uint,
                          ctypes.c_uint, ctypes.c_float, ctypes.POINTER(ctypes.c_float), _vp, _vp, _vp, _vp],
                         ctypes.c_int),
+    "lrh_paths_init": ([_vp, _vp, ctypes.c_uint, _vp], ctypes.c_int),
+    "lrh_shade": ([_vp, _vp, ctypes.c_uint, _vp, ctypes.c_uint, _vp, _vp, _vp, _vp, _vp, _vp,
+                   ctypes.POINTER(LrhShadeParams), _vp, _vp], ctypes.c_int),
     "lrh_event_query": ([_vp], ctypes.c_int),
     "lrh_device_stats": ([_vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_double)],
                          ctypes.c_int),
@@ -543,6 +560,19 @@ class Device:
         check(lib().lrh_rays_bounce(scene.h, rays_buf.ptr, hits_buf.ptr, count, bounce, seed, pixel_base,
                                     rr_start, rr_survive, lc, next_buf.ptr, shadow_buf.ptr if shadow_buf else None,
                                     live_counter, stream))
+
+    def paths_init(self, paths_buf, count, stream=None):
+        check(lib().lrh_paths_init(self.h, paths_buf.ptr, count, stream))
+
+    def shade(self, scene, materials_buf, n_materials, lights_buf, n_lights, rays_buf, hits_buf, shadow_hits_buf,
+              paths_buf, next_buf, shadow_buf, count, max_depth=8, rr_depth=3, rr_survive=0.7, seed=1,
+              pixel_base=0, live_counter=None, stream=None):
+        """lrh_shade: one Lambertian + area-light shading step of a path wavefront."""
+        p = LrhShadeParams(count, max_depth, rr_depth, rr_survive, seed, pixel_base, 0)
+        check(lib().lrh_shade(scene.h, materials_buf.ptr if materials_buf else None, n_materials,
+                              lights_buf.ptr if lights_buf else None, n_lights, rays_buf.ptr, hits_buf.ptr,
+                              shadow_hits_buf.ptr if shadow_hits_buf else None, paths_buf.ptr, next_buf.ptr,
+                              shadow_buf.ptr, ctypes.byref(p), live_counter, stream))
 
     def stats(self):
         r = ctypes.c_ulonglong()

@@ -497,6 +497,8 @@ struct lrh_scenefile {
 		float m[16];
 		std::vector<float> keyTimes;
 		std::vector<float> keyMats;   // 16 per key, global->local
+		float kd[3] = {0.75f, 0.75f, 0.75f};
+		float le[3] = {0.f, 0.f, 0.f};
 	};
 	std::vector<Object> objects;
 	lrh_scenefile_camera camera;
@@ -530,6 +532,43 @@ namespace {
 bool geometryPreserving(const std::string &t) {
 	return t == "pointiness" || t == "harlequin" || t == "islandaov" || t == "randomtriangleaov" ||
 	       t == "edgedetectoraov" || t == "cameraprojuv";
+}
+
+// a constant color: 1 or 3 numbers, or a constfloat1 / constfloat3 texture
+// (parsetextures.cpp); false for anything else
+bool constColor(const Properties &props, const std::string &name, float out[3]) {
+	if (!props.defined(name)) return false;
+	const std::vector<std::string> &v = props.get(name);
+	auto num = [](const std::string &t, float *f) {
+		char *e = nullptr;
+		*f = strtof(t.c_str(), &e);
+		return !t.empty() && !*e;
+	};
+	float f[3];
+	if (v.size() == 3 && num(v[0], &f[0]) && num(v[1], &f[1]) && num(v[2], &f[2])) {
+		for (int k = 0; k < 3; ++k) out[k] = f[k];
+		return true;
+	}
+	if (v.size() == 1 && num(v[0], &f[0])) {
+		for (int k = 0; k < 3; ++k) out[k] = f[0];
+		return true;
+	}
+	if (v.size() == 1) {
+		const std::string tex = "scene.textures." + v[0];
+		const std::string type = getString(props, tex + ".type", "");
+		if (type == "constfloat3" || type == "constfloat1") return constColor(props, tex + ".value", out);
+	}
+	return false;
+}
+
+void materialColors(const Properties &props, const std::string &mat, float kd[3], float le[3]) {
+	for (int k = 0; k < 3; ++k) { kd[k] = 0.75f; le[k] = 0.f; }
+	constColor(props, mat + ".kd", kd);
+	float e[3], g[3] = {1.f, 1.f, 1.f};
+	if (constColor(props, mat + ".emission", e)) {
+		constColor(props, mat + ".emission.gain", g);
+		for (int k = 0; k < 3; ++k) le[k] = e[k] * g[k];
+	}
 }
 
 void applyTransform(MeshData *mesh, const Mat m) {   // TriangleMesh::ApplyTransform (trianglemesh.cpp:85-93)
@@ -661,6 +700,7 @@ void lrh_scenefile::build(const Properties &props) {
 		Object o;
 		o.name = objName;
 		o.mesh = meshIndex.at(shapeName);
+		materialColors(props, "scene.materials." + matName, o.kd, o.le);
 		for (int i = 0; i < 16; ++i) o.m[i] = (i % 5 == 0) ? 1.f : 0.f;
 		if (props.defined(key + ".motion.0.time")) {
 			o.kind = 2;
@@ -836,6 +876,17 @@ int lrh_scenefile_object_name(const lrh_scenefile *sf, unsigned int obj, const c
 		if (obj >= sf->objects.size()) fail(LRH_EINVAL, "object index out of range");
 		if (name) *name = sf->objects[obj].name.c_str();
 		if (meshName) *meshName = sf->meshes[sf->objects[obj].mesh].name.c_str();
+	});
+}
+
+int lrh_scenefile_object_material(const lrh_scenefile *sf, unsigned int obj, float kd[3], float le[3]) {
+	return guard([&] {
+		if (!sf || !kd || !le) fail(LRH_EINVAL, "null argument");
+		if (obj >= sf->objects.size()) fail(LRH_EINVAL, "object index out of range");
+		for (int k = 0; k < 3; ++k) {
+			kd[k] = sf->objects[obj].kd[k];
+			le[k] = sf->objects[obj].le[k];
+		}
 	});
 }
 

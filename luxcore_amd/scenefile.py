@@ -54,6 +54,8 @@ _SIGNATURES = {
     "lrh_scenefile_object_name": ([_vp, ctypes.c_uint, ctypes.POINTER(ctypes.c_char_p),
                                    ctypes.POINTER(ctypes.c_char_p)], ctypes.c_int),
     "lrh_scenefile_get_camera": ([_vp, ctypes.POINTER(SceneFileCamera)], ctypes.c_int),
+    "lrh_scenefile_object_material": ([_vp, ctypes.c_uint, ctypes.POINTER(ctypes.c_float),
+                                       ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
     "lrh_scenefile_destroy": ([_vp], ctypes.c_int),
     "lrh_ingest_last_error": ([], ctypes.c_char_p),
 }
@@ -118,6 +120,9 @@ class SceneFile:
                     d["times"] = list(np.ctypeslib.as_array(o.keyTimes, shape=(o.nKeys,)).copy())
                     km = np.ctypeslib.as_array(o.keyMats, shape=(o.nKeys * 16,)).reshape(-1, 4, 4).copy()
                     d["keys"] = list(km)
+                kd, le = (ctypes.c_float * 3)(), (ctypes.c_float * 3)()
+                _check(lib().lrh_scenefile_object_material(handle, i, kd, le))
+                d["kd"], d["le"] = np.array(kd, np.float32), np.array(le, np.float32)
                 self.objects.append(d)
             cam = SceneFileCamera()
             _check(lib().lrh_scenefile_get_camera(handle, ctypes.byref(cam)))
@@ -167,6 +172,32 @@ class SceneFile:
         if not np.all(np.isfinite(lo)):
             return np.zeros(3), np.ones(3)
         return lo, hi
+
+    def materials(self):
+        """lrh_material records per DataSet mesh (= object), for lrh_shade."""
+        from .device import MATERIAL_DTYPE
+        out = np.zeros(len(self.objects), MATERIAL_DTYPE)
+        for i, o in enumerate(self.objects):
+            out[i]["kd"], out[i]["le"] = o["kd"], o["le"]
+        return out
+
+    def light_triangles(self):
+        """lrh_light_tri records of every emitting plain object's triangles
+        (world space = mesh space for plain objects)."""
+        from .device import LIGHT_DTYPE
+        parts = []
+        for o in self.objects:
+            if o["kind"] != 0 or not np.any(o["le"] > 0):
+                continue
+            m = self.meshes[o["mesh"]]
+            v = m.verts[m.tris]                      # (T,3,3)
+            rec = np.zeros(m.ntris, LIGHT_DTYPE)
+            rec["p0"] = v[:, 0]
+            rec["e1"] = v[:, 1] - v[:, 0]
+            rec["e2"] = v[:, 2] - v[:, 0]
+            rec["le"] = o["le"]
+            parts.append(rec)
+        return np.concatenate(parts) if parts else np.zeros(0, LIGHT_DTYPE)
 
     def dataset_meshes(self):
         """One mesh per object, in object order: the single-level DataSet."""

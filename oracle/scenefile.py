@@ -328,6 +328,38 @@ def _resolve(name, dirs):
     return name
 
 
+def _const_color(props, name):
+    """1 or 3 numbers, or a constfloat1 / constfloat3 texture; None otherwise."""
+    if name not in props:
+        return None
+    v = props[name]
+    try:
+        if len(v) == 3:
+            return np.array([np.float32(float(x)) for x in v], np.float32)
+        if len(v) == 1:
+            return np.full(3, np.float32(float(v[0])), np.float32)
+    except ValueError:
+        pass
+    if len(v) == 1:
+        tex = "scene.textures." + v[0]
+        if props.get(tex + ".type", [""])[0] in ("constfloat3", "constfloat1"):
+            return _const_color(props, tex + ".value")
+    return None
+
+
+def _material_colors(props, mat):
+    """(kd, le) of the Lambertian reduction of a material (parsematerials.cpp:143-186)."""
+    kd = _const_color(props, mat + ".kd")
+    if kd is None:
+        kd = np.full(3, np.float32(0.75), np.float32)
+    le = np.zeros(3, np.float32)
+    e = _const_color(props, mat + ".emission")
+    if e is not None:
+        g = _const_color(props, mat + ".emission.gain")
+        le = (e * (g if g is not None else np.ones(3, np.float32))).astype(np.float32)
+    return kd, le
+
+
 _KEEP_GEOMETRY = {"pointiness", "harlequin", "islandaov", "randomtriangleaov", "edgedetectoraov", "cameraprojuv"}
 
 
@@ -433,6 +465,7 @@ def load_scene(path, search_dirs=()):
             raise IngestError("Missing shape in object definition: " + oname)
         o = {"name": oname, "mesh": index[shape], "kind": 0, "m": np.eye(4, dtype=np.float32),
              "times": [], "keys": []}
+        o["kd"], o["le"] = _material_colors(props, "scene.materials." + mat[0])
         if key + ".motion.0.time" in props:
             o["kind"] = 2
             i = 0

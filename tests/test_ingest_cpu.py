@@ -239,6 +239,7 @@ def _same_scene(a, b):
         assert np.asarray(x["m"], np.float32).tobytes() == np.asarray(y["m"], np.float32).tobytes()
         assert np.asarray(x["times"], np.float32).tobytes() == np.asarray(y["times"], np.float32).tobytes()
         assert np.asarray(x["keys"], np.float32).tobytes() == np.asarray(y["keys"], np.float32).tobytes()
+        assert x["kd"].tobytes() == y["kd"].tobytes() and x["le"].tobytes() == y["le"].tobytes()
     for u, v in zip(a.camera[:3], b.camera[:3]):
         assert np.asarray(u, np.float32).tobytes() == np.asarray(v, np.float32).tobytes()
     assert np.float32(a.camera[3]) == np.float32(b.camera[3])
@@ -375,7 +376,7 @@ def test_ingest_library_exports_every_declared_function():
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     with open(os.path.join(root, "include", "lrh_ingest.h")) as f:
         names = set(re.findall(r"^\s*(?:const\s+)?[a-z_]+\s*\*?\s*(lrh_[a-z_0-9]+)\(", f.read(), re.M))
-    assert len(names) == 10
+    assert len(names) == 11
     L = P.lib()
     for n in names:
         assert hasattr(L, n), n
@@ -385,3 +386,35 @@ def test_ingest_library_exports_every_declared_function():
     assert "amdhip" not in out and "hsa" not in out
     assert L.lrh_scenefile_get_info(None, None) == -1
     assert L.lrh_scenefile_destroy(None) == 0
+
+
+def test_materials_reduced_for_shading(tmp_path):
+    """kd / emission as lrh_shade uses them: numbers, constant textures,
+    emission x gain, the 0.75 default for anything else."""
+    d = unpack("bigmonkey", tmp_path)
+    sf = P.load_scene(os.path.join(d, "bigmonkey-instances.scn"))
+    by = {o["name"]: o for o in sf.objects}
+    assert np.array_equal(by["l1"]["le"], [8, 8, 8]) and np.array_equal(by["l1"]["kd"], [0, 0, 0])
+    assert np.array_equal(by["m1"]["kd"], np.float32([0.75, 0, 0])) and not by["m1"]["le"].any()
+    text = "\n".join([
+        "scene.textures.t3.type = constfloat3", "scene.textures.t3.value = 0.1 0.2 0.3",
+        "scene.textures.t1.type = constfloat1", "scene.textures.t1.value = 0.4",
+        "scene.textures.img.type = imagemap",
+        "scene.materials.a.type = matte", "scene.materials.a.kd = t3",
+        "scene.materials.b.type = glossy2", "scene.materials.b.kd = t1",
+        "scene.materials.c.type = matte", "scene.materials.c.kd = img",
+        "scene.materials.c.emission = 2 3 4", "scene.materials.c.emission.gain = 0.5",
+        "scene.objects.a.material = a", "scene.objects.a.vertices = 0 0 0 1 0 0 0 1 0", "scene.objects.a.faces = 0 1 2",
+        "scene.objects.b.material = b", "scene.objects.b.vertices = 0 0 0 1 0 0 0 1 0", "scene.objects.b.faces = 0 1 2",
+        "scene.objects.c.material = c", "scene.objects.c.vertices = 0 0 0 1 0 0 0 1 0", "scene.objects.c.faces = 0 1 2",
+        ""])
+    a = P.parse_scene(text, str(tmp_path))
+    kd = [o["kd"].tolist() for o in a.objects]
+    assert kd == [np.float32([0.1, 0.2, 0.3]).tolist(), [np.float32(0.4)] * 3, [0.75] * 3]
+    assert a.objects[2]["le"].tolist() == [1.0, 1.5, 2.0]
+    p = tmp_path / "m.scn"
+    p.write_text(text)
+    _same_scene(a, O.load_scene(str(p)))
+    m = a.materials()
+    assert m["kd"].shape == (3, 3) and m["le"][2].tolist() == [1.0, 1.5, 2.0]
+    assert len(a.light_triangles()) == 1
