@@ -393,16 +393,90 @@ def run_configs(args, dev):
             "config": {"workload": "plain 101,932 tris / heightfield 26.2M tris / 1,000 instances"}, "configs": out}
 
 
+def run_render(args, dev):
+    """Path tracing on the GPU end to end: camera rays (lrh_rays_camera), then
+    per vertex a closest-hit trace of the path rays, an any-hit trace of the
+    previous vertex's shadow rays and one shading step (lrh_shade: Lambertian,
+    triangle-light next-event estimation, Russian roulette), 8 vertices, one
+    sample per pixel per pass, 1280x720, on the bundled bigmonkey scene with
+    its light made constant (8.0) and its monkey red.  Reports samples
+    (paths) per second and rays per second, HIP-event timed over whole passes."""
+    import tempfile
+    from luxcore_amd import scenefile as SF
+    from tests.scene_fixtures import unpack
+    d = unpack("bigmonkey", tempfile.mkdtemp(prefix="lrh_render_"))
+    path = os.path.join(d, "bigmonkey-lit.scn")
+    with open(os.path.join(d, "bigmonkey.scn")) as f, open(path, "w") as g:
+        g.write(f.read() + "\nscene.materials.redmat.kd = 0.75 0.0 0.0\n"
+                "scene.materials.whitelight.emission = 8.0 8.0 8.0\n"
+                "scene.materials.whitelight.emission.gain = 1.0 1.0 1.0\n")
+    sf = SF.load_scene(path)
+    scene = SF.scene_on_device(dev, sf)
+    mats, lights = sf.materials(), sf.light_triangles()
+    W, H, depth = 1280, 720, 8
+    n = W * H
+    orig, target, up, fov = sf.camera
+    bufs = {k: dev.alloc(n * 48) for k in ("ray", "next", "shadow")}
+    hb, shb = dev.alloc(n * 20), dev.alloc(n * 20)
+    pb = dev.alloc(n * D.PATH_DTYPE.itemsize)
+    mb, lb = dev.upload(mats), dev.upload(lights)
+    counter = dev.alloc(4)
+    zero = np.zeros(1, np.uint32)
+
+    def one_pass(seed):
+        rb, nb, sb = bufs["ray"], bufs["next"], bufs["shadow"]
+        dev.camera_rays(rb, orig, target, up, fov, W, H)
+        dev.paths_init(pb, n)
+        traced = n
+        for k in range(depth):
+            dev.trace(scene, rb, hb, n, 0)
+            if k:
+                dev.trace(scene, sb, shb, n, 1)
+            dev.shade(scene, mb, len(mats), lb, len(lights), rb, hb, shb if k else None, pb, nb, sb, n,
+                      max_depth=depth, rr_depth=3, rr_survive=0.7, seed=seed, live_counter=counter.ptr)
+            rb, nb = nb, rb
+        return traced
+
+    dev.upload(zero, counter)
+    for w in range(max(1, args.warmup)):
+        one_pass(1000 + w)
+    dev.sync()
+    dev.upload(zero, counter)
+    dev.sync()
+    passes = args.steps
+    wall, lat = _event_timed(dev, lambda: one_pass(7), passes, 0)
+    live = int(dev.download(counter, np.uint32, 1)[0])
+    rad = dev.download(pb, D.PATH_DTYPE, n)["rad"]
+    # camera rays plus the live next-vertex and shadow rays the shading steps wrote (the last
+    # step's are written but not traced)
+    rays_per_pass = n + live / passes
+    out = {"metric": "path tracing on the GPU: samples/s (1280x720, 8 vertices, Lambertian + NEE)",
+           "value": round(n * passes / wall / 1e6, 2), "unit": "Msamples/s", "n_gpus": 1, "steps": passes,
+           "warmup": args.warmup, "ms_per_step": round(float(lat.mean()), 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "bundled bigmonkey scene",
+           "config": {"workload": "bigmonkey.scn (3 objects, 6,320 tris, light emission 8.0), 1280x720, 1 spp "
+                                  "per pass, 8 path vertices, RR 0.7 from vertex 3", "paths_per_pass": n},
+           "rays_written_per_pass": round(rays_per_pass),
+           "mrays_live": round(rays_per_pass * passes / wall / 1e6, 1),
+           "mean_radiance": [round(float(x), 4) for x in rad.mean(axis=0)]}
+    log(f"render: {out['value']} Msamples/s, {out['mrays_live']} Mrays/s live, {out['ms_per_step']} ms per pass")
+    for b in list(bufs.values()) + [hb, shb, pb, mb, lb, counter]:
+        b.free()
+    scene.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--mode", choices=("headline", "batched", "served", "scene", "refit", "configs"),
+    ap.add_argument("--mode", choices=("headline", "batched", "served", "scene", "refit", "configs", "render"),
                     default="headline",
                     help="headline: incoherent closest-hit on the 26.2M-tri scene (the driver's line); "
                          "batched: 64 buffers over 8 resident scenes in one launch; served: continuous batching; "
                          "scene: a LuxCore scene file (--scene, default the bundled lightinstances scene); "
                          "refit: two-level edit latency, host rebuild (lrh_scene_update) vs GPU refit; "
-                         "configs: closest/any-hit rates of the plain, heightfield and instanced workloads")
+                         "configs: closest/any-hit rates of the plain, heightfield and instanced workloads; "
+                         "render: path tracing of a scene file with lrh_shade (samples per second)")
     ap.add_argument("--scene", default=None, help="scene file for --mode scene (.scn/.cfg)")
     ap.add_argument("--scene-rays", type=int, default=1 << 22, help="incoherent rays for --mode scene")
     ap.add_argument("--steps", type=int, default=None,
@@ -449,6 +523,9 @@ def main():
         return
     if args.mode == "configs":
         print(json.dumps(run_configs(args, dev)), flush=True)
+        return
+    if args.mode == "render":
+        print(json.dumps(run_render(args, dev)), flush=True)
         return
     meshes, scene, info, rays, ntris = build_workload(args, dev)
     n = rays.shape[0]
