@@ -1,16 +1,21 @@
 """Served mode: continuous batching of path-tracing sessions.
 
 A session is a small path tracer bound to one resident scene: 512x512 camera
-rays, then up to ``max_bounces`` bounce wavefronts (cosine directions,
-Russian roulette from bounce 2 with survival 0.7) and one shadow ray per hit,
-traced as any-hit.  Each session has one wavefront in flight at a time.
+rays, then up to ``max_bounces`` bounce wavefronts and one shadow ray per
+hit, traced as any-hit.  With ``shading`` (the default) each wavefront is
+shaded on the GPU by lrh_shade: grey Lambertian surfaces (kd 0.6) lit by a
+square area light above the scene through next-event estimation, cosine
+next vertices, Russian roulette from the third vertex with survival 0.7,
+radiance gathered per pixel; without it lrh_rays_bounce draws the same kind
+of directions without materials.  Each session has one wavefront in flight
+at a time.
 
 The server keeps the scenes and every admitted session's ray/hit buffers in
 HBM.  Each tick it admits queued sessions while the HBM budget and the
 per-tick ray budget allow, gathers the ready wavefront of every live session
 (closest-hit rays + the shadow rays of the previous hits) into ONE batched
 persistent launch (lrh_trace_batch), then generates every session's next
-wavefront on the GPU (lrh_rays_bounce) and retires finished sessions, so
+wavefront on the GPU (lrh_shade or lrh_rays_bounce) and retires finished sessions, so
 sessions that arrive while others are on the GPU join at the next tick.
 
 Ticks are pipelined: the host needs no result of a tick to build the next
@@ -56,7 +61,8 @@ class Session:
         self.n = n
         self.bufs = server._take_buffers()
         self.rays = [self.bufs[0], self.bufs[1]]
-        self.shadow, self.hits, self.shadow_hits = self.bufs[2], self.bufs[3], self.bufs[4]
+        self.shadow, self.hits, self.shadow_hits, self.paths = self.bufs[2], self.bufs[3], self.bufs[4], self.bufs[5]
+        self.radiance = None  # per-pixel radiance, kept for recorded sessions
         self.cur = 0
         self.step = 0
         self.ready = None
@@ -69,11 +75,13 @@ class Session:
         look = np.array([ext / 2, ext / 2, ext * 0.3]) + rng.uniform(-0.1 * ext, 0.1 * ext, 3)
         dev.camera_rays(self.rays[0], eye, look, (0.0, 0.0, 1.0), 50.0, server.width, server.height,
                         stream=self.stream)
+        if server.shading:
+            dev.paths_init(self.paths, n, stream=self.stream)
 
     @staticmethod
     def bytes(width, height):
         n = width * height
-        return 3 * n * RAY_B + 2 * n * HIT_B
+        return 3 * n * RAY_B + 2 * n * HIT_B + n * D.PATH_DTYPE.itemsize
 
     def items(self):
         """Batch items of the current wavefront."""
@@ -91,9 +99,21 @@ class Session:
         False when the session is finished."""
         srv = self.server
         last = srv.max_bounces
-        if self.step <= last:
+        sc = srv.scenes[self.spec.scene]
+        if srv.shading:
+            # vertex `step` (its shadow rays' hits come from the previous
+            # vertex); one more step after the last vertex only gathers the
+            # light its shadow rays found
             nxt = 1 - self.cur
-            srv.dev.bounce_rays(srv.scenes[self.spec.scene], self.rays[self.cur], self.hits, self.n, self.step,
+            mats, nm = srv.materials[self.spec.scene]
+            srv.dev.shade(sc, mats, nm, srv.lights_buf, srv.n_lights, self.rays[self.cur], self.hits,
+                          self.shadow_hits if self.step >= 1 else None, self.paths, self.rays[nxt], self.shadow,
+                          self.n, max_depth=last + 1, rr_depth=srv.rr_start + 1, rr_survive=srv.rr_survive,
+                          seed=self.spec.seed, live_counter=srv.live_ptr, stream=self.stream)
+            self.cur = nxt
+        elif self.step <= last:
+            nxt = 1 - self.cur
+            srv.dev.bounce_rays(sc, self.rays[self.cur], self.hits, self.n, self.step,
                                 self.spec.seed, self.rays[nxt], self.shadow, 0, srv.rr_start, srv.rr_survive,
                                 srv.light, srv.live_ptr, stream=self.stream)
             self.cur = nxt
@@ -116,8 +136,9 @@ class Server:
 
     def __init__(self, dev, scenes, width=512, height=512, max_bounces=8, rr_start=2, rr_survive=0.7,
                  hbm_budget=4 << 30, max_rays_per_tick=3 << 20, opts=None, extent=100.0,
-                 light=(50.0, 50.0, 150.0), record=(), cohorts=2):
+                 light=(50.0, 50.0, 150.0), record=(), cohorts=2, shading=True):
         self.dev = dev
+        self.shading = shading
         self.scenes = scenes
         self.width, self.height = width, height
         self.max_bounces = max_bounces
@@ -145,6 +166,22 @@ class Server:
         self.live_ptr = self.counter.ptr
         dev.upload(np.zeros(2, np.uint32), self.counter)
         self.session_bytes = Session.bytes(width, height)
+        if shading:
+            # grey Lambertian surfaces; a square light (two triangles, 0.3 x extent wide,
+            # radiance 30) facing down from above the scene at light[2]
+            # (one table for every scene: entry i is mesh i of whichever scene)
+            m = np.zeros(1024, D.MATERIAL_DTYPE)
+            m["kd"] = 0.6
+            table = (dev.upload(m), len(m))
+            self.materials = [table] * len(scenes)
+            c, h, w = np.array(light[:2], np.float32), np.float32(light[2]), np.float32(0.15 * extent)
+            q = np.array([[c[0] - w, c[1] - w, h], [c[0] + w, c[1] - w, h], [c[0] + w, c[1] + w, h],
+                          [c[0] - w, c[1] + w, h]], np.float32)
+            lt = np.zeros(2, D.LIGHT_DTYPE)
+            for k, (a, b, d) in enumerate(((0, 2, 1), (0, 3, 2))):
+                lt[k]["p0"], lt[k]["e1"], lt[k]["e2"] = q[a], q[b] - q[a], q[d] - q[a]
+                lt[k]["le"] = 30.0
+            self.lights_buf, self.n_lights = dev.upload(lt), 2
         self.max_active_seen = 0
         dev.sync()
         self.ref = dev.event()
@@ -160,7 +197,8 @@ class Server:
         n = self.width * self.height
         d = self.dev
         self.allocated += 1
-        return (d.alloc(n * RAY_B), d.alloc(n * RAY_B), d.alloc(n * RAY_B), d.alloc(n * HIT_B), d.alloc(n * HIT_B))
+        return (d.alloc(n * RAY_B), d.alloc(n * RAY_B), d.alloc(n * RAY_B), d.alloc(n * HIT_B), d.alloc(n * HIT_B),
+                d.alloc(n * D.PATH_DTYPE.itemsize))
 
     def close(self):
         self.drain()
@@ -213,6 +251,8 @@ class Server:
             self.latencies.append(t - s.ready)
             s.ready = t
         for s in finished:
+            if s.spec.sid in self.record and self.shading:
+                s.radiance = self.dev.download(s.paths, D.PATH_DTYPE, s.n)["rad"]
             s.release()
             self.done.append(s)
 
@@ -243,6 +283,16 @@ class Server:
         items = []
         for s in coh:
             items += s.items()
+        if self.record:
+            # masked rays leave their hit records untouched: clear the recorded
+            # sessions' hit buffers so what is compared does not depend on the
+            # buffers' previous contents
+            for s in coh:
+                if s.spec.sid in self.record:
+                    z = np.zeros(s.n, HIT_DTYPE)
+                    self.dev.upload(z, s.hits)
+                    self.dev.upload(z, s.shadow_hits)
+            self.dev.sync()
         e0, e1 = self.dev.event(), self.dev.event()
         e0.record(st)
         self.dev.trace_batch(items, self.opts, stream=st)

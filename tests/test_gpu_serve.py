@@ -81,9 +81,12 @@ def test_session_same_alone_and_in_mix(gpu_device, serve_scenes):
     mix = serve.Server(dev, scs, record={0}, max_rays_per_tick=6 * 2 * 96 * 96, **kw)
     others = [serve.SessionSpec(i, i % 4, 1000 + i, 0.0005 * i) for i in range(1, 8)]
     mix.run([target] + others)
-    a = [s for s in alone.done if s.spec.sid == 0][0].recorded
-    b = [s for s in mix.done if s.spec.sid == 0][0].recorded
+    sa_, sb_ = [s for s in alone.done if s.spec.sid == 0][0], [s for s in mix.done if s.spec.sid == 0][0]
+    a, b = sa_.recorded, sb_.recorded
     assert len(a) == len(b) == 6
+    # the shaded image is the same alone and in the mix, and something was lit
+    assert sa_.radiance is not None and np.array_equal(sa_.radiance, sb_.radiance)
+    assert (sa_.radiance.sum(axis=1) > 0).mean() > 0.05
     for k, ((ca, sa), (cb, sb)) in enumerate(zip(a, b)):
         for x, y in ((ca, cb), (sa, sb)):
             assert (x is None) == (y is None), k
