@@ -178,7 +178,9 @@ int lrh_scene_update(lrh_scene *scene, const lrh_object *objects, unsigned int n
  * (1,025 samples per segment, motionsystem.cpp:75-87) and every root node,
  * bottom-up.  The object boxes are the ones lrh_scene_update would compute;
  * only the tree's grouping is the old one.  Synchronous on the device's
- * stream; LRH_EINVAL if a motion sample is singular. */
+ * stream; no trace of this scene may run on another stream meanwhile (the
+ * root nodes are rewritten in place).  LRH_EINVAL if a motion sample is
+ * singular; the scene is then left as it was. */
 int lrh_scene_refit(lrh_scene *scene, const lrh_object *objects, unsigned int nObjects);
 /* The culling box of every object of a two-level scene (6 floats each: lo
  * xyz, hi xyz; MachineEpsilon-expanded world boxes, mbvhaccel.cpp:187-202),
@@ -261,7 +263,11 @@ int lrh_rays_bounce(lrh_scene *scene, const void *rays, const void *hits, unsign
  *   - the next ray is cosine-weighted (throughput *= kd), with Russian
  *     roulette from rrDepth on; paths end at maxDepth vertices.
  * Random numbers hash (seed, pixelBase + i, depth, dimension).  Single-level
- * scenes; materials are per DataSet mesh; all arrays are device memory. */
+ * scenes; materials are per DataSet mesh; all arrays are device memory.
+ * Pass the hits of the previous step's shadow rays whenever that step wrote
+ * shadow rays: light still pending without them is dropped.  One step after
+ * the last vertex (all paths ended, next rays masked) gathers the light of
+ * the last shadow rays. */
 typedef struct {
 	float kd[3];      /* diffuse reflectance */
 	float le[3];      /* emitted radiance (0: not a light) */
