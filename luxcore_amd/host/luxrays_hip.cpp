@@ -8,6 +8,7 @@
 #include "luxrays_hip.h"
 
 #include <cstring>
+#include <sstream>
 
 namespace luxrays_hip {
 
@@ -325,6 +326,165 @@ bool HIPIntersectionDevice::TraceRay(const Ray *ray, RayHit *rayHit) {
 	if (!running) throw std::runtime_error("TraceRay on a stopped device");
 	statsSerial += 1;
 	return accel->Intersect(ray, rayHit);
+}
+
+//------------------------------------------------------------------------------
+// DeviceDescription (src/luxrays/core/device.cpp:28-89)
+//------------------------------------------------------------------------------
+
+void DeviceDescription::FilterOne(std::vector<DeviceDescription *> &deviceDescriptions) {
+	for (DeviceDescription *d : deviceDescriptions) {
+		if (d->GetType() & (DEVICE_TYPE_OPENCL_GPU | DEVICE_TYPE_HIP_GPU)) {
+			deviceDescriptions.assign(1, d);
+			return;
+		}
+	}
+	deviceDescriptions.clear();
+}
+
+void DeviceDescription::Filter(const DeviceType type, std::vector<DeviceDescription *> &deviceDescriptions) {
+	if (type == DEVICE_TYPE_ALL) return;
+	std::vector<DeviceDescription *> kept;
+	for (DeviceDescription *d : deviceDescriptions)
+		if (d->GetType() & type) kept.push_back(d);
+	deviceDescriptions.swap(kept);
+}
+
+std::string DeviceDescription::GetDeviceType(const DeviceType type) {
+	switch (type) {
+		case DEVICE_TYPE_ALL: return "ALL";
+		case DEVICE_TYPE_NATIVE: return "NATIVE_THREAD";
+		case DEVICE_TYPE_OPENCL_ALL: return "OPENCL_ALL";
+		case DEVICE_TYPE_OPENCL_DEFAULT: return "OPENCL_DEFAULT";
+		case DEVICE_TYPE_OPENCL_CPU: return "OPENCL_CPU";
+		case DEVICE_TYPE_OPENCL_GPU: return "OPENCL_GPU";
+		case DEVICE_TYPE_OPENCL_UNKNOWN: return "OPENCL_UNKNOWN";
+		case DEVICE_TYPE_CUDA_GPU: return "CUDA_GPU";
+		case DEVICE_TYPE_HIP_GPU: return "HIP_GPU";
+		default:
+			throw std::runtime_error("Unknown device type in DeviceDescription::GetDeviceType(): " +
+					std::to_string((int)type));
+	}
+}
+
+void HIPDeviceDescription::AddDeviceDescs(std::vector<DeviceDescription *> &descriptions) {
+	const int n = HIPIntersectionDevice::GetDeviceCount();
+	for (int i = 0; i < n; ++i) {
+		lrh_device *d = nullptr;
+		CheckLrh(lrh_device_open(i, &d));
+		char buf[256];
+		int cus = 0;
+		size_t mem = 0;
+		const int rc = lrh_device_info(d, buf, sizeof(buf), &cus, &mem);
+		lrh_device_close(d);
+		CheckLrh(rc);
+		descriptions.push_back(new HIPDeviceDescription(i, std::string("HIP-") + buf, cus, mem));
+	}
+}
+
+//------------------------------------------------------------------------------
+// Context (src/luxrays/core/context.cpp:44-330)
+//------------------------------------------------------------------------------
+
+namespace {
+template <class F>
+void ctxLog(const Context *c, F &&f) {
+	if (!c->HasDebugHandler() || !c->IsVerbose()) return;
+	std::ostringstream ss;
+	f(ss);
+	c->PrintDebugMsg(ss.str());
+}
+}  // namespace
+
+Context::Context(LuxRaysDebugHandler handler, bool verb) : debugHandler(handler), verbose(verb) {
+	ctxLog(this, [](std::ostream &o) { o << "HIP support: enabled"; });
+	HIPDeviceDescription::AddDeviceDescs(deviceDescriptions);
+	ctxLog(this, [&](std::ostream &o) { o << "HIP device count: " << deviceDescriptions.size(); });
+	for (size_t i = 0; i < deviceDescriptions.size(); ++i) {
+		const DeviceDescription *d = deviceDescriptions[i];
+		ctxLog(this, [&](std::ostream &o) { o << "Device " << i << " name: " << d->GetName(); });
+		ctxLog(this, [&](std::ostream &o) {
+			o << "Device " << i << " type: " << DeviceDescription::GetDeviceType(d->GetType());
+		});
+		ctxLog(this, [&](std::ostream &o) { o << "Device " << i << " compute units: " << d->GetComputeUnits(); });
+		ctxLog(this, [&](std::ostream &o) {
+			o << "Device " << i << " preferred float vector width: " << d->GetNativeVectorWidthFloat();
+		});
+		ctxLog(this, [&](std::ostream &o) {
+			o << "Device " << i << " max allocable memory: " << d->GetMaxMemory() / (1024 * 1024) << "MBytes";
+		});
+		ctxLog(this, [&](std::ostream &o) {
+			o << "Device " << i << " max allocable memory block size: " << d->GetMaxMemoryAllocSize() / (1024 * 1024)
+			  << "MBytes";
+		});
+		ctxLog(this, [&](std::ostream &o) {
+			o << "Device " << i << " has out of core memory support: " << d->HasOutOfCoreMemorySupport();
+		});
+	}
+}
+
+Context::~Context() {
+	try {
+		if (started) Stop();
+	} catch (...) {
+	}
+	for (HIPIntersectionDevice *d : idevices) delete d;
+	for (DeviceDescription *d : deviceDescriptions) delete d;
+}
+
+std::vector<HIPIntersectionDevice *> Context::AddIntersectionDevices(std::vector<DeviceDescription *> &deviceDesc) {
+	if (started) throw std::runtime_error("Context::AddIntersectionDevices on a started context");
+	ctxLog(this, [&](std::ostream &o) { o << "Creating " << deviceDesc.size() << " intersection device(s)"; });
+	// check every description first, so a bad list creates nothing
+	for (DeviceDescription *d : deviceDesc)
+		if (!(d->GetType() & DEVICE_TYPE_HIP_ALL))
+			throw std::runtime_error("Unknown device type in Context::CreateIntersectionDevices(): " +
+					std::to_string((int)d->GetType()));
+	std::vector<HIPIntersectionDevice *> created;
+	try {
+		for (size_t i = 0; i < deviceDesc.size(); ++i) {
+			const HIPDeviceDescription *hd = static_cast<const HIPDeviceDescription *>(deviceDesc[i]);
+			ctxLog(this, [&](std::ostream &o) {
+				o << "Allocating intersection device " << i << ": " << hd->GetName()
+				  << " (Type = " << DeviceDescription::GetDeviceType(hd->GetType()) << ")";
+			});
+			std::unique_ptr<HIPIntersectionDevice> dev(new HIPIntersectionDevice(hd->GetOrdinal()));
+			dev->desc = hd;
+			dev->deviceIndex = idevices.size() + created.size();
+			if (currentDataSet) dev->SetDataSet(currentDataSet, accelType);
+			created.push_back(dev.release());
+		}
+	} catch (...) {
+		for (HIPIntersectionDevice *d : created) delete d;
+		throw;
+	}
+	idevices.insert(idevices.end(), created.begin(), created.end());
+	return created;
+}
+
+void Context::SetDataSet(DataSet *dataSet, AcceleratorType type) {
+	if (started) throw std::runtime_error("Context::SetDataSet on a started context");
+	currentDataSet = dataSet;
+	accelType = type;
+	for (HIPIntersectionDevice *d : idevices) d->SetDataSet(dataSet, type);
+}
+
+void Context::UpdateDataSet() {
+	if (!started) throw std::runtime_error("Context::UpdateDataSet on a stopped context");
+	for (HIPIntersectionDevice *d : idevices) d->Update();
+}
+
+void Context::Start() {
+	if (started) throw std::runtime_error("Context::Start on a started context");
+	for (HIPIntersectionDevice *d : idevices) d->Start();
+	started = true;
+}
+
+void Context::Stop() {
+	if (!started) throw std::runtime_error("Context::Stop on a stopped context");
+	Interrupt();
+	for (HIPIntersectionDevice *d : idevices) d->Stop();
+	started = false;
 }
 
 }  // namespace luxrays_hip

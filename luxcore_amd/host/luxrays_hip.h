@@ -18,6 +18,10 @@
  *                                include/luxrays/core/hardwareintersectiondevice.h:35-72
  *   IntersectionDevice::TraceRay + statistics
  *                                include/luxrays/core/intersectiondevice.h:35-85
+ *   DeviceType, DeviceDescription (+ DEVICE_TYPE_HIP_GPU, HIPDeviceDescription)
+ *                                include/luxrays/core/device.h:39-82, src/luxrays/core/device.cpp
+ *   Context (descriptions, AddIntersectionDevices, SetDataSet, UpdateDataSet, Start/Stop)
+ *                                include/luxrays/core/context.h, src/luxrays/core/context.cpp:44-330
  *
  * The namespace is luxrays_hip so the mirror can be linked next to a real
  * LuxRays build.  Meshes are owned by the application, as in the reference.
@@ -181,6 +185,79 @@ private:
 
 typedef enum { ACCEL_AUTO, ACCEL_BVH, ACCEL_MBVH } AcceleratorType;
 
+// device.h:39-53 with the bit this device registers under.  The HIP GPU is
+// both a hardware and an intersection device, so it is part of ALL,
+// ALL_HARDWARE and ALL_INTERSECTION.
+typedef enum {
+	DEVICE_TYPE_NATIVE = 1 << 0,
+	DEVICE_TYPE_OPENCL_DEFAULT = 1 << 1,
+	DEVICE_TYPE_OPENCL_CPU = 1 << 2,
+	DEVICE_TYPE_OPENCL_GPU = 1 << 3,
+	DEVICE_TYPE_OPENCL_UNKNOWN = 1 << 4,
+	DEVICE_TYPE_CUDA_GPU = 1 << 5,
+	DEVICE_TYPE_HIP_GPU = 1 << 6,
+	DEVICE_TYPE_OPENCL_ALL = DEVICE_TYPE_OPENCL_DEFAULT | DEVICE_TYPE_OPENCL_CPU | DEVICE_TYPE_OPENCL_GPU |
+			DEVICE_TYPE_OPENCL_UNKNOWN,
+	DEVICE_TYPE_CUDA_ALL = DEVICE_TYPE_CUDA_GPU,
+	DEVICE_TYPE_HIP_ALL = DEVICE_TYPE_HIP_GPU,
+	DEVICE_TYPE_ALL = DEVICE_TYPE_NATIVE | DEVICE_TYPE_OPENCL_ALL | DEVICE_TYPE_HIP_ALL,
+	DEVICE_TYPE_ALL_HARDWARE = DEVICE_TYPE_OPENCL_ALL | DEVICE_TYPE_CUDA_ALL | DEVICE_TYPE_HIP_ALL,
+	DEVICE_TYPE_ALL_INTERSECTION = DEVICE_TYPE_NATIVE | DEVICE_TYPE_OPENCL_ALL | DEVICE_TYPE_HIP_ALL,
+} DeviceType;
+
+// device.h:55-82
+class DeviceDescription {
+public:
+	DeviceDescription(const std::string &deviceName, const DeviceType deviceType)
+		: name(deviceName), type(deviceType) {}
+	virtual ~DeviceDescription() {}
+
+	const std::string &GetName() const { return name; }
+	DeviceType GetType() const { return type; }
+	virtual int GetComputeUnits() const { return 1; }
+	virtual u_int GetNativeVectorWidthFloat() const { return 4; }
+	virtual size_t GetMaxMemory() const { return ~(size_t)0; }
+	virtual size_t GetMaxMemoryAllocSize() const { return ~(size_t)0; }
+	virtual bool HasOutOfCoreMemorySupport() const { return false; }
+	virtual u_int GetForceWorkGroupSize() const { return forceWorkGroupSize; }
+	virtual void SetForceWorkGroupSize(const u_int size) { forceWorkGroupSize = size; }
+
+	// device.cpp:28-52: keep the first GPU description, or none.  (The
+	// reference's CPU fallback branch repeats the GPU test and is never
+	// taken, so without a GPU the list ends empty; kept so.)
+	static void FilterOne(std::vector<DeviceDescription *> &deviceDescriptions);
+	// device.cpp:54-66: drop the descriptions with no bit of `type` (ALL keeps all)
+	static void Filter(const DeviceType type, std::vector<DeviceDescription *> &deviceDescriptions);
+	// device.cpp:68-89, plus "HIP_GPU"; other values throw
+	static std::string GetDeviceType(const DeviceType type);
+
+protected:
+	std::string name;
+	DeviceType type;
+	u_int forceWorkGroupSize = 0;
+};
+
+// The HIP counterpart of CUDADeviceDescription (cudadevice.h:36-66,
+// cudadevice.cpp:124-136): one description per visible GPU, in ordinal order.
+class HIPDeviceDescription : public DeviceDescription {
+public:
+	HIPDeviceDescription(int ordinal, const std::string &deviceName, int computeUnits, size_t totalMemory)
+		: DeviceDescription(deviceName, DEVICE_TYPE_HIP_GPU), ordinal(ordinal), cus(computeUnits),
+		  memory(totalMemory) {}
+	int GetOrdinal() const { return ordinal; }
+	int GetComputeUnits() const override { return cus; }
+	size_t GetMaxMemory() const override { return memory; }
+	// one hipMalloc may take the whole of HBM
+	size_t GetMaxMemoryAllocSize() const override { return memory; }
+	// adds nothing when no GPU is visible (the C ABI reports no device)
+	static void AddDeviceDescs(std::vector<DeviceDescription *> &descriptions);
+
+private:
+	int ordinal;
+	int cus;
+	size_t memory;
+};
+
 class HIPIntersectionDevice;
 class HardwareIntersectionKernel;
 class DataSet;
@@ -321,9 +398,16 @@ public:
 	void ResetPerformaceStats() { statsSerial = statsDataParallel = 0; }
 
 	lrh_device *GetHandle() const { return dev; }
+	// the description the Context created this device from (null when the
+	// device was constructed directly from an ordinal)
+	const DeviceDescription *GetDeviceDesc() const { return desc; }
+	size_t GetDeviceIndex() const { return deviceIndex; }
 
 private:
+	friend class Context;
 	lrh_device *dev = nullptr;
+	const DeviceDescription *desc = nullptr;
+	size_t deviceIndex = 0;
 	std::string name;
 	DataSet *dataSet = nullptr;
 	AcceleratorType accelType = ACCEL_AUTO;
@@ -333,6 +417,49 @@ private:
 	unsigned long long statsSerial = 0, statsDataParallel = 0;
 	friend class Accelerator;
 	friend class HardwareIntersectionKernel;
+};
+
+typedef void (*LuxRaysDebugHandler)(const char *msg);
+
+// context.h / context.cpp:44-330, reduced to the intersection-device part:
+// the constructor enumerates the device descriptions, AddIntersectionDevices
+// creates devices from (filtered) descriptions, SetDataSet / UpdateDataSet /
+// Start / Stop drive every device.  Descriptions and devices are owned by the
+// Context.  The reference's native-thread devices are not part of this
+// library (the CPU path is LuxRays' own), so only HIP GPUs are enumerated.
+class Context {
+public:
+	explicit Context(LuxRaysDebugHandler handler = nullptr, bool verbose = true);
+	~Context();
+
+	const std::vector<DeviceDescription *> &GetAvailableDeviceDescriptions() const { return deviceDescriptions; }
+	const std::vector<HIPIntersectionDevice *> &GetIntersectionDevices() const { return idevices; }
+	// context.cpp:250-291: a description of another type throws
+	// "Unknown device type in Context::CreateIntersectionDevices(): <type>"
+	std::vector<HIPIntersectionDevice *> AddIntersectionDevices(std::vector<DeviceDescription *> &deviceDesc);
+
+	void SetDataSet(DataSet *dataSet, AcceleratorType type = ACCEL_AUTO);  // before Start()
+	DataSet *GetCurrentDataSet() const { return currentDataSet; }
+	void UpdateDataSet();  // after a DataSet edit, while started
+	void Start();
+	void Interrupt() {}
+	void Stop();
+	bool IsRunning() const { return started; }
+
+	bool HasDebugHandler() const { return debugHandler != nullptr; }
+	bool IsVerbose() const { return verbose; }
+	void PrintDebugMsg(const std::string &msg) const {
+		if (debugHandler && verbose) debugHandler(msg.c_str());
+	}
+
+private:
+	LuxRaysDebugHandler debugHandler;
+	bool verbose;
+	bool started = false;
+	DataSet *currentDataSet = nullptr;
+	AcceleratorType accelType = ACCEL_AUTO;
+	std::vector<DeviceDescription *> deviceDescriptions;
+	std::vector<HIPIntersectionDevice *> idevices;
 };
 
 void CheckLrh(int rc);

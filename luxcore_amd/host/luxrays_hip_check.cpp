@@ -3,9 +3,15 @@
  * the reference drives an intersection device (DataSet::Add for every mesh,
  * SetDataSet/Start, AllocBufferRO/RW, EnqueueTraceRayBuffer, EnqueueReadBuffer,
  * FinishQueue, TraceRay, Update after a transform edit) on a scene read from
- * files, and writes the hit records.
+ * files, and writes the hit records.  The device comes from a Context, as in
+ * an application: the HIP GPU descriptions, filtered to one, then
+ * AddIntersectionDevices, SetDataSet, Start, UpdateDataSet, Stop.
  *
  *   luxrays_hip_check <dir> [--accel auto|bvh|mbvh] [--anyhit] [--serial N] [--scene FILE] [--refit]
+ *   luxrays_hip_check --devices     (print the Context's device descriptions)
+ *   luxrays_hip_check --registry    (DeviceDescription::Filter / FilterOne /
+ *                                    GetDeviceType and the Context's type check
+ *                                    on synthetic descriptions; no GPU needed)
  *
  * --refit            Update() refits the root tree on the GPU instead of
  *                    rebuilding it on the host
@@ -103,9 +109,82 @@ MotionSystem motion(const ObjDesc &o) {
 	return MotionSystem(o.times, keys);
 }
 
+void logMsg(const char *msg) { std::cout << "[LuxRays] " << msg << std::endl; }
+
+int listDevices() {
+	try {
+		Context ctx(logMsg, true);
+		const std::vector<DeviceDescription *> &all = ctx.GetAvailableDeviceDescriptions();
+		std::cout << all.size() << " device description(s)" << std::endl;
+		std::vector<DeviceDescription *> gpus = all;
+		DeviceDescription::Filter(DEVICE_TYPE_HIP_GPU, gpus);
+		DeviceDescription::FilterOne(gpus);
+		for (const DeviceDescription *d : gpus)
+			std::cout << "selected " << d->GetName() << " " << DeviceDescription::GetDeviceType(d->GetType())
+					  << std::endl;
+	} catch (const std::exception &e) {
+		std::cerr << "error: " << e.what() << std::endl;
+		return 1;
+	}
+	return 0;
+}
+
+int registry() {
+	const DeviceType types[] = {DEVICE_TYPE_NATIVE, DEVICE_TYPE_OPENCL_CPU, DEVICE_TYPE_OPENCL_GPU, DEVICE_TYPE_CUDA_GPU,
+			DEVICE_TYPE_HIP_GPU, DEVICE_TYPE_HIP_GPU};
+	std::vector<std::unique_ptr<DeviceDescription>> owned;
+	std::vector<DeviceDescription *> all;
+	for (size_t i = 0; i < sizeof(types) / sizeof(types[0]); ++i) {
+		owned.emplace_back(new DeviceDescription("dev" + std::to_string(i), types[i]));
+		all.push_back(owned.back().get());
+	}
+	auto show = [](const char *what, const std::vector<DeviceDescription *> &v) {
+		std::cout << what;
+		for (const DeviceDescription *d : v) std::cout << " " << d->GetName();
+		std::cout << std::endl;
+	};
+	for (DeviceType t : {DEVICE_TYPE_ALL, DEVICE_TYPE_NATIVE, DEVICE_TYPE_OPENCL_ALL, DEVICE_TYPE_OPENCL_DEFAULT,
+				 DEVICE_TYPE_OPENCL_CPU, DEVICE_TYPE_OPENCL_GPU, DEVICE_TYPE_OPENCL_UNKNOWN, DEVICE_TYPE_CUDA_GPU,
+				 DEVICE_TYPE_HIP_GPU})
+		std::cout << "type " << (int)t << " " << DeviceDescription::GetDeviceType(t) << std::endl;
+	try {
+		DeviceDescription::GetDeviceType(DEVICE_TYPE_ALL_HARDWARE);
+		std::cout << "type ALL_HARDWARE named" << std::endl;
+	} catch (const std::exception &e) {
+		std::cout << "type error: " << e.what() << std::endl;
+	}
+	for (DeviceType t : {DEVICE_TYPE_ALL, DEVICE_TYPE_HIP_GPU, DEVICE_TYPE_ALL_HARDWARE, DEVICE_TYPE_ALL_INTERSECTION,
+				 DEVICE_TYPE_OPENCL_ALL}) {
+		std::vector<DeviceDescription *> v = all;
+		DeviceDescription::Filter(t, v);
+		show(("filter " + std::to_string((int)t) + ":").c_str(), v);
+	}
+	std::vector<DeviceDescription *> one = all;
+	DeviceDescription::FilterOne(one);
+	show("filterone all:", one);
+	std::vector<DeviceDescription *> hip = all;
+	DeviceDescription::Filter(DEVICE_TYPE_HIP_GPU, hip);
+	DeviceDescription::FilterOne(hip);
+	show("filterone hip:", hip);
+	std::vector<DeviceDescription *> cpu = {all[0], all[1]};
+	DeviceDescription::FilterOne(cpu);
+	show("filterone cpu-only:", cpu);
+	try {
+		Context ctx(nullptr, false);
+		std::vector<DeviceDescription *> bad = {all[4], all[0]};
+		ctx.AddIntersectionDevices(bad);
+		std::cout << "context accepted" << std::endl;
+	} catch (const std::exception &e) {
+		std::cout << "context error: " << e.what() << std::endl;
+	}
+	return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
+	if (argc == 2 && std::string(argv[1]) == "--devices") return listDevices();
+	if (argc == 2 && std::string(argv[1]) == "--registry") return registry();
 	if (argc < 2) {
 		std::cerr << "usage: luxrays_hip_check <dir> [--accel auto|bvh|mbvh] [--anyhit] [--serial N] [--scene FILE] "
 		             "[--refit]\n";
@@ -226,9 +305,14 @@ int main(int argc, char **argv) {
 		const u_int n = (u_int)(rb.size() / sizeof(Ray));
 		std::vector<RayHit> hits(n);
 
-		HIPIntersectionDevice device(0);
-		device.SetDataSet(&ds, accelType);
-		device.Start();
+		Context ctx(nullptr, false);
+		std::vector<DeviceDescription *> devDescs = ctx.GetAvailableDeviceDescriptions();
+		DeviceDescription::Filter(DEVICE_TYPE_HIP_GPU, devDescs);
+		DeviceDescription::FilterOne(devDescs);
+		if (devDescs.empty()) throw std::runtime_error("lrhip: no HIP GPU device available");
+		HIPIntersectionDevice &device = *ctx.AddIntersectionDevices(devDescs)[0];
+		ctx.SetDataSet(&ds, accelType);
+		ctx.Start();
 		std::cout << "device " << device.GetName() << " accel "
 				  << (device.GetAccelerator()->GetType() == ACCEL_BVH ? "BVH" : "MBVH") << " rays " << n << std::endl;
 
@@ -269,7 +353,7 @@ int main(int argc, char **argv) {
 			}
 			if (refit)
 				const_cast<Accelerator *>(device.GetAccelerator())->SetUpdateMode(Accelerator::UPDATE_REFIT);
-			device.Update();
+			ctx.UpdateDataSet();
 			traceAll("hits2.bin");
 		}
 
@@ -277,7 +361,7 @@ int main(int argc, char **argv) {
 				  << device.GetSerialRayCount() << " used memory " << device.GetUsedMemory() << std::endl;
 		device.FreeBuffer(&rayBuff);
 		device.FreeBuffer(&hitBuff);
-		device.Stop();
+		ctx.Stop();
 	} catch (const std::exception &e) {
 		std::cerr << "error: " << e.what() << std::endl;
 		return 1;
