@@ -34,6 +34,9 @@ using lrh::Error;
 static_assert(sizeof(lrh_node4q) == 64, "node4q must be 64 bytes");
 static_assert(sizeof(lrh_node4f) == 128, "node4f must be 128 bytes");
 static_assert(sizeof(lrh_tri) == 48, "tri record must be 48 bytes");
+static_assert(sizeof(lrh_motion) == 32, "motion record must be 32 bytes");
+static_assert(sizeof(lrh_motion_seg) == 304 && offsetof(lrh_motion_seg, start) == LRH_SEG_HOT_BYTES,
+		"motion segment: 112 hot bytes, then the matrices");
 static_assert(sizeof(lrh_ray_rec) == LRH_RAY_BYTES, "Ray must be 48 bytes");
 static_assert(sizeof(lrh_hit_rec) == LRH_HIT_BYTES, "RayHit must be 20 bytes");
 
@@ -328,6 +331,8 @@ void prepareObjects(lrh_scene *s, const lrh_object *objs, unsigned n, ObjectData
 			md.nTimes = (uint32_t)ms.times.size();
 			md.firstSeg = (uint32_t)od.segs.size();
 			md.pad = 0;
+			for (unsigned k = 0; k < LRH_MOTION_INLINE_TIMES; ++k)
+				md.t[k] = k < ms.times.size() ? ms.times[k] : 0.f;
 			od.times.insert(od.times.end(), ms.times.begin(), ms.times.end());
 			od.segs.insert(od.segs.end(), ms.segs.begin(), ms.segs.end());
 			in.index = (uint32_t)od.mot.size();
@@ -829,7 +834,7 @@ int lrh_scene_create_mbvh(lrh_device *dev, const lrh_mesh *meshes, unsigned int 
 		s->insts.upload(instOut.data(), instOut.size() * sizeof(lrh_inst), dev->stream);
 		s->xforms.alloc(dev, std::max<size_t>(od.xf.size() * 4, 64));
 		s->xforms.upload(od.xf.data(), od.xf.size() * 4, dev->stream);
-		s->motions.alloc(dev, std::max<size_t>(od.mot.size() * sizeof(lrh_motion), 16));
+		s->motions.alloc(dev, std::max<size_t>(od.mot.size() * sizeof(lrh_motion), sizeof(lrh_motion)));
 		s->motions.upload(od.mot.data(), od.mot.size() * sizeof(lrh_motion), dev->stream);
 		s->segs.alloc(dev, std::max<size_t>(od.segs.size() * sizeof(lrh_motion_seg), sizeof(lrh_motion_seg)));
 		s->segs.upload(od.segs.data(), od.segs.size() * sizeof(lrh_motion_seg), dev->stream);

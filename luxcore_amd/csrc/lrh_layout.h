@@ -87,32 +87,40 @@ typedef struct __attribute__((aligned(16))) {
 /* Restated InterpolatedTransform for the device
  * (src/luxrays/core/geometry/motionsystem.cpp:37-159).  flags bits:
  * 0 active, 1 rotation, 2 translation, 3 scale, 4/5/6 translation x/y/z,
- * 7/8/9 scale x/y/z. */
+ * 7/8/9 scale x/y/z.  The first 112 bytes hold everything a sample between
+ * the key times reads (seven 16-byte loads issued together); the endpoint
+ * matrices and startT's rotation follow. */
 typedef struct __attribute__((aligned(16))) {
 	float startTime, endTime;
 	uint32_t flags;
-	uint32_t pad0;
-	float start[16];    /* start.m */
-	float end[16];      /* end.m */
-	float startR[16];   /* startT.R */
-	float startQ[4];    /* w, x, y, z */
-	float endQ[4];
-	float sS[3], eS[3]; /* Sx Sy Sz of startT / endT */
-	float sT[3], eT[3]; /* Tx Ty Tz of startT / endT */
 	/* Slerp's per-segment constants (quaternion.cpp:143-164): the sign that
 	 * takes the shorter arc, and, when 1 - cos_phi > 1e-6 (slerpArc = 1),
 	 * phi = acos(cos_phi) and sin(phi), each evaluated in double and rounded
 	 * once to float; only sin((1-t) phi) and sin(t phi) depend on the time */
-	float qsign, phi, sinPhi;
 	uint32_t slerpArc;
+	float qsign, phi, sinPhi;
+	uint32_t pad0;
+	float startQ[4];    /* w, x, y, z */
+	float endQ[4];
+	float sS[3], eS[3]; /* Sx Sy Sz of startT / endT */
+	float sT[3], eT[3]; /* Tx Ty Tz of startT / endT */
+	float start[16];    /* start.m */
+	float end[16];      /* end.m */
+	float startR[16];   /* startT.R */
 } lrh_motion_seg;
 
-/* A motion system: keys times[0..nTimes) and nTimes+1 segments. */
+#define LRH_SEG_HOT_BYTES 112
+
+/* A motion system: keys times[0..nTimes) and nTimes+1 segments.  The first
+ * LRH_MOTION_INLINE_TIMES key times are repeated in the record, so choosing
+ * the segment of a ray's time needs no load beyond the record. */
+#define LRH_MOTION_INLINE_TIMES 4
 typedef struct __attribute__((aligned(16))) {
 	uint32_t firstTime;
 	uint32_t nTimes;
 	uint32_t firstSeg;
 	uint32_t pad;
+	float t[LRH_MOTION_INLINE_TIMES];
 } lrh_motion;
 
 enum {

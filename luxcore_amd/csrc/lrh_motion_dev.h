@@ -17,22 +17,55 @@
 
 __device__ __forceinline__ float lrhLerp(float t, float a, float b) { return a + t * (b - a); }
 
+// The first LRH_SEG_HOT_BYTES of a segment in registers: the seven loads
+// are issued together and waited for once (read field by field behind the
+// branches, they would wait one after another).
+struct LrhSegHot {
+	float startTime, endTime;
+	uint32_t flags, slerpArc;
+	float qsign, phi, sinPhi, pad0;
+	float startQ[4], endQ[4];
+	float sS[3], eS[3], sT[3], eT[3];
+};
+static_assert(sizeof(LrhSegHot) == LRH_SEG_HOT_BYTES, "hot part of lrh_motion_seg");
+
+__device__ __forceinline__ LrhSegHot lrhLoadSegHot(const lrh_motion_seg &g) {
+	const uint4 *p = reinterpret_cast<const uint4 *>(&g);
+	uint32_t w[28];
+#pragma unroll
+	for (int k = 0; k < 7; ++k) {
+		const uint4 q = p[k];
+		w[4 * k] = q.x;
+		w[4 * k + 1] = q.y;
+		w[4 * k + 2] = q.z;
+		w[4 * k + 3] = q.w;
+	}
+	// an empty asm that takes the first 32 bytes (times, flags, slerp
+	// constants): those loads stay here, ahead of the branches below
+#pragma unroll
+	for (int k = 0; k < 8; ++k) asm volatile("" : "+v"(w[k]));
+	LrhSegHot h;
+	__builtin_memcpy(&h, w, sizeof(h));
+	return h;
+}
+
 // one segment at `time` -> row-major M (global->local for scene objects)
-__device__ __forceinline__ void lrhSegSample(const lrh_motion_seg &g, float time, float M[16]) {
+__device__ __forceinline__ void lrhSegSample(const lrh_motion_seg &seg, float time, float M[16]) {
+	const LrhSegHot g = lrhLoadSegHot(seg);
 	const uint32_t f = g.flags;
 	if (!(f & LRH_MF_ACTIVE) || time <= g.startTime) {
-		for (int i = 0; i < 16; ++i) M[i] = g.start[i];
+		for (int i = 0; i < 16; ++i) M[i] = seg.start[i];
 		return;
 	}
 	if (time >= g.endTime) {
-		for (int i = 0; i < 16; ++i) M[i] = g.end[i];
+		for (int i = 0; i < 16; ++i) M[i] = seg.end[i];
 		return;
 	}
 	const float w = g.endTime - g.startTime;
 	const float d = time - g.startTime;
 	const float le = d / w;
 	if ((f & LRH_MF_TRANS) && !(f & (LRH_MF_SCALE | LRH_MF_ROT))) {
-		for (int i = 0; i < 16; ++i) M[i] = g.start[i];
+		for (int i = 0; i < 16; ++i) M[i] = seg.start[i];
 		if (f & LRH_MF_TX) M[3] = lrhLerp(le, g.sT[0], g.eT[0]);
 		if (f & LRH_MF_TY) M[7] = lrhLerp(le, g.sT[1], g.eT[1]);
 		if (f & LRH_MF_TZ) M[11] = lrhLerp(le, g.sT[2], g.eT[2]);
@@ -70,7 +103,7 @@ __device__ __forceinline__ void lrhSegSample(const lrh_motion_seg &g, float time
 		M[12] = M[13] = M[14] = 0.f;
 		M[15] = 1.f;
 	} else {
-		for (int i = 0; i < 16; ++i) M[i] = g.startR[i];
+		for (int i = 0; i < 16; ++i) M[i] = seg.startR[i];
 	}
 	float S[3];
 	for (int k = 0; k < 3; ++k) S[k] = (f & LRH_MF_SCALE) ? lrhLerp(le, g.sS[k], g.eS[k]) : g.sS[k];

@@ -89,13 +89,28 @@ DEVI void xfRay(const float *M, const float wo[3], const float wd[3], LocalRay &
 // evaluates it (InterpolatedTransform::Sample, :89-159) ----
 __device__ __forceinline__ void motionSample(const lrh_motion *mot, const lrh_motion_seg *segs, const float *times,
 		float time, float M[16]) {
-	// upper_bound(times, time), clamped to the last key
-	const uint32_t n = mot->nTimes;
-	const float *tk = times + mot->firstTime;
+	// upper_bound(times, time), clamped to the last key; the record's
+	// inline key times serve systems of up to LRH_MOTION_INLINE_TIMES keys
+	const uint4 a = reinterpret_cast<const uint4 *>(mot)[0];
+	const uint4 b = reinterpret_cast<const uint4 *>(mot)[1];
+	// both halves of the record in one wait
+	uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+	for (int k = 0; k < 8; ++k) asm volatile("" : "+v"(w[k]));
+	const uint32_t n = w[1];
 	uint32_t idx = 0;
-	while (idx < n && !(time < tk[idx])) ++idx;
+	if (n <= LRH_MOTION_INLINE_TIMES) {
+		const float tk[4] = {__uint_as_float(w[4]), __uint_as_float(w[5]), __uint_as_float(w[6]),
+				__uint_as_float(w[7])};
+#pragma unroll
+		for (uint32_t k = 0; k < 4; ++k)
+			if (idx == k && k < n && !(time < tk[k])) idx = k + 1;
+	} else {
+		const float *tk = times + w[0];
+		while (idx < n && !(time < tk[idx])) ++idx;
+	}
 	if (idx > n - 1) idx = n - 1;
-	lrhSegSample(segs[mot->firstSeg + idx], time, M);
+	lrhSegSample(segs[w[2] + idx], time, M);
 }
 
 // ---- node tests: up to 4 children -> (entry distance, child ref) ----
@@ -540,11 +555,12 @@ __device__ __forceinline__ void traceBody(const LrhTraceParams &P) {
 
 // Single-level traces are built for 7 waves per SIMD (at most 72 VGPRs, no
 // spills), measured 1-3% faster than the 6 the compiler picks on its own.
-// Two-level traces over 64-byte nodes are built for 6 waves (80 VGPRs, no
-// spills): with the motion sample inlined, 7% faster on moving instances
-// than the compiler's 5 waves and within 1% on static ones; the other
-// two-level variants (fp32 nodes, batched, counting) keep the compiler's
-// choice, which does not spill.
+// Two-level traces over 64-byte nodes are built for 5 waves (96 VGPRs, the
+// only spills in the prologue): the motion sample is inlined and reads its
+// motion record and the hot part of its segment with all loads in flight
+// at once, which needs the registers (6 waves spill 40 in the loop and
+// lose 19%); the other two-level variants (fp32 nodes, batched, counting)
+// keep the compiler's choice, which does not spill.
 template <int LAYOUT, bool ANY, bool COUNT, bool BATCH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void lrh_trace_kernel(LrhTraceParams P) {
 	traceBody<LAYOUT, false, ANY, COUNT, BATCH>(P);
@@ -556,7 +572,7 @@ __global__ __launch_bounds__(256) void lrh_trace2_kernel(LrhTraceParams P) {
 }
 
 template <bool ANY, bool COUNT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void lrh_trace2q_kernel(LrhTraceParams P) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void lrh_trace2q_kernel(LrhTraceParams P) {
 	traceBody<0, true, ANY, COUNT, false>(P);
 }  // (the counting variants keep lrh_trace2_kernel: at 80 VGPRs they would spill)
 
