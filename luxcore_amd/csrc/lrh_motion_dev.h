@@ -2,8 +2,8 @@
  * InterpolatedTransform::Sample on the device
  * (src/luxrays/core/geometry/motionsystem.cpp:89-159), rounded as written:
  * the files that include it build with -ffp-contract=off and correctly
- * rounded division; slerp's sines are evaluated in double and rounded
- * once, and its time-independent constants (sign, phi = acos(cos_phi),
+ * rounded division; slerp's sines are evaluated in double (lrh_sin.h) and
+ * rounded once, and its time-independent constants (sign, phi = acos(cos_phi),
  * sin(phi)) come with the segment from the host, so host and device
  * produce the same bits.
  */
@@ -14,6 +14,7 @@
 #include <stdint.h>
 
 #include "lrh_layout.h"
+#include "lrh_sin.h"
 
 __device__ __forceinline__ float lrhLerp(float t, float a, float b) { return a + t * (b - a); }
 
@@ -76,10 +77,11 @@ __device__ __forceinline__ void lrhSegSample(const lrh_motion_seg &seg, float ti
 		// the segment holds the shorter-arc sign, phi and sin(phi)
 		float f1, f2;
 		if (g.slerpArc) {
-			// correctly rounded fp32 sin: evaluated in double, rounded once,
-			// so every platform gets the same bits
-			f1 = (float)sin((double)((1.f - le) * g.phi)) / g.sinPhi;
-			f2 = (float)sin((double)(le * g.phi)) / g.sinPhi;
+			// sin evaluated in double and rounded once to float: lrhSin0Pi
+			// gives the float the C library's sin gives for every float
+			// argument in [0, pi] (the slerp's angles lie in [0, pi/2])
+			f1 = (float)lrhSin0Pi((double)((1.f - le) * g.phi)) / g.sinPhi;
+			f2 = (float)lrhSin0Pi((double)(le * g.phi)) / g.sinPhi;
 		} else {
 			f1 = 1.f - le;
 			f2 = le;

@@ -555,12 +555,12 @@ __device__ __forceinline__ void traceBody(const LrhTraceParams &P) {
 
 // Single-level traces are built for 7 waves per SIMD (at most 72 VGPRs, no
 // spills), measured 1-3% faster than the 6 the compiler picks on its own.
-// Two-level traces over 64-byte nodes are built for 5 waves (96 VGPRs, the
-// only spills in the prologue): the motion sample is inlined and reads its
-// motion record and the hot part of its segment with all loads in flight
-// at once, which needs the registers (6 waves spill 40 in the loop and
-// lose 19%); the other two-level variants (fp32 nodes, batched, counting)
-// keep the compiler's choice, which does not spill.
+// Two-level traces over 64-byte nodes are built for 6 waves (80 VGPRs, 8
+// spilled): the motion sample is inlined and reads its motion record and
+// the hot part of its segment with all loads in flight at once; measured on
+// moving instances, 6 waves are 3% faster than 5 (90 VGPRs, no spills);
+// the other two-level variants (fp32 nodes, batched, counting) keep the
+// compiler's choice, which does not spill.
 template <int LAYOUT, bool ANY, bool COUNT, bool BATCH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void lrh_trace_kernel(LrhTraceParams P) {
 	traceBody<LAYOUT, false, ANY, COUNT, BATCH>(P);
@@ -572,7 +572,7 @@ __global__ __launch_bounds__(256) void lrh_trace2_kernel(LrhTraceParams P) {
 }
 
 template <bool ANY, bool COUNT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void lrh_trace2q_kernel(LrhTraceParams P) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void lrh_trace2q_kernel(LrhTraceParams P) {
 	traceBody<0, true, ANY, COUNT, false>(P);
 }  // (the counting variants keep lrh_trace2_kernel: at 80 VGPRs they would spill)
 
