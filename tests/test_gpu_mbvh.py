@@ -157,6 +157,55 @@ def test_translation_and_scale_motion_bit_exact(gpu_device):
     assert st["hits"] > 20000 and st["mismatch"] == st["ties"] + st["edge_band"]
 
 
+def test_multi_key_motion_and_key_times(gpu_device):
+    """Motion systems of 1 to 7 keys at uneven times (up to 4 key times are
+    read from the motion record, more from the key-time array), rotating,
+    scaling and translating; ray times before the first key, after the last,
+    exactly on keys and between them. Closest hits equal the oracle bit for
+    bit (MotionSystem::Sample, src/luxrays/core/geometry/motionsystem.cpp:
+    296-346), before and after an edit through the device refit."""
+    _case("multi_key_motion")
+    base = scenes.displaced_sphere(freq=16, seed=5)
+    rng = np.random.Generator(np.random.PCG64(77))
+    key_counts = [1, 2, 3, 4, 5, 6, 7] * 6
+    n = len(key_counts)
+    key_off = np.zeros(n + 1, np.uint32)
+    kt, km = [], []
+    for o, k in enumerate(key_counts):
+        times = np.sort(rng.uniform(-0.1, 1.2, size=k)).astype(np.float32)
+        times[0] = np.float32(0.0) if o % 3 == 0 else times[0]
+        times = np.unique(times)
+        M = scenes.random_trs(rng, 6.0)
+        for t in times:
+            M = M.copy()
+            M[:3, 3] += rng.uniform(-1.0, 1.0, 3).astype(np.float32)
+            if rng.random() < 0.6:
+                R = scenes._random_rotation(rng).astype(np.float32)
+                a = np.float32(rng.uniform(0.05, 0.6))
+                M[:3, :3] = M[:3, :3] @ ((1 - a) * np.eye(3, dtype=np.float32) + a * R)
+            kt.append(float(t))
+            km.append(np.linalg.inv(M.astype(np.float64)))
+        key_off[o + 1] = key_off[o] + len(times)
+    sc = scenes.InstancedScene([base], np.zeros(n, np.uint32), np.full(n, 2, np.uint32),
+                               np.zeros((n, 4, 4), np.float32), key_off, np.asarray(kt, np.float32),
+                               np.asarray(km, np.float32).reshape(-1, 4, 4))
+    rays = scenes.instanced_rays(200000, extent=7.0, seed=78)
+    rays["time"] = rng.uniform(-0.3, 1.4, size=rays.shape[0]).astype(np.float32)
+    on_key = rng.choice(rays.shape[0], 20000, replace=False)
+    rays["time"][on_key] = np.asarray(kt, np.float32)[rng.integers(0, len(kt), size=on_key.size)]
+    dsc = gpu_device.scene_mbvh(sc)
+    st = _check(oracle.OracleMBVH(sc), gpu_device, dsc, rays)
+    assert st["hits"] > 20000 and st["mismatch"] == st["ties"] + st["edge_band"]
+    # an edit of every key through the device refit
+    sc2 = scenes.InstancedScene(sc.meshes, sc.obj_mesh, sc.obj_kind, sc.obj_mat, sc.key_off, sc.key_times,
+                                sc.key_mats.copy())
+    sc2.key_mats[:, :3, 3] += rng.uniform(-0.3, 0.3, size=(sc2.key_mats.shape[0], 3)).astype(np.float32)
+    dsc.refit(sc2)
+    st = _check(oracle.OracleMBVH(sc2), gpu_device, dsc, rays)
+    assert st["hits"] > 20000 and st["mismatch"] == st["ties"] + st["edge_band"]
+    dsc.close()
+
+
 def test_empty_two_level(gpu_device):
     _case("empty_mbvh")
     sc = scenes.InstancedScene([scenes.geodesic_sphere(4)], np.zeros(0, np.uint32), np.zeros(0, np.uint32),
