@@ -206,6 +206,54 @@ def test_multi_key_motion_and_key_times(gpu_device):
     dsc.close()
 
 
+def test_projective_instance_matrices(gpu_device):
+    """Instances whose matrices have a last row other than (0, 0, 0, 1): the
+    ray origin takes the projective divide of Matrix4x4 * Point
+    (include/luxrays/core/geometry/matrix4x4op.h:31-59) and the instance box
+    the divided corners (transform.h:271-280). The local ray of a projective
+    transform is not the image of the world ray (its parameter is not the
+    world t), so MBVHAccel's result depends on its root tree: root leaves are
+    not box-tested, only their parent nodes (mbvhaccel.cpp:290-347), and a
+    hit found inside an instance can lie outside that instance's world box or
+    before its world entry distance. Parity therefore holds exactly for the
+    affine instances of the scene; a hit the oracle finds on a projective
+    instance may be missed by the device (its root test is per instance and
+    front to back), on a small fraction of rays."""
+    _case("projective_instances")
+    base = scenes.displaced_sphere(freq=16, seed=6)
+    rng = np.random.Generator(np.random.PCG64(91))
+    n = 30
+    mats = np.zeros((n, 4, 4), np.float32)
+    for o in range(n):
+        M = scenes.random_trs(rng, 6.0)
+        if o % 2 == 0:
+            M[3, :3] = rng.uniform(-0.02, 0.02, 3).astype(np.float32)
+            M[3, 3] = np.float32(rng.uniform(0.9, 1.1))
+        mats[o] = M
+    projective = set(range(0, n, 2))
+    sc = scenes.InstancedScene([base], np.zeros(n, np.uint32), np.ones(n, np.uint32), mats,
+                               np.zeros(n + 1, np.uint32), np.zeros(0, np.float32), np.zeros((0, 4, 4), np.float32))
+    rays = scenes.instanced_rays(200000, extent=7.0, seed=92)
+    ref = oracle.OracleMBVH(sc).intersect(rays)
+    got = gpu_device.trace_numpy(gpu_device.scene_mbvh(sc), rays)
+    st = compare(ref, got)
+    print(st, flush=True)
+    assert st["hits"] > 20000 and st["miss_t_ok"]
+    mism = np.nonzero((ref["meshIndex"] != got["meshIndex"]) | (ref["triangleIndex"] != got["triangleIndex"]))[0]
+    hit_ref = ref["meshIndex"] != NULL_INDEX
+    hit_got = got["meshIndex"] != NULL_INDEX
+    for i in mism:
+        # one of the two hits is on a projective instance
+        assert int(ref["meshIndex"][i]) in projective or int(got["meshIndex"][i]) in projective, i
+    closer_ref = int(np.sum(hit_ref[mism] & (~hit_got[mism] | (ref["t"][mism] < got["t"][mism]))))
+    print(f"projective mismatches {mism.size}: the oracle's hit closer in {closer_ref}", flush=True)
+    assert mism.size <= 1e-3 * rays.shape[0]
+    # where the records agree they agree bit for bit
+    same = np.setdiff1d(np.nonzero(hit_ref)[0], mism)
+    for f in ("t", "b1", "b2"):
+        assert np.array_equal(ref[f][same], got[f][same])
+
+
 def test_empty_two_level(gpu_device):
     _case("empty_mbvh")
     sc = scenes.InstancedScene([scenes.geodesic_sphere(4)], np.zeros(0, np.uint32), np.zeros(0, np.uint32),
