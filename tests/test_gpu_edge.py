@@ -85,6 +85,32 @@ def test_special_rays_plain_scene(gpu_device):
     assert (got["meshIndex"][nan_mint] != NULL_INDEX).sum() > 10
 
 
+def test_special_rays_two_level(gpu_device):
+    """The same special rays through MBVHAccel (instances and 2-key motion):
+    NaN or out-of-range ray times as well (MotionSystem::Sample clamps to the
+    end keys; a NaN time samples the last segment with NaN weights)."""
+    _case("special_rays_two_level")
+    rng = np.random.default_rng(73)
+    sc = scenes.instanced_scene(n_inst=40, motion_frac=0.25, freq=10, extent=8.0, seed=74)
+    rays = _special_rays(rng, n=8192)
+    rays["o"] = (rays["o"] - np.float32(5.0)).astype(np.float32)
+    rays["time"] = rng.random(rays.shape[0], dtype=np.float32)
+    k = rays.shape[0] // 16
+    rays["time"][15 * k:15 * k + k // 3] = np.nan
+    rays["time"][15 * k + k // 3:15 * k + 2 * k // 3] = -1.0
+    rays["time"][15 * k + 2 * k // 3:16 * k] = 2.0
+    init = np.zeros(rays.shape[0], HIT_DTYPE)
+    init["t"] = -7.0
+    got = gpu_device.trace_numpy(gpu_device.scene_mbvh(sc), rays, hits_init=init)
+    live = (rays["flags"] & 1) == 0
+    assert np.all(got["t"][~live] == -7.0)
+    ref = oracle.OracleMBVH(sc).intersect(rays[live])
+    st = compare(ref, got[live])
+    print(st, flush=True)
+    assert_parity(st)
+    assert st["hits"] > 500
+
+
 def test_far_from_origin(gpu_device):
     _case("far_coordinates")
     rng = np.random.default_rng(72)
