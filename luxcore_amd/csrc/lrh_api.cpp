@@ -198,6 +198,9 @@ lrh::BuildParams toParams(const lrh_build_opts *o) {
 		if (o->nThreads) p.nThreads = o->nThreads;
 		if (o->travCost > 0.f) p.travCost = o->travCost;
 	}
+	// early split clipping: 0 selects the default budget, a negative value turns it off
+	p.splitBudget = (o && o->splitBudget != 0.f) ? o->splitBudget : 4.0f;
+	if (!(p.splitBudget <= 64.f)) throw Error(LRH_EINVAL, "splitBudget must be at most 64");
 	if (p.maxLeaf < 1 || p.maxLeaf > LRH_LEAF_MAX) throw Error(LRH_EINVAL, "maxLeafSize must be 1..8");
 	return p;
 }
@@ -252,17 +255,42 @@ uint32_t buildMeshTree(const lrh_mesh *meshes, const std::vector<unsigned> &mesh
 			}
 		});
 	}
+	// large triangles cut into several references (early split clipping)
+	struct Source : lrh::TriangleSource {
+		const lrh_mesh *meshes;
+		const std::vector<unsigned> *list;
+		const std::vector<size_t> *off;
+		void get(size_t prim, float q[3][3]) const override {
+			const size_t mi = std::upper_bound(off->begin(), off->end(), prim) - off->begin() - 1;
+			const lrh_mesh &m = meshes[(*list)[mi]];
+			const unsigned *t = m.tris + 3 * (prim - (*off)[mi]);
+			for (int c = 0; c < 3; ++c)
+				for (int k = 0; k < 3; ++k) q[c][k] = m.verts[3 * (size_t)t[c] + k];
+		}
+	} src;
+	src.meshes = meshes;
+	src.list = &meshList;
+	src.off = &off;
+	std::vector<Box> refBoxes;
+	std::vector<uint32_t> refPrim;
+	lrh::splitLargeTriangles(boxes, src, p.splitBudget, nt, &refBoxes, &refPrim);
+	const bool cut = !refPrim.empty();
+	const size_t nRefs = cut ? refPrim.size() : n;
+	if (tris.size() + nRefs > LRH_MAX_LEAF_START)
+		throw Error(LRH_ERANGE, "scene exceeds 268,435,455 triangle records");
 	lrh::WideBVH w;
 	const uint32_t nodeBase = (uint32_t)ht.count(p.layout);
 	const uint32_t primBase = (uint32_t)tris.size();
-	lrh::buildWideBVH(boxes, p, nodeBase, primBase, &w);
+	lrh::buildWideBVH(cut ? refBoxes : boxes, p, nodeBase, primBase, &w);
 	boxes.clear();
 	boxes.shrink_to_fit();
+	refBoxes.clear();
+	refBoxes.shrink_to_fit();
 	ht.append(w, p.layout);
-	tris.resize(primBase + n);
-	parallelRange(n, nt, [&](size_t b, size_t e) {
+	tris.resize(primBase + nRefs);
+	parallelRange(nRefs, nt, [&](size_t b, size_t e) {
 		for (size_t i = b; i < e; ++i) {
-			const size_t prim = w.order[i];
+			const size_t prim = cut ? refPrim[w.order[i]] : w.order[i];
 			const size_t mi = std::upper_bound(off.begin(), off.end(), prim) - off.begin() - 1;
 			const lrh_mesh &m = meshes[meshList[mi]];
 			const size_t k = prim - off[mi];
@@ -280,8 +308,12 @@ uint32_t buildMeshTree(const lrh_mesh *meshes, const std::vector<unsigned> &mesh
 		}
 	});
 	if (invPerm) {
-		invPerm->resize(n);
-		for (size_t i = 0; i < n; ++i) (*invPerm)[w.order[i]] = (uint32_t)(primBase + i);
+		// triangle -> its record (the first one of a cut triangle)
+		invPerm->assign(n, UINT32_MAX);
+		for (size_t i = 0; i < nRefs; ++i) {
+			const size_t prim = cut ? refPrim[w.order[i]] : w.order[i];
+			if ((*invPerm)[prim] == UINT32_MAX) (*invPerm)[prim] = (uint32_t)(primBase + i);
+		}
 	}
 	if (w.maxDepth > *maxDepth) *maxDepth = w.maxDepth;
 	return w.root;

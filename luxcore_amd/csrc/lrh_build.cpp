@@ -417,6 +417,167 @@ unsigned hardwareThreads() {
 }
 
 // ---------------------------------------------------------------------------
+// early split clipping
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Poly {
+	// a triangle clipped to an axis-aligned box has at most 9 vertices;
+	// rounding may repeat some, hence the room
+	float v[16][3];
+	int n = 0;
+};
+
+// the part of `in` on one side of the plane x[axis] = c (keepLow: x <= c);
+// false if the result would not fit (the caller then keeps the piece whole)
+bool clipPoly(const Poly &in, int axis, float c, bool keepLow, Poly *out) {
+	out->n = 0;
+	for (int i = 0; i < in.n; ++i) {
+		const float *a = in.v[i], *b = in.v[(i + 1) % in.n];
+		const bool ina = keepLow ? a[axis] <= c : a[axis] >= c;
+		const bool inb = keepLow ? b[axis] <= c : b[axis] >= c;
+		if (ina) {
+			if (out->n == 16) return false;
+			for (int k = 0; k < 3; ++k) out->v[out->n][k] = a[k];
+			++out->n;
+		}
+		if (ina != inb) {
+			if (out->n == 16) return false;
+			const float t = (c - a[axis]) / (b[axis] - a[axis]);
+			for (int k = 0; k < 3; ++k) out->v[out->n][k] = a[k] + t * (b[k] - a[k]);
+			out->v[out->n][axis] = c;
+			++out->n;
+		}
+	}
+	return true;
+}
+
+// box of the polygon, clamped to `cell`, expanded by MachineEpsilon::E
+Box polyBox(const Poly &p, const Box &cell) {
+	Box b = Box::Empty();
+	for (int i = 0; i < p.n; ++i) b.growPoint(p.v[i]);
+	for (int k = 0; k < 3; ++k) {
+		b.lo[k] = fmax_t(b.lo[k], cell.lo[k]);
+		b.hi[k] = fmin_t(b.hi[k], cell.hi[k]);
+	}
+	const float e = machineEpsilon(b);
+	for (int k = 0; k < 3; ++k) {
+		b.lo[k] -= e;
+		b.hi[k] += e;
+	}
+	return b;
+}
+
+// cut one triangle's box until each piece's area is below `target` or
+// `pieces` references are made; returns the references' boxes
+void cutTriangle(const float p[3][3], const Box &box, float target, unsigned maxPieces, std::vector<Box> &out) {
+	struct Item { Poly poly; Box cell; };
+	std::vector<Item> work;
+	Item first;
+	first.poly.n = 3;
+	for (int i = 0; i < 3; ++i)
+		for (int k = 0; k < 3; ++k) first.poly.v[i][k] = p[i][k];
+	first.cell = box;
+	work.push_back(first);
+	std::vector<Box> done;
+	while (!work.empty()) {
+		Item it = work.back();
+		work.pop_back();
+		const Box pb = polyBox(it.poly, it.cell);
+		if (pb.area() <= target || done.size() + work.size() + 2 > maxPieces) {
+			done.push_back(pb);
+			continue;
+		}
+		int axis = 0;
+		for (int k = 1; k < 3; ++k)
+			if (pb.hi[k] - pb.lo[k] > pb.hi[axis] - pb.lo[axis]) axis = k;
+		const float c = 0.5f * (pb.lo[axis] + pb.hi[axis]);
+		Item lo, hi;
+		if (!clipPoly(it.poly, axis, c, true, &lo.poly) || !clipPoly(it.poly, axis, c, false, &hi.poly)) {
+			done.push_back(pb);
+			continue;
+		}
+		lo.cell = pb;
+		lo.cell.hi[axis] = c;
+		hi.cell = pb;
+		hi.cell.lo[axis] = c;
+		if (lo.poly.n >= 3) work.push_back(lo);
+		if (hi.poly.n >= 3) work.push_back(hi);
+		if (lo.poly.n < 3 && hi.poly.n < 3) done.push_back(pb);  // degenerate: keep the piece whole
+	}
+	out.insert(out.end(), done.begin(), done.end());
+}
+
+}  // namespace
+
+void splitLargeTriangles(const std::vector<Box> &boxes, const TriangleSource &tris, float budget, unsigned nThreads,
+		std::vector<Box> *refBoxes, std::vector<uint32_t> *refPrim) {
+	refBoxes->clear();
+	refPrim->clear();
+	const size_t n = boxes.size();
+	if (!(budget > 0.f) || n < 2) return;
+	// typical box: the median area; a box is cut when it is above `factor`
+	// times that (2 to start with), the factor raised until the cut pieces
+	// fit the budget
+	std::vector<float> area(n);
+	for (size_t i = 0; i < n; ++i) area[i] = boxes[i].area();
+	std::vector<float> tmp(area);
+	std::nth_element(tmp.begin(), tmp.begin() + n / 2, tmp.end());
+	const float median = tmp[n / 2];
+	if (!(median > 0.f)) return;
+	const double allowed = (double)budget * (double)n;
+	float factor = 2.f;
+	std::vector<uint32_t> cand;
+	for (int round = 0; round < 24; ++round, factor *= 2.f) {
+		// pieces needed: about area / target per candidate (halving a box
+		// roughly halves its area), at most 64 per triangle
+		const float target = factor * median;
+		double extra = 0.0;
+		cand.clear();
+		for (size_t i = 0; i < n; ++i)
+			if (area[i] > target) {
+				cand.push_back((uint32_t)i);
+				extra += std::min(63.0, (double)area[i] / target);
+			}
+		if (extra <= allowed) break;
+	}
+	if (cand.empty()) return;
+	const float target = factor * median;
+	// cut the candidates in parallel, in chunks, then place the references:
+	// every triangle keeps its slot, its extra pieces follow all triangles
+	const unsigned nt = std::max(1u, std::min<unsigned>(nThreads ? nThreads : hardwareThreads(), 64));
+	std::vector<std::vector<Box>> pieces(cand.size());
+	std::atomic<size_t> next{0};
+	std::vector<std::thread> th;
+	for (unsigned t = 0; t < nt; ++t)
+		th.emplace_back([&]() {
+			for (;;) {
+				const size_t c0 = next.fetch_add(256);
+				if (c0 >= cand.size()) break;
+				const size_t c1 = std::min(cand.size(), c0 + 256);
+				for (size_t c = c0; c < c1; ++c) {
+					float p[3][3];
+					tris.get(cand[c], p);
+					cutTriangle(p, boxes[cand[c]], target, 64, pieces[c]);
+				}
+			}
+		});
+	for (auto &x : th) x.join();
+	*refBoxes = boxes;
+	refPrim->resize(n);
+	for (size_t i = 0; i < n; ++i) (*refPrim)[i] = (uint32_t)i;
+	for (size_t c = 0; c < cand.size(); ++c) {
+		const std::vector<Box> &pc = pieces[c];
+		if (pc.empty()) continue;
+		(*refBoxes)[cand[c]] = pc[0];
+		for (size_t j = 1; j < pc.size(); ++j) {
+			refBoxes->push_back(pc[j]);
+			refPrim->push_back(cand[c]);
+		}
+	}
+}
+
+// ---------------------------------------------------------------------------
 // 4-wide binned-SAH builder
 // ---------------------------------------------------------------------------
 namespace {

@@ -60,7 +60,23 @@ struct BuildParams {
 	unsigned nThreads = 0;
 	float travCost = 1.0f;
 	unsigned layout = 0;  // 0 quantized 64 B, 1 fp32 128 B
+	float splitBudget = 0.f;  // extra triangle references of early split clipping, per triangle; 0 = none
 };
+
+// Early split clipping (Ernst & Greiner, "Early Split Clipping for Bounding
+// Volume Hierarchies", 2007): a triangle whose box is larger than the typical
+// one is cut, by halving its box along the longest axis and clipping
+// the triangle to each half, into several references with tighter boxes.
+// Each reference box holds the triangle's part inside it (expanded by
+// MachineEpsilon::E), so the union still covers the triangle.  boxes: one
+// per triangle on input; refBoxes / refPrim: the references and their
+// triangle, or both empty when nothing is cut.
+struct TriangleSource {
+	virtual ~TriangleSource() {}
+	virtual void get(size_t prim, float p[3][3]) const = 0;
+};
+void splitLargeTriangles(const std::vector<Box> &boxes, const TriangleSource &tris, float budget, unsigned nThreads,
+		std::vector<Box> *refBoxes, std::vector<uint32_t> *refPrim);
 
 struct WideBVH {
 	std::vector<lrh_node4q> qnodes;

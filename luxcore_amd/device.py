@@ -44,7 +44,7 @@ class LrhObject(ctypes.Structure):
 
 class LrhBuildOpts(ctypes.Structure):
     _fields_ = [("layout", ctypes.c_uint), ("maxLeafSize", ctypes.c_uint), ("nThreads", ctypes.c_uint),
-                ("travCost", ctypes.c_float), ("builder", ctypes.c_uint)]
+                ("travCost", ctypes.c_float), ("builder", ctypes.c_uint), ("splitBudget", ctypes.c_float)]
 
 
 BUILD_HOST_SAH, BUILD_DEVICE_LBVH = 0, 1
@@ -280,13 +280,14 @@ def _object_structs(scene):
     return arr, keep
 
 
-def _build_opts(layout=0, max_leaf=4, threads=0, trav_cost=1.0, builder=BUILD_HOST_SAH):
+def _build_opts(layout=0, max_leaf=4, threads=0, trav_cost=1.0, builder=BUILD_HOST_SAH, split_budget=0.0):
     o = LrhBuildOpts()
     o.builder = builder
     o.layout = layout
     o.maxLeafSize = max_leaf
     o.nThreads = threads
     o.travCost = trav_cost
+    o.splitBudget = split_budget
     return o
 
 
@@ -350,19 +351,19 @@ class Scene:
         self.h = None
 
 
-def host_scene_bvh(meshes, layout=0, max_leaf=4, threads=0, trav_cost=1.0):
+def host_scene_bvh(meshes, layout=0, max_leaf=4, threads=0, trav_cost=1.0, split_budget=0.0):
     """Build a scene on the host only (no GPU): for inspecting the builder."""
     arr, keep = _mesh_structs(meshes)
-    opts = _build_opts(layout, max_leaf, threads, trav_cost)
+    opts = _build_opts(layout, max_leaf, threads, trav_cost, split_budget=split_budget)
     p = ctypes.c_void_p()
     check(lib().lrh_scene_create_bvh(None, arr, len(meshes), ctypes.byref(opts), ctypes.byref(p)))
     return Scene(None, p.value, False)
 
 
-def host_scene_mbvh(inst_scene, layout=0, max_leaf=4, threads=0, trav_cost=1.0):
+def host_scene_mbvh(inst_scene, layout=0, max_leaf=4, threads=0, trav_cost=1.0, split_budget=0.0):
     marr, keep1 = _mesh_structs(inst_scene.meshes)
     oarr, keep2 = _object_structs(inst_scene)
-    opts = _build_opts(layout, max_leaf, threads, trav_cost)
+    opts = _build_opts(layout, max_leaf, threads, trav_cost, split_budget=split_budget)
     p = ctypes.c_void_p()
     check(lib().lrh_scene_create_mbvh(None, marr, len(inst_scene.meshes), oarr, len(inst_scene.obj_mesh),
                                       ctypes.byref(opts), ctypes.byref(p)))
@@ -483,18 +484,20 @@ class Device:
         return lib().lrh_used_memory(self.h)
 
     # -- scenes --
-    def scene_bvh(self, meshes, layout=0, max_leaf=4, threads=0, trav_cost=1.0, builder=BUILD_HOST_SAH):
-        """lrh_scene_create_bvh; builder=BUILD_DEVICE_LBVH builds the tree on the GPU."""
+    def scene_bvh(self, meshes, layout=0, max_leaf=4, threads=0, trav_cost=1.0, builder=BUILD_HOST_SAH,
+                  split_budget=0.0):
+        """lrh_scene_create_bvh; builder=BUILD_DEVICE_LBVH builds the tree on the GPU; split_budget:
+        extra references of early split clipping per triangle (0 = default, negative = off)."""
         arr, keep = _mesh_structs(meshes)
-        opts = _build_opts(layout, max_leaf, threads, trav_cost, builder)
+        opts = _build_opts(layout, max_leaf, threads, trav_cost, builder, split_budget)
         p = ctypes.c_void_p()
         check(lib().lrh_scene_create_bvh(self.h, arr, len(meshes), ctypes.byref(opts), ctypes.byref(p)))
         return Scene(self, p.value, False)
 
-    def scene_mbvh(self, inst_scene, layout=0, max_leaf=4, threads=0, trav_cost=1.0):
+    def scene_mbvh(self, inst_scene, layout=0, max_leaf=4, threads=0, trav_cost=1.0, split_budget=0.0):
         marr, keep1 = _mesh_structs(inst_scene.meshes)
         oarr, keep2 = _object_structs(inst_scene)
-        opts = _build_opts(layout, max_leaf, threads, trav_cost)
+        opts = _build_opts(layout, max_leaf, threads, trav_cost, split_budget=split_budget)
         p = ctypes.c_void_p()
         check(lib().lrh_scene_create_mbvh(self.h, marr, len(inst_scene.meshes), oarr, len(inst_scene.obj_mesh),
                                           ctypes.byref(opts), ctypes.byref(p)))

@@ -93,8 +93,9 @@ def check_tree(nodes, boxes_of_child, root, n_nodes, n_prims, layout):
 
 @pytest.mark.parametrize("layout,max_leaf", [(0, 4), (0, 1), (0, 8), (1, 4), (1, 2)])
 def test_host_builder_records(layout, max_leaf):
+    # one record per triangle (early split clipping off), boxes of whole triangles
     meshes = scenes.plain_scene(seed=1, n_random=2000)
-    sc = D.host_scene_bvh(meshes, layout=layout, max_leaf=max_leaf, threads=4)
+    sc = D.host_scene_bvh(meshes, layout=layout, max_leaf=max_leaf, threads=4, split_budget=-1.0)
     info = sc.info()
     nb, pb = sc.download()
     nodes = nb.view(NODE_Q if layout == 0 else NODE_F)
@@ -123,6 +124,83 @@ def test_host_builder_records(layout, max_leaf):
         return lo[s:s + c].min(0), hi[s:s + c].max(0)
     check_tree(nodes, boxes_of_child, 0, nodes.shape[0], tris.shape[0], layout)
     assert info["maxDepth"] < 40
+
+
+def _grid(k):
+    x, y = np.meshgrid(np.arange(k + 1, dtype=np.float32), np.arange(k + 1, dtype=np.float32))
+    v = np.stack([x.ravel(), y.ravel(), np.zeros(x.size, np.float32)], 1)
+    i = np.arange(k * (k + 1)).reshape(k, k + 1)[:, :k].ravel()
+    t = np.concatenate([np.stack([i, i + 1, i + k + 2], 1), np.stack([i, i + k + 2, i + k + 1], 1)])
+    return v, t.astype(np.uint32)
+
+
+def leaf_boxes_of_records(nodes, root, n_prims, layout):
+    """(lo, hi) of the leaf child slot that holds each record."""
+    dec = decode_q(nodes) if layout == 0 else {k: nodes[k] for k in ("lo_x", "hi_x", "lo_y", "hi_y", "lo_z", "hi_z")}
+    lo = np.full((n_prims, 3), np.nan, np.float32)
+    hi = np.full((n_prims, 3), np.nan, np.float32)
+    stack = [root]
+    while stack:
+        i = stack.pop()
+        for c in range(4):
+            ref = int(nodes["child"][i][c])
+            if ref == 0xFFFFFFFF:
+                continue
+            if ref & 0x80000000:
+                start, cnt = ref & 0x0FFFFFFF, ((ref >> 28) & 7) + 1
+                lo[start:start + cnt] = [dec["lo_x"][i][c], dec["lo_y"][i][c], dec["lo_z"][i][c]]
+                hi[start:start + cnt] = [dec["hi_x"][i][c], dec["hi_y"][i][c], dec["hi_z"][i][c]]
+            else:
+                stack.append(ref)
+    return lo, hi
+
+
+@pytest.mark.parametrize("layout,budget", [(0, 0.0), (0, 4.0), (1, 1.0)])
+def test_split_triangles_cover_their_triangles(layout, budget):
+    """Early split clipping: triangles whose box is far above the median are
+    cut into references with tighter boxes.  Every triangle keeps at least
+    one record with its exact data, and every point of every triangle lies
+    in the leaf box of one of its records (the culling stays conservative);
+    triangles of typical size are not cut."""
+    rng = np.random.default_rng(5)
+    meshes = scenes.plain_scene(seed=2, n_random=3000)
+    n_tris = sum(m.ntris for m in meshes)
+    sc = D.host_scene_bvh(meshes, layout=layout, threads=4, split_budget=budget)
+    nb, pb = sc.download()
+    nodes = nb.view(NODE_Q if layout == 0 else NODE_F)
+    tris = pb.view(TRI)
+    assert n_tris < tris.shape[0] <= n_tris * (1 + (budget or 4.0)) + 64
+    lo, hi = leaf_boxes_of_records(nodes, 0, tris.shape[0], layout)
+    assert not np.isnan(lo).any()
+    gid = np.zeros(tris.shape[0], np.int64)
+    base = np.cumsum([0] + [m.ntris for m in meshes])
+    gid[:] = base[tris["mesh"]] + tris["tri"]
+    counts = np.bincount(gid, minlength=n_tris)
+    assert np.all(counts >= 1)
+    # records carry the triangle's own data
+    for i in rng.choice(tris.shape[0], 400, replace=False):
+        v = meshes[tris["mesh"][i]].verts[meshes[tris["mesh"][i]].tris[tris["tri"][i]]]
+        assert np.array_equal(tris["p0"][i], v[0]) and np.array_equal(tris["e1"][i], (v[1] - v[0]).astype(np.float32))
+    # coverage: points of every cut triangle (vertices, edge midpoints, random interior points)
+    cut = np.nonzero(counts > 1)[0]
+    assert cut.size > 100
+    order = np.argsort(gid, kind="stable")
+    starts = np.searchsorted(gid[order], np.arange(n_tris))
+    w = rng.dirichlet((1.0, 1.0, 1.0), size=60)
+    w = np.vstack([np.eye(3), [[0.5, 0.5, 0], [0, 0.5, 0.5], [0.5, 0, 0.5]], w])
+    for g in cut:
+        mi = int(np.searchsorted(base, g, side="right") - 1)
+        v = meshes[mi].verts[meshes[mi].tris[g - base[mi]]].astype(np.float64)
+        pts = w @ v
+        recs = order[starts[g]:starts[g] + counts[g]]
+        inside = np.zeros(pts.shape[0], bool)
+        for r in recs:
+            inside |= np.all((pts >= lo[r]) & (pts <= hi[r]), axis=1)
+        assert inside.all(), (g, pts[~inside][:3])
+    # triangles of one size (a flat grid: every box area equal) are not cut
+    g = scenes.Mesh(*_grid(32))
+    h = D.host_scene_bvh([g], threads=4)
+    assert h.info()["primCount"] == g.ntris
 
 
 def test_host_builder_two_level():
