@@ -105,7 +105,8 @@ typedef struct {
 	float travCost;            /* SAH traversal cost relative to one triangle test, default 1.0 */
 	unsigned int builder;      /* LRH_BUILD_HOST_SAH or LRH_BUILD_DEVICE_LBVH */
 	float splitBudget;         /* early split clipping: triangles whose box area is above twice the
-	                              median are cut into up to 64 references with tighter boxes, at most
+	                              median and above 2.5x the triangle's two-sided area (a poor fit) are
+	                              cut into up to 64 references with tighter boxes, at most
 	                              this many extra references per triangle over the scene (the size
 	                              threshold rises until they fit); 0 = default (4.0), negative = off.
 	                              Results are unchanged (a cut triangle is tested through any of its

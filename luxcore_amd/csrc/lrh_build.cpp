@@ -527,6 +527,34 @@ void splitLargeTriangles(const std::vector<Box> &boxes, const TriangleSource &tr
 	if (!(median > 0.f)) return;
 	const double allowed = (double)budget * (double)n;
 	float factor = 2.f;
+	// only boxes that fit their triangle poorly are worth cutting: box area
+	// above 2.5x the triangle's two-sided area (a triangle lying in an axis
+	// plane has exactly 2x: its box is a flat slab already, and cutting such
+	// triangles was measured to cost up to 16%; triangles of random
+	// orientation have 4.5x at the median)
+	const unsigned ntFit = std::max(1u, std::min<unsigned>(nThreads ? nThreads : hardwareThreads(), 64));
+	std::vector<uint8_t> loose(n, 0);
+	{
+		std::vector<std::thread> th;
+		for (unsigned t = 0; t < ntFit; ++t)
+			th.emplace_back([&, t]() {
+				for (size_t i = t; i < n; i += ntFit) {
+					if (!(area[i] > factor * median)) continue;
+					float p[3][3];
+					tris.get(i, p);
+					double e1[3], e2[3];
+					for (int k = 0; k < 3; ++k) {
+						e1[k] = (double)p[1][k] - p[0][k];
+						e2[k] = (double)p[2][k] - p[0][k];
+					}
+					const double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+							cz = e1[0] * e2[1] - e1[1] * e2[0];
+					const double twoSided = std::sqrt(cx * cx + cy * cy + cz * cz);  // 2 x triangle area
+					loose[i] = (double)area[i] > 2.5 * twoSided ? 1 : 0;
+				}
+			});
+		for (auto &x : th) x.join();
+	}
 	std::vector<uint32_t> cand;
 	for (int round = 0; round < 24; ++round, factor *= 2.f) {
 		// pieces needed: about area / target per candidate (halving a box
@@ -535,7 +563,7 @@ void splitLargeTriangles(const std::vector<Box> &boxes, const TriangleSource &tr
 		double extra = 0.0;
 		cand.clear();
 		for (size_t i = 0; i < n; ++i)
-			if (area[i] > target) {
+			if (loose[i] && area[i] > target) {
 				cand.push_back((uint32_t)i);
 				extra += std::min(63.0, (double)area[i] / target);
 			}
