@@ -273,7 +273,15 @@ uint32_t buildMeshTree(const lrh_mesh *meshes, const std::vector<unsigned> &mesh
 	src.off = &off;
 	std::vector<Box> refBoxes;
 	std::vector<uint32_t> refPrim;
-	lrh::splitLargeTriangles(boxes, src, p.splitBudget, nt, &refBoxes, &refPrim);
+	// the extra references must fit the 28-bit record offsets of the leaves
+	const double room = (double)LRH_MAX_LEAF_START - (double)tris.size() - (double)n;
+	const float budget = (float)std::min((double)p.splitBudget, room / (double)n);
+	lrh::splitLargeTriangles(boxes, src, budget, nt, &refBoxes, &refPrim);
+	if (!refPrim.empty() && (double)tris.size() + (double)refPrim.size() > (double)LRH_MAX_LEAF_START) {
+		// the pieces came out above the estimate and past the record limit: keep the triangles whole
+		refBoxes.clear();
+		refPrim.clear();
+	}
 	const bool cut = !refPrim.empty();
 	const size_t nRefs = cut ? refPrim.size() : n;
 	if (tris.size() + nRefs > LRH_MAX_LEAF_START)
