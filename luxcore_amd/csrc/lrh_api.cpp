@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <atomic>
+#include <exception>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -343,6 +344,10 @@ struct ObjectData {
 };
 
 void prepareObjects(lrh_scene *s, const lrh_object *objs, unsigned n, ObjectData &od, bool motionBounds = true) {
+	// motion bounds (1,025 samples per segment, each with a matrix inverse)
+	// are computed after the loop, in parallel over the objects
+	struct Deferred { size_t slot; lrh::MotionSystemHost ms; Box local; };
+	std::vector<Deferred> deferred;
 	for (unsigned o = 0; o < n; ++o) {
 		const lrh_object &ob = objs[o];
 		if (ob.kind != s->objKind[o] || ob.mesh != s->objMesh[o])
@@ -377,12 +382,11 @@ void prepareObjects(lrh_scene *s, const lrh_object *objs, unsigned n, ObjectData
 			od.segs.insert(od.segs.end(), ms.segs.begin(), ms.segs.end());
 			in.index = (uint32_t)od.mot.size();
 			od.mot.push_back(md);
-			if (!motionBounds) {  // the device refit computes it
-				od.boxes.push_back(Box::Empty());
-				od.inst.push_back(in);
-				continue;
-			}
-			wb = ms.bound(wb);  // MotionTriangleMesh::GetBBox
+			if (motionBounds) deferred.push_back(Deferred{od.boxes.size(), std::move(ms), wb});
+			// filled below (or, without motionBounds, by the device refit)
+			od.boxes.push_back(Box::Empty());
+			od.inst.push_back(in);
+			continue;
 		} else if (ob.kind != 0)
 			throw Error(LRH_EINVAL, "object kind must be 0, 1 or 2");
 		const float e = lrh::machineEpsilon(wb);
@@ -390,6 +394,31 @@ void prepareObjects(lrh_scene *s, const lrh_object *objs, unsigned n, ObjectData
 		od.boxes.push_back(wb);
 		od.inst.push_back(in);
 	}
+	if (deferred.empty()) return;
+	const unsigned nt = std::max(1u, std::min<unsigned>(s->params.nThreads ? s->params.nThreads : lrh::hardwareThreads(),
+			(unsigned)deferred.size()));
+	std::atomic<size_t> next{0};
+	std::exception_ptr failed;
+	std::mutex failMu;
+	std::vector<std::thread> th;
+	for (unsigned t = 0; t < nt; ++t)
+		th.emplace_back([&]() {
+			try {
+				for (size_t i; (i = next.fetch_add(1)) < deferred.size();) {
+					Box wb = deferred[i].ms.bound(deferred[i].local);  // MotionTriangleMesh::GetBBox
+					const float e = lrh::machineEpsilon(wb);
+					for (int k = 0; k < 3; ++k) { wb.lo[k] -= e; wb.hi[k] += e; }
+					od.boxes[deferred[i].slot] = wb;
+				}
+			} catch (...) {
+				// a singular motion sample: stop the others and report it after the join
+				next.store(deferred.size());
+				std::lock_guard<std::mutex> g(failMu);
+				if (!failed) failed = std::current_exception();
+			}
+		});
+	for (auto &x : th) x.join();
+	if (failed) std::rethrow_exception(failed);
 }
 
 // root tree over the objects, written into the reserved node slots [0, rootReserve)
