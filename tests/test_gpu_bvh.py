@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 import oracle
+from luxcore_amd import device as D
 from luxcore_amd import scenes
 from luxcore_amd.raytypes import HIT_DTYPE, NULL_INDEX, RAY_DTYPE, RAY_FLAGS_MASKED, make_rays
 from tests.parity import assert_parity, compare, tri_tie_checker
@@ -139,6 +140,29 @@ def test_heightfield_bounce_parity(gpu_device):
     ref_s = ob.intersect(shadow)
     got_s = gpu_device.trace_numpy(sc, shadow, query=1)
     assert np.array_equal(ref_s["meshIndex"] == NULL_INDEX, got_s["meshIndex"] == NULL_INDEX)
+
+
+def test_stack_depths_give_identical_hits(gpu_device):
+    """The per-lane stack keeps its top ldsStack entries in an LDS ring and
+    spills the rest: every depth and launch shape must give the default's
+    hits bit for bit (deep traversals spill)."""
+    _case("stack_depths")
+    meshes = scenes.heightfield_scene(cells=256, n_spheres=8, seed=3)
+    prim = scenes.pinhole_rays(320, 180)
+    sc = gpu_device.scene_bvh(meshes)
+    h = gpu_device.trace_numpy(sc, prim)
+    rays = np.concatenate([scenes.bounce_rays(meshes, prim, h), scenes.plain_rays(20000, seed=9)])
+    base = gpu_device.trace_numpy(sc, rays)
+    for block, lds, refill in ((256, 4, 8), (256, 16, 8), (128, 32, 16), (64, 64, 4), (256, 4, 1)):
+        o = D.trace_opts(block, lds, 0, refill)
+        for q in (0, 1):
+            got = gpu_device.trace_numpy(sc, rays, query=q, opts=o)
+            ref = base if q == 0 else gpu_device.trace_numpy(sc, rays, query=1)
+            if q == 0:
+                assert np.array_equal(got, ref), (block, lds, refill)
+            else:
+                assert np.array_equal(got["meshIndex"] == NULL_INDEX, ref["meshIndex"] == NULL_INDEX)
+    sc.close()
 
 
 def test_full_size_plain_properties(gpu_device):
