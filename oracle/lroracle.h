@@ -21,7 +21,14 @@
  *   orc_mbvh_*              identity instance == plain BVH; static instance
  *                           == pre-transformed copy; brute force per instance
  *   orc_motion_sample       closed forms (endpoints, translation lerp,
- *                           axis rotation angle lerp, TRS round trip)
+ *                           axis rotation angle lerp, TRS round trip); its
+ *                           slerp sines are the C library's sin in double,
+ *                           which the device's own sine matches on every
+ *                           float argument in [0, pi] (exhaustive test)
+ *   orc_mbvh_* with projective instance matrices (last row not 0,0,0,1):
+ *                           parity unpinned — MBVHAccel's result then depends
+ *                           on its root-tree topology (root leaves are not
+ *                           box-tested, only their parents)
  */
 #ifndef LRORACLE_H
 #define LRORACLE_H
