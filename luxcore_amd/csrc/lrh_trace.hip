@@ -114,8 +114,12 @@ __device__ __forceinline__ void motionSample(const lrh_motion *mot, const lrh_mo
 }
 
 // ---- node tests: up to 4 children -> (entry distance, child ref) ----
-constexpr float kLo = 1.0f - 1.0f / 1048576.0f;  // conservative widening of the slab interval
-constexpr float kHi = 1.0f + 1.0f / 1048576.0f;
+// conservative widening of the slab interval by 2^-20 of |t| on each side
+// (entry t - |t| eps, exit t + |t| eps: wider for either sign of t, which a
+// plain scale by 1 -/+ eps is not for negative t, i.e. negative mint)
+constexpr float kWiden = 1.0f / 1048576.0f;
+DEVI float widenLo(float t) { return fmaf(-fabsf(t), kWiden, t); }
+DEVI float widenHi(float t) { return fmaf(fabsf(t), kWiden, t); }
 
 DEVI void testNodeQ(const lrh_node4q *__restrict__ nodes, uint32_t idx, const LocalRay &r, float mint, float best,
 		float t[4], uint32_t c[4]) {
@@ -134,8 +138,8 @@ DEVI void testNodeQ(const lrh_node4q *__restrict__ nodes, uint32_t idx, const Lo
 		const float tnx = fmaf(byteF(nX, k), Ax, Bx), tfx = fmaf(byteF(fX, k), Ax, Bx);
 		const float tny = fmaf(byteF(nY, k), Ay, By), tfy = fmaf(byteF(fY, k), Ay, By);
 		const float tnz = fmaf(byteF(nZ, k), Az, Bz), tfz = fmaf(byteF(fZ, k), Az, Bz);
-		const float tn = fmaxf(fmaxf(tnx, tny), tnz) * kLo;
-		const float tf = fminf(fminf(tfx, tfy), tfz) * kHi;
+		const float tn = widenLo(fmaxf(fmaxf(tnx, tny), tnz));
+		const float tf = widenHi(fminf(fminf(tfx, tfy), tfz));
 		const float a = fmaxf(tn, mint), b = fminf(tf, best);
 		// empty slots hold inverted boxes; one that still passes (a far node
 		// whose inverted extent is below the widening) is dropped when its
@@ -159,8 +163,8 @@ DEVI void testNodeF(const lrh_node4f *__restrict__ nodes, uint32_t idx, const Lo
 		const float tnx = fmaf(sx ? LX[k] : HX[k], r.ix, r.nx), tfx = fmaf(sx ? HX[k] : LX[k], r.ix, r.nx);
 		const float tny = fmaf(sy ? LY[k] : HY[k], r.iy, r.ny), tfy = fmaf(sy ? HY[k] : LY[k], r.iy, r.ny);
 		const float tnz = fmaf(sz ? LZ[k] : HZ[k], r.iz, r.nz), tfz = fmaf(sz ? HZ[k] : LZ[k], r.iz, r.nz);
-		const float tn = fmaxf(fmaxf(tnx, tny), tnz) * kLo;
-		const float tf = fminf(fminf(tfx, tfy), tfz) * kHi;
+		const float tn = widenLo(fmaxf(fmaxf(tnx, tny), tnz));
+		const float tf = widenHi(fminf(fminf(tfx, tfy), tfz));
 		const float a = fmaxf(tn, mint), b = fminf(tf, best);
 		t[k] = (a <= b) ? a : INFINITY;  // empty slots: lo = +inf, hi = -inf
 	}
