@@ -532,12 +532,11 @@ void splitLargeTriangles(const std::vector<Box> &boxes, const TriangleSource &tr
 	// plane has exactly 2x: its box is a flat slab already, and cutting such
 	// triangles was measured to cost up to 16%; triangles of random
 	// orientation have 4.5x at the median)
-	const unsigned ntFit = std::max(1u, std::min<unsigned>(nThreads ? nThreads : hardwareThreads(), 64));
+	// threads only for large meshes: scenes may hold thousands of small ones
+	const unsigned ntFit = n < 65536 ? 1u : std::max(1u, std::min<unsigned>(nThreads ? nThreads : hardwareThreads(), 64));
 	std::vector<uint8_t> loose(n, 0);
 	{
-		std::vector<std::thread> th;
-		for (unsigned t = 0; t < ntFit; ++t)
-			th.emplace_back([&, t]() {
+		auto fit = [&](unsigned t) {
 				for (size_t i = t; i < n; i += ntFit) {
 					if (!(area[i] > factor * median)) continue;
 					float p[3][3];
@@ -552,8 +551,14 @@ void splitLargeTriangles(const std::vector<Box> &boxes, const TriangleSource &tr
 					const double twoSided = std::sqrt(cx * cx + cy * cy + cz * cz);  // 2 x triangle area
 					loose[i] = (double)area[i] > 2.5 * twoSided ? 1 : 0;
 				}
-			});
-		for (auto &x : th) x.join();
+			};
+		if (ntFit == 1) {
+			fit(0);
+		} else {
+			std::vector<std::thread> th;
+			for (unsigned t = 0; t < ntFit; ++t) th.emplace_back(fit, t);
+			for (auto &x : th) x.join();
+		}
 	}
 	std::vector<uint32_t> cand;
 	for (int round = 0; round < 24; ++round, factor *= 2.f) {
@@ -573,24 +578,29 @@ void splitLargeTriangles(const std::vector<Box> &boxes, const TriangleSource &tr
 	const float target = factor * median;
 	// cut the candidates in parallel, in chunks, then place the references:
 	// every triangle keeps its slot, its extra pieces follow all triangles
-	const unsigned nt = std::max(1u, std::min<unsigned>(nThreads ? nThreads : hardwareThreads(), 64));
+	const unsigned nt = cand.size() < 1024 ? 1u
+			: std::max(1u, std::min<unsigned>(nThreads ? nThreads : hardwareThreads(), 64));
 	std::vector<std::vector<Box>> pieces(cand.size());
 	std::atomic<size_t> next{0};
-	std::vector<std::thread> th;
-	for (unsigned t = 0; t < nt; ++t)
-		th.emplace_back([&]() {
-			for (;;) {
-				const size_t c0 = next.fetch_add(256);
-				if (c0 >= cand.size()) break;
-				const size_t c1 = std::min(cand.size(), c0 + 256);
-				for (size_t c = c0; c < c1; ++c) {
-					float p[3][3];
-					tris.get(cand[c], p);
-					cutTriangle(p, boxes[cand[c]], target, 64, pieces[c]);
-				}
+	auto cut = [&]() {
+		for (;;) {
+			const size_t c0 = next.fetch_add(256);
+			if (c0 >= cand.size()) break;
+			const size_t c1 = std::min(cand.size(), c0 + 256);
+			for (size_t c = c0; c < c1; ++c) {
+				float p[3][3];
+				tris.get(cand[c], p);
+				cutTriangle(p, boxes[cand[c]], target, 64, pieces[c]);
 			}
-		});
-	for (auto &x : th) x.join();
+		}
+	};
+	if (nt == 1) {
+		cut();
+	} else {
+		std::vector<std::thread> th;
+		for (unsigned t = 0; t < nt; ++t) th.emplace_back(cut);
+		for (auto &x : th) x.join();
+	}
 	*refBoxes = boxes;
 	refPrim->resize(n);
 	for (size_t i = 0; i < n; ++i) (*refPrim)[i] = (uint32_t)i;
