@@ -254,6 +254,32 @@ def test_projective_instance_matrices(gpu_device):
         assert np.array_equal(ref[f][same], got[f][same])
 
 
+def test_split_triangles_in_instanced_mesh(gpu_device):
+    """Early split clipping inside the leaf tree of an instanced mesh whose
+    triangles span it (the base mesh is cut into many references): hits
+    equal the oracle's (which has no splitting) bit for bit, and equal the
+    same scene built without splitting."""
+    _case("split_instanced")
+    rng = np.random.default_rng(81)
+    small = scenes.geodesic_sphere(3)
+    big = scenes.random_triangles(60, (-2, -2, -2), (2, 2, 2), rng)
+    base = scenes.Mesh(np.concatenate([small.verts, big.verts]).astype(np.float32),
+                       np.concatenate([small.tris, big.tris + small.verts.shape[0]]).astype(np.uint32))
+    sc = scenes.instanced_scene(n_inst=40, motion_frac=0.25, freq=6, extent=8.0, seed=82)
+    sc = scenes.InstancedScene([base], sc.obj_mesh, sc.obj_kind, sc.obj_mat, sc.key_off, sc.key_times, sc.key_mats)
+    rays = scenes.instanced_rays(100000, extent=8.0, seed=83)
+    cut = gpu_device.scene_mbvh(sc)
+    whole = gpu_device.scene_mbvh(sc, split_budget=-1.0)
+    assert cut.info()["primCount"] > whole.info()["primCount"]
+    st = _check(oracle.OracleMBVH(sc), gpu_device, cut, rays)
+    assert st["hits"] > 5000 and st["mismatch"] == st["ties"] + st["edge_band"]
+    a = gpu_device.trace_numpy(cut, rays)
+    b = gpu_device.trace_numpy(whole, rays)
+    assert np.array_equal(a["meshIndex"], b["meshIndex"]) and np.array_equal(a["t"], b["t"])
+    cut.close()
+    whole.close()
+
+
 def test_empty_two_level(gpu_device):
     _case("empty_mbvh")
     sc = scenes.InstancedScene([scenes.geodesic_sphere(4)], np.zeros(0, np.uint32), np.zeros(0, np.uint32),
