@@ -471,7 +471,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--mode", choices=("headline", "batched", "served", "scene", "refit", "configs", "render"),
                     default="headline",
-                    help="headline: incoherent closest-hit on the 26.2M-tri scene (the driver's line); "
+                    help="headline: incoherent closest-hit on the 26.2M-tri scene (the default JSON line); "
                          "batched: 64 buffers over 8 resident scenes in one launch; served: continuous batching; "
                          "scene: a LuxCore scene file (--scene, default the bundled lightinstances scene); "
                          "refit: two-level edit latency, host rebuild (lrh_scene_update) vs GPU refit; "
