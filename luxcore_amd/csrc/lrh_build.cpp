@@ -275,10 +275,11 @@ void quatToMatrix(const float q[4], float m[4][4]) {  // quaternion.cpp:115-140
 void slerpSeg(float t, const lrh_motion_seg &g, float out[4]) {  // quaternion.cpp:143-164
 	float f1, f2;
 	if (g.slerpArc) {
-		// correctly rounded fp32 sin (double evaluation, one rounding), the
-		// same values the device computes; phi and sin(phi) come with the segment
-		f1 = (float)sin((double)((1.f - t) * g.phi)) / g.sinPhi;
-		f2 = (float)sin((double)(t * g.phi)) / g.sinPhi;
+		// the C library's sinf, as the reference calls it; the device's
+		// restatement (lrh_sinf.h) gives the same floats; phi and sin(phi)
+		// come with the segment
+		f1 = sinf((1.f - t) * g.phi) / g.sinPhi;
+		f2 = sinf(t * g.phi) / g.sinPhi;
 	} else {
 		f1 = 1.f - t;
 		f2 = t;
@@ -294,8 +295,9 @@ void slerpConstants(lrh_motion_seg &g) {
 	cos_phi *= g.qsign;
 	g.slerpArc = (1.f - cos_phi > 1e-6f) ? 1u : 0u;
 	if (g.slerpArc) {
-		g.phi = (float)acos((double)cos_phi);
-		g.sinPhi = (float)sin((double)g.phi);
+		// acosf and sinf of the C library, as quaternion.cpp:151-153 calls them
+		g.phi = acosf(cos_phi);
+		g.sinPhi = sinf(g.phi);
 	}
 }
 

@@ -2,10 +2,11 @@
  * InterpolatedTransform::Sample on the device
  * (src/luxrays/core/geometry/motionsystem.cpp:89-159), rounded as written:
  * the files that include it build with -ffp-contract=off and correctly
- * rounded division; slerp's sines are evaluated in double (lrh_sin.h) and
- * rounded once, and its time-independent constants (sign, phi = acos(cos_phi),
- * sin(phi)) come with the segment from the host, so host and device
- * produce the same bits.
+ * rounded division; slerp's two per-sample sines are the C library's sinf
+ * restated (lrh_sinf.h), and its time-independent constants (sign,
+ * phi = acosf(cos_phi), sinf(phi)) come with the segment from the host, which
+ * calls the C library as the reference does, so host and device produce the
+ * reference's bits.
  */
 #ifndef LRH_MOTION_DEV_H
 #define LRH_MOTION_DEV_H
@@ -14,7 +15,7 @@
 #include <stdint.h>
 
 #include "lrh_layout.h"
-#include "lrh_sin.h"
+#include "lrh_sinf.h"
 
 __device__ __forceinline__ float lrhLerp(float t, float a, float b) { return a + t * (b - a); }
 
@@ -77,11 +78,10 @@ __device__ __forceinline__ void lrhSegSample(const lrh_motion_seg &seg, float ti
 		// the segment holds the shorter-arc sign, phi and sin(phi)
 		float f1, f2;
 		if (g.slerpArc) {
-			// sin evaluated in double and rounded once to float: lrhSin0Pi
-			// gives the float the C library's sin gives for every float
-			// argument in [0, pi] (the slerp's angles lie in [0, pi/2])
-			f1 = (float)lrhSin0Pi((double)((1.f - le) * g.phi)) / g.sinPhi;
-			f2 = (float)lrhSin0Pi((double)(le * g.phi)) / g.sinPhi;
+			// sinf as the reference's C library computes it (lrh_sinf.h): the
+			// same float for every argument (the slerp's lie in [0, pi/2])
+			f1 = lrhSinf((1.f - le) * g.phi) / g.sinPhi;
+			f2 = lrhSinf(le * g.phi) / g.sinPhi;
 		} else {
 			f1 = 1.f - le;
 			f2 = le;

@@ -115,8 +115,9 @@ def epsilon_bbox(pmin, pmax):
     return lib().orc_epsilon_bbox(_p(a, _f32p), _p(b, _f32p))
 
 
-def set_libm(correctly_rounded=True):
-    """Slerp's acosf/sinf: correctly rounded (default) or the host libm's."""
+def set_libm(correctly_rounded=False):
+    """Slerp's acosf/sinf: the host libm's (the default, as the reference
+    calls them) or correctly rounded ones."""
     lib().orc_set_libm(1 if correctly_rounded else 0)
 
 

@@ -39,10 +39,11 @@ inline float Lerp(float t, float v1, float v2) { return v1 + t * (v2 - v1); }
 
 // The reference's slerp calls the platform's acosf/sinf
 // (src/luxrays/core/geometry/quaternion.cpp:151-155), whose last bit is
-// libm-specific (glibc's are not correctly rounded).  The oracle pins them to
-// correctly rounded results (computed in double, then rounded once) unless
-// orc_set_libm(0) selects the host libm.
-int g_crMath = 1;
+// libm-specific (glibc's are not correctly rounded).  The oracle calls the
+// host libm, as the reference does; orc_set_libm(1) switches to correctly
+// rounded results (computed in double, then rounded once) instead, for
+// checks that must not depend on the platform's library.
+int g_crMath = 0;
 inline float AcosF(float x) { return g_crMath ? (float)acos((double)x) : acosf(x); }
 inline float SinF(float x) { return g_crMath ? (float)sin((double)x) : sinf(x); }
 

@@ -99,9 +99,10 @@ void orc_bruteforce(unsigned nMeshes, const unsigned *vertOff, const unsigned *t
 		size_t n, int nThreads);
 
 /* ---- transforms and motion ---- */
-/* Slerp's acosf/sinf: 1 (default) = correctly rounded fp32 results, 0 = the
- * host libm's acosf/sinf.  Process-wide; set before building a two-level
- * oracle (motion bounds) and before tracing it. */
+/* Slerp's acosf/sinf: 0 (default) = the host libm's acosf/sinf, as the
+ * reference calls them; 1 = correctly rounded fp32 results.  Process-wide;
+ * set before building a two-level oracle (motion bounds) and before tracing
+ * it. */
 void orc_set_libm(int correctlyRounded);
 /* Gauss-Jordan inverse; returns 0 on a singular matrix. */
 int orc_matrix_inverse(const float m[16], float out[16]);

@@ -6,7 +6,7 @@ import pytest
 import oracle
 from luxcore_amd import scenes
 from luxcore_amd.raytypes import NULL_INDEX
-from tests.parity import assert_parity, compare
+from tests.parity import EDGE_BAND, _min_bary, assert_parity, compare
 
 pytestmark = pytest.mark.gpu
 
@@ -19,14 +19,13 @@ def _check(ob, dev, sc_dev, rays, query=0, rel_tol=1e-5, motion=()):
     ref = ob.intersect(rays)
     got = dev.trace_numpy(sc_dev, rays, query=query)
     if query == 1:
+        # any hit: occluded exactly where the oracle finds a hit, except in the edge band
         diff = np.nonzero((ref["meshIndex"] == NULL_INDEX) != (got["meshIndex"] == NULL_INDEX))[0]
-        # a flip is only allowed on motion objects within the motion band
         for i in diff:
             h = ref if ref["meshIndex"][i] != NULL_INDEX else got
-            assert int(h["meshIndex"][i]) in set(int(x) for x in motion), i
-        assert diff.size <= 1e-5 * rays.shape[0] + 5
+            assert _min_bary(h[i:i + 1])[0] < EDGE_BAND, i
         return None
-    st = compare(ref, got, motion_objects=motion, mint=rays["mint"])
+    st = compare(ref, got, motion_objects=motion)
     print(st, flush=True)
     assert_parity(st, rel_tol=rel_tol)
     return st
@@ -41,33 +40,43 @@ def full_instanced():
 
 def test_instanced_motion_full_size(gpu_device, full_instanced):
     """1,000 instances of a 50,000-tri displaced sphere, 200 with 2-key
-    motion, 4,194,304 incoherent rays with time ~ U[0,1].  Slerp's acos/sin
-    are correctly rounded on both sides, so motion hits are held to the same
-    bar as static ones."""
+    motion, 4,194,304 incoherent rays with time ~ U[0,1], against the oracle
+    whose slerp calls the host C library's acosf/sinf as the reference does
+    (quaternion.cpp:151-155): motion hits are held to the same bar as static
+    ones (edge band 1e-6, t/b1/b2 within 1e-5 relative)."""
     _case("instanced_full", rays=1 << 22)
     sc, rays = full_instanced
-    oracle.set_libm(True)
     ob = oracle.OracleMBVH(sc)
     dev_sc = gpu_device.scene_mbvh(sc)
-    st = _check(ob, gpu_device, dev_sc, rays)
-    assert st["hits"] > 100000
+    st = _check(ob, gpu_device, dev_sc, rays, motion=np.nonzero(sc.obj_kind == 2)[0])
+    assert st["hits"] > 100000 and st["motion_hits"] > 10000
     assert st["edge_band"] <= 1e-5 * rays.shape[0]
+    print(f"motion hits {st['motion_hits']}: worst |dt| {st['motion_worst_dt']}, worst |db| "
+          f"{st['motion_worst_db']}", flush=True)
     _check(ob, gpu_device, dev_sc, rays[: 1 << 20], query=1)
 
 
-def test_instanced_motion_against_host_libm(gpu_device, full_instanced):
-    """Same workload with the oracle's slerp on the host libm's acosf/sinf
-    (not correctly rounded): the differences stay inside the motion band."""
-    _case("instanced_full_host_libm", rays=1 << 22)
+def test_motion_bar_sees_a_one_ulp_sine(gpu_device, full_instanced):
+    """The same trace against the oracle switched to correctly rounded
+    acosf/sinf, whose last bit differs from the C library's on about 1% of
+    arguments: motion hits then differ from the oracle (the comparison is
+    fine enough to see a one-ulp slerp difference), static hits do not."""
+    _case("instanced_cr_oracle", rays=1 << 20)
     sc, rays = full_instanced
-    oracle.set_libm(False)
+    rays = rays[: 1 << 20]
+    oracle.set_libm(True)
     try:
         ob = oracle.OracleMBVH(sc)
-        dev_sc = gpu_device.scene_mbvh(sc)
-        st = _check(ob, gpu_device, dev_sc, rays, motion=np.nonzero(sc.obj_kind == 2)[0])
+        ref = ob.intersect(rays)
     finally:
-        oracle.set_libm(True)
-    assert st["motion_band"] + st["edge_band"] <= 2e-5 * rays.shape[0]
+        oracle.set_libm(False)
+    got = gpu_device.trace_numpy(gpu_device.scene_mbvh(sc), rays)
+    moving = np.nonzero(sc.obj_kind == 2)[0]
+    st = compare(ref, got, motion_objects=moving)
+    print(st, flush=True)
+    assert st["motion_worst_dt"] > 0 or st["motion_worst_db"] > 0
+    static = ~np.isin(ref["meshIndex"], moving) & ~np.isin(got["meshIndex"], moving)
+    assert_parity(compare(ref[static], got[static]))
 
 
 @pytest.mark.parametrize("layout", [0, 1])

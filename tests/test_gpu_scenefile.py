@@ -65,7 +65,7 @@ def _trace_both(dev, path, n_random=200000, width=480, height=270, seed=1):
         ob = oracle.OracleBVH(ref_sf.dataset_meshes())
     ref = ob.intersect(rays)
     motion = [i for i, o in enumerate(sf.objects) if o["kind"] == 2]
-    st = compare(ref, got, motion_objects=motion, mint=rays["mint"])
+    st = compare(ref, got, motion_objects=motion)
     print(os.path.basename(path), sf.info, st, flush=True)
     assert_parity(st)
     anyh = dev.trace_numpy(dsc, rays, query=1)

@@ -142,3 +142,61 @@ def test_motion_bound_contains_samples():
         g2l = oracle.motion_sample([0, 1], keys, t).astype(np.float64)
         pts = (np.linalg.inv(g2l) @ corners.T).T[:, :3]
         assert np.all(pts >= lo - 1e-3) and np.all(pts <= hi + 1e-3)
+
+
+# MotionSystem::Bound (src/luxrays/core/geometry/motionsystem.cpp:75-87,
+# 356-365) is the union over i = 0..1024 of Sample(Lerp(i/1024, start, end))
+# inverted and applied to the box.  Closed form: a point at radius 1000 from
+# the z axis, rotated about z through 170 degrees over [0, 1] (keys are
+# global->local, so they are the inverse rotations).  Its bound is the box of
+# the 1,025 points at angles -85 + 170 i/1024 degrees (plus the point's own
+# angle), computed here in float64.  Probe A puts the largest x exactly on
+# sample i = 513, which neither 1,023 nor 16 samples contain; probe B puts it
+# midway between samples 512 and 513, where a continuous (swept) bound would
+# reach the full radius.
+BOUND_R = 1000.0
+BOUND_TH = np.deg2rad(170.0)
+BOUND_TOL = 2.5e-4   # 4 ulps at 1000; the float evaluation stays within 1.2 ulps
+
+
+def _rotz(a):
+    c, s = np.cos(a), np.sin(a)
+    m = np.eye(4)
+    m[0, 0], m[0, 1], m[1, 0], m[1, 1] = c, -s, s, c
+    return m
+
+
+def bound_probe(tstar):
+    """(keys, point, expected box for N samples) of a probe whose largest x
+    is reached at time tstar."""
+    keys = np.stack([np.linalg.inv(_rotz(-BOUND_TH / 2)), np.linalg.inv(_rotz(BOUND_TH / 2))]).astype(np.float32)
+    alpha = BOUND_TH / 2 - BOUND_TH * tstar
+    p = np.array([BOUND_R * np.cos(alpha), BOUND_R * np.sin(alpha), 0.0], np.float32)
+
+    def expected(n=1024, swept=False):
+        ang = -BOUND_TH / 2 + BOUND_TH * np.arange(n + 1) / n + np.arctan2(float(p[1]), float(p[0]))
+        r = float(np.hypot(float(p[0]), float(p[1])))
+        x, y = r * np.cos(ang), r * np.sin(ang)
+        lo, hi = np.array([x.min(), y.min(), 0.0]), np.array([x.max(), y.max(), 0.0])
+        if swept:
+            hi[0] = r   # the arc passes angle 0
+        return lo, hi
+    return keys, p, expected
+
+
+BOUND_PROBES = {"A": 513 / 1024, "B": 512.5 / 1024}
+
+
+def _bound_err(lo, hi, e):
+    return max(np.abs(np.asarray(lo, np.float64) - e[0]).max(), np.abs(np.asarray(hi, np.float64) - e[1]).max())
+
+
+@pytest.mark.parametrize("probe", sorted(BOUND_PROBES))
+def test_motion_bound_closed_form(probe):
+    keys, p, expected = bound_probe(BOUND_PROBES[probe])
+    lo, hi = oracle.motion_bound([0, 1], keys, p, p)
+    assert _bound_err(lo, hi, expected(1024)) <= BOUND_TOL, (lo, hi, expected(1024))
+    # what a wrong sample count or a swept bound would give is outside the tolerance
+    wrong = {"A": [expected(1023), expected(16)], "B": [expected(1023), expected(swept=True)]}[probe]
+    for e in wrong:
+        assert _bound_err(lo, hi, e) > 2 * BOUND_TOL
