@@ -17,12 +17,116 @@ void CheckLrh(int rc) {
 }
 
 //------------------------------------------------------------------------------
-// DataSet (src/luxrays/core/dataset.cpp: Add)
+// Properties
 //------------------------------------------------------------------------------
 
-void DataSet::Add(const Mesh *mesh) {
+namespace {
+std::string trim(const std::string &s) {
+	const size_t a = s.find_first_not_of(" \t\r\n");
+	if (a == std::string::npos) return "";
+	const size_t b = s.find_last_not_of(" \t\r\n");
+	return s.substr(a, b - a + 1);
+}
+}  // namespace
+
+Properties &Properties::Set(const std::string &name, const std::string &value) {
+	if (!values.count(name)) names.push_back(name);
+	values[name] = value;
+	return *this;
+}
+
+Properties &Properties::SetFromString(const std::string &text) {
+	std::istringstream in(text);
+	std::string line;
+	while (std::getline(in, line)) {
+		const size_t hash = line.find('#');
+		if (hash != std::string::npos) line = line.substr(0, hash);
+		line = trim(line);
+		if (line.empty()) continue;
+		const size_t eq = line.find('=');
+		if (eq == std::string::npos) throw std::runtime_error("Syntax error in a Properties line: " + line);
+		std::string v = trim(line.substr(eq + 1));
+		if (v.size() >= 2 && v.front() == '"' && v.back() == '"') v = v.substr(1, v.size() - 2);
+		Set(trim(line.substr(0, eq)), v);
+	}
+	return *this;
+}
+
+std::string Properties::Get(const std::string &name, const std::string &def) const {
+	auto it = values.find(name);
+	return it == values.end() ? def : it->second;
+}
+
+int Properties::GetInt(const std::string &name, int def) const {
+	auto it = values.find(name);
+	if (it == values.end()) return def;
+	char *end = nullptr;
+	const long v = std::strtol(it->second.c_str(), &end, 10);
+	if (end == it->second.c_str() || *end) throw std::runtime_error("Property " + name + " is not an integer: " + it->second);
+	return (int)v;
+}
+
+float Properties::GetFloat(const std::string &name, float def) const {
+	auto it = values.find(name);
+	if (it == values.end()) return def;
+	char *end = nullptr;
+	const float v = std::strtof(it->second.c_str(), &end);
+	if (end == it->second.c_str() || *end) throw std::runtime_error("Property " + name + " is not a number: " + it->second);
+	return v;
+}
+
+bool Properties::GetBool(const std::string &name, bool def) const {
+	auto it = values.find(name);
+	if (it == values.end()) return def;
+	const std::string &v = it->second;
+	if (v == "1" || v == "true") return true;
+	if (v == "0" || v == "false") return false;
+	throw std::runtime_error("Property " + name + " is not a flag: " + v);
+}
+
+//------------------------------------------------------------------------------
+// Accelerator type names (src/luxrays/core/accelerator.cpp:25-55)
+//------------------------------------------------------------------------------
+
+std::string Accelerator::AcceleratorType2String(const AcceleratorType type) {
+	switch (type) {
+		case ACCEL_AUTO: return "AUTO";
+		case ACCEL_BVH: return "BVH";
+		case ACCEL_MBVH: return "MBVH";
+		case ACCEL_EMBREE: return "EMBREE";
+		case ACCEL_OPTIX: return "OPTIX";
+	}
+	throw std::runtime_error("Unknown accelerator type in AcceleratorType2String(): " + std::to_string((int)type));
+}
+
+AcceleratorType Accelerator::String2AcceleratorType(const std::string &type) {
+	if (type == "AUTO") return ACCEL_AUTO;
+	if (type == "BVH") return ACCEL_BVH;
+	if (type == "MBVH") return ACCEL_MBVH;
+	if (type == "EMBREE") return ACCEL_EMBREE;
+	if (type == "OPTIX") return ACCEL_OPTIX;
+	throw std::runtime_error("Unknown accelerator type in String2AcceleratorType(): " + type);
+}
+
+//------------------------------------------------------------------------------
+// DataSet (src/luxrays/core/dataset.cpp:39-176)
+//------------------------------------------------------------------------------
+
+DataSet::DataSet(const Context *luxRaysContext) : context(luxRaysContext) {
+	if (context) {
+		const Properties &cfg = context->GetConfig();
+		accelType = Accelerator::String2AcceleratorType(cfg.Get("accelerator.type", "AUTO"));
+		enableInstanceSupport = cfg.GetBool("accelerator.instances.enable", true);
+		enableMotionBlurSupport = cfg.GetBool("accelerator.motionblur.enable", true);
+	}
+}
+
+DataSet::~DataSet() {}
+
+u_int DataSet::Add(const Mesh *mesh) {
 	if (!mesh) throw std::runtime_error("DataSet::Add: null mesh");
 	if (preprocessed) throw std::runtime_error("DataSet::Add: the data set has already been preprocessed");
+	const u_int id = (u_int)meshes.size();
 	meshes.push_back(mesh);
 	totalVertexCount += mesh->GetTotalVertexCount();
 	totalTriangleCount += mesh->GetTotalTriangleCount();
@@ -33,15 +137,47 @@ void DataSet::Add(const Mesh *mesh) {
 			break;
 		case TYPE_TRIANGLE_MOTION:
 		case TYPE_EXT_TRIANGLE_MOTION:
-			hasMotion = true;
+			hasMotionBlur = true;
 			break;
 		default:
 			break;
 	}
+	return id;
+}
+
+bool DataSet::HasAccelerator(const AcceleratorType type) const {
+	std::lock_guard<std::mutex> g(accelsMutex);
+	return accels.count(type) != 0;
+}
+
+const Accelerator *DataSet::GetAccelerator(const AcceleratorType type) {
+	std::lock_guard<std::mutex> g(accelsMutex);
+	auto it = accels.find(type);
+	if (it != accels.end()) return it->second.get();
+	if (type != ACCEL_BVH && type != ACCEL_MBVH)
+		throw std::runtime_error("The HIP intersection device supports the BVH and MBVH accelerators, not " +
+				Accelerator::AcceleratorType2String(type));
+	std::unique_ptr<HIPAccelerator> a(new HIPAccelerator(context, type));
+	a->Init(meshes, totalVertexCount, totalTriangleCount);
+	const Accelerator *r = a.get();
+	accels[type] = std::move(a);
+	return r;
+}
+
+bool DataSet::DoesAllAcceleratorsSupportUpdate() const {
+	std::lock_guard<std::mutex> g(accelsMutex);
+	for (const auto &a : accels)
+		if (!a.second->DoesSupportUpdate()) return false;
+	return true;
+}
+
+void DataSet::UpdateAccelerators() {
+	std::lock_guard<std::mutex> g(accelsMutex);
+	for (auto &a : accels) a.second->Update();
 }
 
 //------------------------------------------------------------------------------
-// Accelerator: BVHAccel / MBVHAccel Init + the kernel upload
+// HIPAccelerator: BVHAccel / MBVHAccel Init + the per-device upload
 //------------------------------------------------------------------------------
 
 static const TriangleMesh *baseOf(const Mesh *m) {
@@ -68,11 +204,64 @@ static lrh_mesh toLrh(const TriangleMesh *m) {
 	return r;
 }
 
-Accelerator::~Accelerator() {
-	if (scene) lrh_scene_destroy(scene);
+HIPAccelerator::HIPAccelerator(const Context *c, AcceleratorType t) : ctx(c), type(t) {
+	if (type != ACCEL_BVH && type != ACCEL_MBVH) throw std::runtime_error("HIPAccelerator: BVH or MBVH only");
+	const Properties none;
+	const Properties &cfg = ctx ? ctx->GetConfig() : none;
+	// accelerator.bvh.* as BVHAccel::ToBVHParams reads them (bvhaccel.cpp:49-70):
+	// the tree type is validated (the device's nodes are 4-wide whatever it
+	// says: hits do not depend on it); the SAH costs apply to the CLASSIC
+	// builder, as in the reference, whose Embree builders ignore them
+	(void)cfg.GetInt("accelerator.bvh.treetype", 4);
+	(void)cfg.GetInt("accelerator.bvh.costsamples", 0);
+	const int isectCost = cfg.GetInt("accelerator.bvh.isectcost", 80);
+	const int travCost = cfg.GetInt("accelerator.bvh.travcost", 10);
+	(void)cfg.GetFloat("accelerator.bvh.emptybonus", 0.5f);
+	const std::string builder = cfg.Get("accelerator.bvh.builder.type", "EMBREE_BINNED_SAH");
+	buildOpts.builder = LRH_BUILD_HOST_SAH;
+	if (builder == "CLASSIC") {
+		if (isectCost <= 0 || travCost <= 0)
+			throw std::runtime_error("accelerator.bvh.isectcost and .travcost must be positive");
+		buildOpts.travCost = (float)travCost / (float)isectCost;
+	} else if (builder == "EMBREE_MORTON") {
+		// Morton-order build: the device LBVH (single-level scenes)
+		if (type == ACCEL_BVH) buildOpts.builder = LRH_BUILD_DEVICE_LBVH;
+	} else if (builder != "EMBREE_BINNED_SAH") {
+		throw std::runtime_error("Unknown BVH builder type in BVHAccel::Init(): " + builder);
+	}
+	const std::string layout = cfg.Get("hip.bvh.layout", "QUANTIZED");
+	if (layout == "QUANTIZED") buildOpts.layout = 0;
+	else if (layout == "FP32") buildOpts.layout = 1;
+	else throw std::runtime_error("hip.bvh.layout must be QUANTIZED or FP32: " + layout);
+	const int leaf = cfg.GetInt("hip.bvh.maxleafsize", 4);
+	if (leaf < 1 || leaf > 8) throw std::runtime_error("hip.bvh.maxleafsize must be 1..8");
+	buildOpts.maxLeafSize = (unsigned)leaf;
+	if (cfg.IsDefined("hip.bvh.travcost")) {
+		const float tc = cfg.GetFloat("hip.bvh.travcost", 1.f);
+		if (!(tc > 0.f)) throw std::runtime_error("hip.bvh.travcost must be positive");
+		buildOpts.travCost = tc;
+	}
+	if (cfg.IsDefined("hip.bvh.splitbudget")) {
+		const float sb = cfg.GetFloat("hip.bvh.splitbudget", 0.f);
+		buildOpts.splitBudget = sb == 0.f ? -1.f : sb;  // 0 means off here; the C ABI's 0 is its default
+	}
+	traceOpts.blockSize = (unsigned)cfg.GetInt("hip.kernel.blocksize", 0);
+	traceOpts.ldsStack = (unsigned)cfg.GetInt("hip.kernel.ldsstack", 0);
+	traceOpts.refillThreshold = (unsigned)cfg.GetInt("hip.kernel.refill", 0);
+	traceOpts.leafTris = (unsigned)cfg.GetInt("hip.kernel.leaftris", 0);
+	const std::string upd = cfg.Get("hip.accelerator.update", "REBUILD");
+	if (upd == "REBUILD") updateMode = UPDATE_REBUILD;
+	else if (upd == "REFIT") updateMode = UPDATE_REFIT;
+	else throw std::runtime_error("hip.accelerator.update must be REBUILD or REFIT: " + upd);
 }
 
-void Accelerator::fillObjects(std::vector<lrh_object> &objs, std::vector<float> &store) const {
+HIPAccelerator::~HIPAccelerator() {}
+
+bool HIPAccelerator::HasHWSupport(const IntersectionDevice &device) const {
+	return dynamic_cast<const HIPIntersectionDevice *>(&device) != nullptr;
+}
+
+void HIPAccelerator::fillObjects(std::vector<lrh_object> &objs, std::vector<float> &store) const {
 	// one pass to size the float store so pointers into it stay valid
 	size_t need = 0;
 	for (const Mesh *m : meshes) {
@@ -118,13 +307,11 @@ void Accelerator::fillObjects(std::vector<lrh_object> &objs, std::vector<float> 
 	}
 }
 
-void Accelerator::Init(const std::deque<const Mesh *> &ms, unsigned long long, unsigned long long) {
-	if (scene) throw std::runtime_error("Accelerator::Init called twice");
+void HIPAccelerator::Init(const std::deque<const Mesh *> &ms, const u_longlong, const u_longlong) {
+	if (initialized) throw std::runtime_error("Accelerator::Init called twice");
 	meshes = ms;
 	baseMeshes.clear();
 	objMesh.clear();
-	if (type == ACCEL_AUTO) throw std::runtime_error("Accelerator::Init: resolve ACCEL_AUTO first");
-	std::vector<lrh_mesh> lm;
 	if (type == ACCEL_BVH) {
 		// BVHAccel flattens the triangles of every mesh (bvhaccel.cpp:72-168):
 		// instances and motion meshes are not accepted
@@ -132,9 +319,7 @@ void Accelerator::Init(const std::deque<const Mesh *> &ms, unsigned long long, u
 			const MeshType t = m->GetType();
 			if (t != TYPE_TRIANGLE && t != TYPE_EXT_TRIANGLE)
 				throw std::runtime_error("BVHAccel supports only plain triangle meshes; use MBVH");
-			lm.push_back(toLrh(static_cast<const TriangleMesh *>(m)));
 		}
-		CheckLrh(lrh_scene_create_bvh(device.dev, lm.data(), (unsigned int)lm.size(), nullptr, &scene));
 	} else {
 		// MBVHAccel: one leaf tree per distinct base mesh (mbvhaccel.cpp:90-150)
 		for (const Mesh *m : meshes) {
@@ -144,65 +329,119 @@ void Accelerator::Init(const std::deque<const Mesh *> &ms, unsigned long long, u
 			if (k == baseMeshes.size()) baseMeshes.push_back(b);
 			objMesh.push_back((u_int)k);
 		}
+	}
+	initialized = true;
+}
+
+lrh_scene *HIPAccelerator::build(lrh_device *dev) const {
+	if (!initialized) throw std::runtime_error("Accelerator used before Init");
+	lrh_scene *scene = nullptr;
+	std::vector<lrh_mesh> lm;
+	if (type == ACCEL_BVH) {
+		for (const Mesh *m : meshes) lm.push_back(toLrh(static_cast<const TriangleMesh *>(m)));
+		CheckLrh(lrh_scene_create_bvh(dev, lm.data(), (unsigned int)lm.size(), &buildOpts, &scene));
+	} else {
 		for (const TriangleMesh *b : baseMeshes) lm.push_back(toLrh(b));
 		std::vector<lrh_object> objs;
 		std::vector<float> store;
 		fillObjects(objs, store);
-		CheckLrh(lrh_scene_create_mbvh(device.dev, lm.data(), (unsigned int)lm.size(), objs.data(),
-				(unsigned int)objs.size(), nullptr, &scene));
+		CheckLrh(lrh_scene_create_mbvh(dev, lm.data(), (unsigned int)lm.size(), objs.data(), (unsigned int)objs.size(),
+				&buildOpts, &scene));
 	}
+	return scene;
 }
 
-void Accelerator::Update() {
+void HIPAccelerator::Update() {
 	if (!DoesSupportUpdate()) throw std::runtime_error("BVHAccel does not support Update()");
-	if (!scene) throw std::runtime_error("Accelerator::Update before Init");
+	if (!initialized) throw std::runtime_error("Accelerator::Update before Init");
+	// the objects must stay the same (mbvhaccel.cpp:237-250 rebuilds the root
+	// over the same leaves); their matrices and keys are read by each kernel
+	for (size_t i = 0; i < meshes.size(); ++i)
+		if (meshes[i]->GetType() != TYPE_TRIANGLE && meshes[i]->GetType() != TYPE_EXT_TRIANGLE &&
+				baseOf(meshes[i]) != baseMeshes[objMesh[i]])
+			throw std::runtime_error("MBVHAccel::Update: an object changed its base mesh");
+}
+
+bool HIPAccelerator::Intersect(const Ray *ray, RayHit *hit) const {
+	lrh_scene *scene = nullptr;
+	lrh_device *dev = nullptr;
+	{
+		std::lock_guard<std::mutex> g(kernelsMutex);
+		if (kernels.empty())
+			throw std::runtime_error("HIPAccelerator::Intersect needs a started HIP intersection device");
+		scene = kernels.front()->GetScene();
+		dev = kernels.front()->GetDeviceHandle();
+	}
+	// one ray through the same kernel: device staging, synchronous
+	void *d = nullptr;
+	CheckLrh(lrh_alloc(dev, sizeof(Ray) + sizeof(RayHit), &d));
+	int rc = lrh_memcpy(dev, d, ray, sizeof(Ray), LRH_H2D, nullptr);
+	if (rc == LRH_OK) rc = lrh_trace(scene, d, static_cast<char *>(d) + sizeof(Ray), 1, LRH_QUERY_CLOSEST, nullptr);
+	if (rc == LRH_OK) rc = lrh_memcpy(dev, hit, static_cast<char *>(d) + sizeof(Ray), sizeof(RayHit), LRH_D2H, nullptr);
+	lrh_free(dev, d);
+	CheckLrh(rc);
+	return !hit->Miss();
+}
+
+HardwareIntersectionKernel *HIPAccelerator::NewHardwareIntersectionKernel(HardwareIntersectionDevice &device) const {
+	HIPIntersectionDevice *hip = dynamic_cast<HIPIntersectionDevice *>(&device);
+	if (!hip) throw std::runtime_error("HIPAccelerator::NewHardwareIntersectionKernel: not a HIP device");
+	return new HIPIntersectionKernel(*hip, *this);
+}
+
+//------------------------------------------------------------------------------
+// HIPIntersectionKernel (bvhaccelhw.cpp:239-268, mbvhaccelhw.cpp:442-496)
+//------------------------------------------------------------------------------
+
+HIPIntersectionKernel::HIPIntersectionKernel(HIPIntersectionDevice &dev, const HIPAccelerator &acc)
+	: HardwareIntersectionKernel(dev), accel(acc), devHandle(dev.dev) {
+	scene = accel.build(dev.dev);
+	std::lock_guard<std::mutex> g(accel.kernelsMutex);
+	accel.kernels.push_back(this);
+}
+
+HIPIntersectionKernel::~HIPIntersectionKernel() {
+	{
+		std::lock_guard<std::mutex> g(accel.kernelsMutex);
+		for (size_t i = 0; i < accel.kernels.size(); ++i)
+			if (accel.kernels[i] == this) {
+				accel.kernels.erase(accel.kernels.begin() + (long)i);
+				break;
+			}
+	}
+	if (scene) lrh_scene_destroy(scene);
+}
+
+void HIPIntersectionKernel::Update(const DataSet *) {
+	if (!accel.DoesSupportUpdate()) throw std::runtime_error("BVHKernel does not support Update()");
 	std::vector<lrh_object> objs;
 	std::vector<float> store;
-	fillObjects(objs, store);
-	if (updateMode == UPDATE_REFIT)
+	accel.fillObjects(objs, store);
+	if (accel.GetUpdateMode() == HIPAccelerator::UPDATE_REFIT)
 		CheckLrh(lrh_scene_refit(scene, objs.data(), (unsigned int)objs.size()));
 	else
 		CheckLrh(lrh_scene_update(scene, objs.data(), (unsigned int)objs.size()));
 }
 
-bool Accelerator::Intersect(const Ray *ray, RayHit *hit) const {
-	// one ray through the same kernel: device staging, synchronous
-	void *d = nullptr;
-	CheckLrh(lrh_alloc(device.dev, sizeof(Ray) + sizeof(RayHit), &d));
-	int rc = lrh_memcpy(device.dev, d, ray, sizeof(Ray), LRH_H2D, nullptr);
-	if (rc == LRH_OK)
-		rc = lrh_trace(scene, d, static_cast<char *>(d) + sizeof(Ray), 1, LRH_QUERY_CLOSEST, nullptr);
-	if (rc == LRH_OK)
-		rc = lrh_memcpy(device.dev, hit, static_cast<char *>(d) + sizeof(Ray), sizeof(RayHit), LRH_D2H, nullptr);
-	lrh_free(device.dev, d);
-	CheckLrh(rc);
-	return !hit->Miss();
-}
-
-HardwareIntersectionKernel *Accelerator::NewHardwareIntersectionKernel(HIPIntersectionDevice &dev) const {
-	return new HardwareIntersectionKernel(dev, *this);
-}
-
-//------------------------------------------------------------------------------
-// HardwareIntersectionKernel (bvhaccelhw.cpp:239-268, mbvhaccelhw.cpp:442-496)
-//------------------------------------------------------------------------------
-
-void HardwareIntersectionKernel::Update(const DataSet *) {
-	const_cast<Accelerator &>(accel).Update();
-}
-
-void HardwareIntersectionKernel::EnqueueTraceRayBuffer(HardwareDeviceBuffer *rayBuff,
-		HardwareDeviceBuffer *rayHitBuff, const unsigned int rayCount) {
-	if (rayBuff->GetSize() < (size_t)rayCount * sizeof(Ray) || rayHitBuff->GetSize() < (size_t)rayCount * sizeof(RayHit))
+void HIPIntersectionKernel::trace(HardwareDeviceBuffer *rayBuff, HardwareDeviceBuffer *rayHitBuff,
+		const unsigned int rayCount, unsigned int query) {
+	const HIPDeviceBuffer *rb = dynamic_cast<const HIPDeviceBuffer *>(rayBuff);
+	const HIPDeviceBuffer *hb = dynamic_cast<const HIPDeviceBuffer *>(rayHitBuff);
+	if (!rb || !hb) throw std::runtime_error("EnqueueTraceRayBuffer: buffers not allocated by a HIP device");
+	if (rb->GetSize() < (size_t)rayCount * sizeof(Ray) || hb->GetSize() < (size_t)rayCount * sizeof(RayHit))
 		throw std::runtime_error("EnqueueTraceRayBuffer: buffer smaller than rayCount records");
-	CheckLrh(lrh_trace(accel.GetScene(), rayBuff->GetPtr(), rayHitBuff->GetPtr(), rayCount, LRH_QUERY_CLOSEST, nullptr));
+	const lrh_trace_opts &o = accel.GetTraceOptions();
+	CheckLrh(lrh_trace_ex(scene, rb->GetPtr(), hb->GetPtr(), rayCount, query, &o, nullptr));
 }
 
-void HardwareIntersectionKernel::EnqueueTraceRayBufferAnyHit(HardwareDeviceBuffer *rayBuff,
-		HardwareDeviceBuffer *rayHitBuff, const unsigned int rayCount) {
-	if (rayBuff->GetSize() < (size_t)rayCount * sizeof(Ray) || rayHitBuff->GetSize() < (size_t)rayCount * sizeof(RayHit))
-		throw std::runtime_error("EnqueueTraceRayBufferAnyHit: buffer smaller than rayCount records");
-	CheckLrh(lrh_trace(accel.GetScene(), rayBuff->GetPtr(), rayHitBuff->GetPtr(), rayCount, LRH_QUERY_ANY, nullptr));
+void HIPIntersectionKernel::EnqueueTraceRayBuffer(HardwareDeviceBuffer *rayBuff, HardwareDeviceBuffer *rayHitBuff,
+		const unsigned int rayCount) {
+	trace(rayBuff, rayHitBuff, rayCount, LRH_QUERY_CLOSEST);
+}
+
+void HIPIntersectionKernel::EnqueueTraceRayBufferAnyHit(HardwareDeviceBuffer *rayBuff, HardwareDeviceBuffer *rayHitBuff,
+		const unsigned int rayCount) {
+	trace(rayBuff, rayHitBuff, rayCount, LRH_QUERY_ANY);
 }
 
 //------------------------------------------------------------------------------
@@ -215,117 +454,160 @@ int HIPIntersectionDevice::GetDeviceCount() {
 	return n;
 }
 
-HIPIntersectionDevice::HIPIntersectionDevice(int ordinal) {
-	CheckLrh(lrh_device_open(ordinal, &dev));
-	char buf[256];
-	int cus = 0;
-	size_t mem = 0;
-	CheckLrh(lrh_device_info(dev, buf, sizeof(buf), &cus, &mem));
-	name = std::string("HIP-") + buf;
+HIPIntersectionDevice::HIPIntersectionDevice(const Context *context, HIPDeviceDescription *desc, const size_t devIndex)
+	: Device(context, devIndex), deviceDesc(desc) {
+	if (!desc) throw std::runtime_error("HIPIntersectionDevice: null description");
+	CheckLrh(lrh_device_open(desc->GetOrdinal(), &dev));
+	deviceName = desc->GetName() + " Intersect";
 }
 
 HIPIntersectionDevice::~HIPIntersectionDevice() {
-	kernel.reset();
-	accel.reset();
+	try {
+		if (started) Stop();
+	} catch (...) {
+	}
 	if (dev) lrh_device_close(dev);
 }
 
-void HIPIntersectionDevice::AllocBuffer(HardwareDeviceBuffer **buff, int type, void *src, size_t size,
-		const std::string &) {
+static void noPrograms(const char *what) {
+	throw std::runtime_error(std::string("HIPIntersectionDevice::") + what +
+			": run-time compiled programs are not supported (the HIP device's kernels are built into liblrhip)");
+}
+
+void HIPIntersectionDevice::CompileProgram(HardwareDeviceProgram **, const std::vector<std::string> &,
+		const std::string &, const std::string &) {
+	noPrograms("CompileProgram");
+}
+void HIPIntersectionDevice::GetKernel(HardwareDeviceProgram *, HardwareDeviceKernel **, const std::string &) {
+	noPrograms("GetKernel");
+}
+u_int HIPIntersectionDevice::GetKernelWorkGroupSize(HardwareDeviceKernel *) {
+	noPrograms("GetKernelWorkGroupSize");
+	return 0;
+}
+void HIPIntersectionDevice::SetKernelArg(HardwareDeviceKernel *, const u_int, const size_t, const void *) {
+	noPrograms("SetKernelArg");
+}
+void HIPIntersectionDevice::SetKernelArgBuffer(HardwareDeviceKernel *, const u_int, const HardwareDeviceBuffer *) {
+	noPrograms("SetKernelArgBuffer");
+}
+void HIPIntersectionDevice::EnqueueKernel(HardwareDeviceKernel *, const HardwareDeviceRange &,
+		const HardwareDeviceRange &) {
+	noPrograms("EnqueueKernel");
+}
+
+void HIPIntersectionDevice::AllocBuffer(HardwareDeviceBuffer **buff, const BufferType type, void *src,
+		const size_t size, const std::string &) {
 	// cudadevice.cpp:448-520: a buffer of the same size is reused (content
 	// updated from src); another size frees and reallocates; size 0 leaves a
 	// null buffer
-	if (!*buff) *buff = new HardwareDeviceBuffer();
-	HardwareDeviceBuffer *b = *buff;
+	if (*buff && !dynamic_cast<HIPDeviceBuffer *>(*buff))
+		throw std::runtime_error("AllocBuffer: buffer not allocated by a HIP device");
+	if (!(type & (BUFFER_TYPE_READ_ONLY | BUFFER_TYPE_READ_WRITE)))
+		throw std::runtime_error("AllocBuffer: buffer type must be read-only or read-write");
+	if (!*buff) *buff = new HIPDeviceBuffer();
+	HIPDeviceBuffer *b = static_cast<HIPDeviceBuffer *>(*buff);
 	if (b->ptr) {
 		if (size == b->size) {
 			if (src) EnqueueWriteBuffer(b, false, size, src);
 			return;
 		}
 		lrh_free(dev, b->ptr);
+		FreeMemory(b->size);
 		b->ptr = nullptr;
 		b->size = 0;
 	}
 	if (!size) return;
-	if (!(type & (BUFFER_TYPE_READ_ONLY | BUFFER_TYPE_READ_WRITE)))
-		throw std::runtime_error("AllocBuffer: buffer type must be read-only or read-write");
 	CheckLrh(lrh_alloc(dev, size, &b->ptr));
 	b->size = size;
+	AllocMemory(size);
 	if (src) EnqueueWriteBuffer(b, false, size, src);
 }
 
 void HIPIntersectionDevice::FreeBuffer(HardwareDeviceBuffer **buff) {
 	if (!*buff) return;
-	if ((*buff)->ptr) lrh_free(dev, (*buff)->ptr);
-	delete *buff;
+	HIPDeviceBuffer *b = dynamic_cast<HIPDeviceBuffer *>(*buff);
+	if (!b) throw std::runtime_error("FreeBuffer: buffer not allocated by a HIP device");
+	if (b->ptr) {
+		lrh_free(dev, b->ptr);
+		FreeMemory(b->size);
+	}
+	delete b;
 	*buff = nullptr;
 }
 
-void HIPIntersectionDevice::EnqueueReadBuffer(const HardwareDeviceBuffer *buff, bool blocking, size_t size, void *ptr) {
-	if (size > buff->size) throw std::runtime_error("EnqueueReadBuffer: size beyond the buffer");
-	CheckLrh(blocking ? lrh_memcpy(dev, ptr, buff->ptr, size, LRH_D2H, nullptr)
-					  : lrh_memcpy_async(dev, ptr, buff->ptr, size, LRH_D2H, nullptr));
+void HIPIntersectionDevice::EnqueueReadBuffer(const HardwareDeviceBuffer *buff, const bool blocking, const size_t size,
+		void *ptr) {
+	const HIPDeviceBuffer *b = dynamic_cast<const HIPDeviceBuffer *>(buff);
+	if (!b) throw std::runtime_error("EnqueueReadBuffer: buffer not allocated by a HIP device");
+	if (size > b->size) throw std::runtime_error("EnqueueReadBuffer: size beyond the buffer");
+	CheckLrh(blocking ? lrh_memcpy(dev, ptr, b->ptr, size, LRH_D2H, nullptr)
+					  : lrh_memcpy_async(dev, ptr, b->ptr, size, LRH_D2H, nullptr));
 }
 
-void HIPIntersectionDevice::EnqueueWriteBuffer(const HardwareDeviceBuffer *buff, bool blocking, size_t size,
+void HIPIntersectionDevice::EnqueueWriteBuffer(const HardwareDeviceBuffer *buff, const bool blocking, const size_t size,
 		const void *ptr) {
-	if (size > buff->size) throw std::runtime_error("EnqueueWriteBuffer: size beyond the buffer");
-	CheckLrh(blocking ? lrh_memcpy(dev, buff->ptr, ptr, size, LRH_H2D, nullptr)
-					  : lrh_memcpy_async(dev, buff->ptr, ptr, size, LRH_H2D, nullptr));
+	const HIPDeviceBuffer *b = dynamic_cast<const HIPDeviceBuffer *>(buff);
+	if (!b) throw std::runtime_error("EnqueueWriteBuffer: buffer not allocated by a HIP device");
+	if (size > b->size) throw std::runtime_error("EnqueueWriteBuffer: size beyond the buffer");
+	CheckLrh(blocking ? lrh_memcpy(dev, b->ptr, ptr, size, LRH_H2D, nullptr)
+					  : lrh_memcpy_async(dev, b->ptr, ptr, size, LRH_H2D, nullptr));
 }
 
 void HIPIntersectionDevice::FinishQueue() { CheckLrh(lrh_sync(dev, nullptr)); }
 
-size_t HIPIntersectionDevice::GetUsedMemory() const { return lrh_used_memory(dev); }
-
-void HIPIntersectionDevice::SetDataSet(DataSet *ds, AcceleratorType type) {
-	if (running) throw std::runtime_error("SetDataSet on a running device");
-	dataSet = ds;
-	accelType = type;
+void HIPIntersectionDevice::SetDataSet(DataSet *newDataSet) {
+	IntersectionDevice::SetDataSet(newDataSet);
+	accel = nullptr;
+	if (dataSet) {
+		// oclintersectiondevice.cpp:42-56: AUTO is MBVH with instances or motion, else BVH
+		const AcceleratorType t = dataSet->GetAcceleratorType();
+		if (t != ACCEL_AUTO)
+			accel = dataSet->GetAccelerator(t);
+		else if (dataSet->RequiresInstanceSupport() || dataSet->RequiresMotionBlurSupport())
+			accel = dataSet->GetAccelerator(ACCEL_MBVH);
+		else
+			accel = dataSet->GetAccelerator(ACCEL_BVH);
+	}
 }
 
 void HIPIntersectionDevice::Start() {
-	if (running) throw std::runtime_error("Start on a running device");
-	if (!dataSet) throw std::runtime_error("Start: no DataSet set");
-	if (!dataSet->IsPreprocessed()) dataSet->Preprocess();
-	const AcceleratorType t = dataSet->ResolveType(accelType);
-	accel.reset(new Accelerator(*this, t));
-	accel->Init(dataSet->GetMeshes(), dataSet->GetTotalVertexCount(), dataSet->GetTotalTriangleCount());
-	kernel.reset(accel->NewHardwareIntersectionKernel(*this));
-	running = true;
+	if (started) throw std::runtime_error("Start on a running device");
+	if (!accel) throw std::runtime_error("Start of an intersection device without a DataSet");
+	IntersectionDevice::Start();
+	try {
+		kernel = static_cast<HIPIntersectionKernel *>(accel->NewHardwareIntersectionKernel(*this));
+	} catch (...) {
+		Device::Stop();
+		throw;
+	}
 }
 
 void HIPIntersectionDevice::Stop() {
-	if (!running) return;
+	if (!started) return;
 	FinishQueue();
-	kernel.reset();
-	accel.reset();
-	running = false;
+	delete kernel;
+	kernel = nullptr;
+	Device::Stop();
 }
 
 void HIPIntersectionDevice::Update() {
-	if (!running) throw std::runtime_error("Update on a stopped device");
+	if (!started) throw std::runtime_error("Update on a stopped device");
 	kernel->Update(dataSet);
 }
 
 void HIPIntersectionDevice::EnqueueTraceRayBuffer(HardwareDeviceBuffer *rayBuff, HardwareDeviceBuffer *rayHitBuff,
 		const unsigned int rayCount) {
-	if (!running) throw std::runtime_error("EnqueueTraceRayBuffer on a stopped device");
+	if (!started) throw std::runtime_error("EnqueueTraceRayBuffer on a stopped device");
 	kernel->EnqueueTraceRayBuffer(rayBuff, rayHitBuff, rayCount);
-	statsDataParallel += rayCount;
+	statsTotalDataParallelRayCount += rayCount;
 }
 
 void HIPIntersectionDevice::EnqueueTraceRayBufferAnyHit(HardwareDeviceBuffer *rayBuff,
 		HardwareDeviceBuffer *rayHitBuff, const unsigned int rayCount) {
-	if (!running) throw std::runtime_error("EnqueueTraceRayBufferAnyHit on a stopped device");
+	if (!started) throw std::runtime_error("EnqueueTraceRayBufferAnyHit on a stopped device");
 	kernel->EnqueueTraceRayBufferAnyHit(rayBuff, rayHitBuff, rayCount);
-	statsDataParallel += rayCount;
-}
-
-bool HIPIntersectionDevice::TraceRay(const Ray *ray, RayHit *rayHit) {
-	if (!running) throw std::runtime_error("TraceRay on a stopped device");
-	statsSerial += 1;
-	return accel->Intersect(ray, rayHit);
+	statsTotalDataParallelRayCount += rayCount;
 }
 
 //------------------------------------------------------------------------------
@@ -396,7 +678,8 @@ void ctxLog(const Context *c, F &&f) {
 }
 }  // namespace
 
-Context::Context(LuxRaysDebugHandler handler, bool verb) : debugHandler(handler), verbose(verb) {
+Context::Context(LuxRaysDebugHandler handler, const Properties &config)
+	: debugHandler(handler), cfg(config), verbose(config.GetBool("context.verbose", true)) {
 	ctxLog(this, [](std::ostream &o) { o << "HIP support: enabled"; });
 	HIPDeviceDescription::AddDeviceDescs(deviceDescriptions);
 	ctxLog(this, [&](std::ostream &o) { o << "HIP device count: " << deviceDescriptions.size(); });
@@ -428,11 +711,11 @@ Context::~Context() {
 		if (started) Stop();
 	} catch (...) {
 	}
-	for (HIPIntersectionDevice *d : idevices) delete d;
+	for (IntersectionDevice *d : idevices) delete d;
 	for (DeviceDescription *d : deviceDescriptions) delete d;
 }
 
-std::vector<HIPIntersectionDevice *> Context::AddIntersectionDevices(std::vector<DeviceDescription *> &deviceDesc) {
+std::vector<IntersectionDevice *> Context::AddIntersectionDevices(std::vector<DeviceDescription *> &deviceDesc) {
 	if (started) throw std::runtime_error("Context::AddIntersectionDevices on a started context");
 	ctxLog(this, [&](std::ostream &o) { o << "Creating " << deviceDesc.size() << " intersection device(s)"; });
 	// check every description first, so a bad list creates nothing
@@ -440,50 +723,56 @@ std::vector<HIPIntersectionDevice *> Context::AddIntersectionDevices(std::vector
 		if (!(d->GetType() & DEVICE_TYPE_HIP_ALL))
 			throw std::runtime_error("Unknown device type in Context::CreateIntersectionDevices(): " +
 					std::to_string((int)d->GetType()));
-	std::vector<HIPIntersectionDevice *> created;
+	for (DeviceDescription *d : deviceDesc)
+		if (!dynamic_cast<HIPDeviceDescription *>(d))
+			throw std::runtime_error("Context::CreateIntersectionDevices(): " + d->GetName() +
+					" is not a description this Context enumerated");
+	std::vector<IntersectionDevice *> created;
 	try {
 		for (size_t i = 0; i < deviceDesc.size(); ++i) {
-			const HIPDeviceDescription *hd = static_cast<const HIPDeviceDescription *>(deviceDesc[i]);
+			HIPDeviceDescription *hd = static_cast<HIPDeviceDescription *>(deviceDesc[i]);
 			ctxLog(this, [&](std::ostream &o) {
 				o << "Allocating intersection device " << i << ": " << hd->GetName()
 				  << " (Type = " << DeviceDescription::GetDeviceType(hd->GetType()) << ")";
 			});
-			std::unique_ptr<HIPIntersectionDevice> dev(new HIPIntersectionDevice(hd->GetOrdinal()));
-			dev->desc = hd;
-			dev->deviceIndex = idevices.size() + created.size();
-			if (currentDataSet) dev->SetDataSet(currentDataSet, accelType);
+			std::unique_ptr<HIPIntersectionDevice> dev(new HIPIntersectionDevice(this, hd, idevices.size() + created.size()));
+			if (currentDataSet) dev->SetDataSet(currentDataSet);
 			created.push_back(dev.release());
 		}
 	} catch (...) {
-		for (HIPIntersectionDevice *d : created) delete d;
+		for (IntersectionDevice *d : created) delete d;
 		throw;
 	}
 	idevices.insert(idevices.end(), created.begin(), created.end());
 	return created;
 }
 
-void Context::SetDataSet(DataSet *dataSet, AcceleratorType type) {
+void Context::SetDataSet(DataSet *dataSet) {
 	if (started) throw std::runtime_error("Context::SetDataSet on a started context");
 	currentDataSet = dataSet;
-	accelType = type;
-	for (HIPIntersectionDevice *d : idevices) d->SetDataSet(dataSet, type);
+	for (IntersectionDevice *d : idevices) d->SetDataSet(dataSet);
 }
 
 void Context::UpdateDataSet() {
+	// context.cpp:182-196: the accelerators first, then every hardware device
 	if (!started) throw std::runtime_error("Context::UpdateDataSet on a stopped context");
-	for (HIPIntersectionDevice *d : idevices) d->Update();
+	currentDataSet->UpdateAccelerators();
+	for (IntersectionDevice *d : idevices) {
+		HardwareIntersectionDevice *hw = dynamic_cast<HardwareIntersectionDevice *>(d);
+		if (hw) hw->Update();
+	}
 }
 
 void Context::Start() {
 	if (started) throw std::runtime_error("Context::Start on a started context");
-	for (HIPIntersectionDevice *d : idevices) d->Start();
+	for (IntersectionDevice *d : idevices) d->Start();
 	started = true;
 }
 
 void Context::Stop() {
 	if (!started) throw std::runtime_error("Context::Stop on a stopped context");
 	Interrupt();
-	for (HIPIntersectionDevice *d : idevices) d->Stop();
+	for (IntersectionDevice *d : idevices) d->Stop();
 	started = false;
 }
 

@@ -113,7 +113,7 @@ void logMsg(const char *msg) { std::cout << "[LuxRays] " << msg << std::endl; }
 
 int listDevices() {
 	try {
-		Context ctx(logMsg, true);
+		Context ctx(logMsg, Properties().Set("context.verbose", "true"));
 		const std::vector<DeviceDescription *> &all = ctx.GetAvailableDeviceDescriptions();
 		std::cout << all.size() << " device description(s)" << std::endl;
 		std::vector<DeviceDescription *> gpus = all;
@@ -170,7 +170,7 @@ int registry() {
 	DeviceDescription::FilterOne(cpu);
 	show("filterone cpu-only:", cpu);
 	try {
-		Context ctx(nullptr, false);
+		Context ctx(nullptr, Properties().Set("context.verbose", "false"));
 		std::vector<DeviceDescription *> bad = {all[4], all[0]};
 		ctx.AddIntersectionDevices(bad);
 		std::cout << "context accepted" << std::endl;
@@ -187,11 +187,13 @@ int main(int argc, char **argv) {
 	if (argc == 2 && std::string(argv[1]) == "--registry") return registry();
 	if (argc < 2) {
 		std::cerr << "usage: luxrays_hip_check <dir> [--accel auto|bvh|mbvh] [--anyhit] [--serial N] [--scene FILE] "
-		             "[--refit]\n";
+		             "[--refit] [--prop key=value]...\n";
 		return 2;
 	}
 	const std::string dir = argv[1];
-	AcceleratorType accelType = ACCEL_AUTO;
+	// the Context configuration, as LuxRays reads it (accelerator.*, hip.*)
+	Properties cfg;
+	cfg.Set("context.verbose", "false");
 	bool anyHit = false;
 	u_int serial = 0;
 	std::string sceneFile;
@@ -200,7 +202,7 @@ int main(int argc, char **argv) {
 		const std::string a = argv[i];
 		if (a == "--accel" && i + 1 < argc) {
 			const std::string v = argv[++i];
-			accelType = v == "bvh" ? ACCEL_BVH : v == "mbvh" ? ACCEL_MBVH : ACCEL_AUTO;
+			cfg.Set("accelerator.type", v == "bvh" ? "BVH" : v == "mbvh" ? "MBVH" : "AUTO");
 		} else if (a == "--anyhit")
 			anyHit = true;
 		else if (a == "--serial" && i + 1 < argc)
@@ -209,6 +211,8 @@ int main(int argc, char **argv) {
 			sceneFile = argv[++i];
 		else if (a == "--refit")
 			refit = true;
+		else if (a == "--prop" && i + 1 < argc)
+			cfg.SetFromString(argv[++i]);
 		else {
 			std::cerr << "unknown argument " << a << "\n";
 			return 2;
@@ -295,7 +299,9 @@ int main(int argc, char **argv) {
 				}
 			}
 		}
-		DataSet ds;
+		if (refit) cfg.Set("hip.accelerator.update", "REFIT");
+		Context ctx(nullptr, cfg);
+		DataSet ds(&ctx);
 		for (Mesh *m : objects) ds.Add(m);
 		ds.Preprocess();
 
@@ -305,13 +311,13 @@ int main(int argc, char **argv) {
 		const u_int n = (u_int)(rb.size() / sizeof(Ray));
 		std::vector<RayHit> hits(n);
 
-		Context ctx(nullptr, false);
 		std::vector<DeviceDescription *> devDescs = ctx.GetAvailableDeviceDescriptions();
 		DeviceDescription::Filter(DEVICE_TYPE_HIP_GPU, devDescs);
 		DeviceDescription::FilterOne(devDescs);
 		if (devDescs.empty()) throw std::runtime_error("lrhip: no HIP GPU device available");
-		HIPIntersectionDevice &device = *ctx.AddIntersectionDevices(devDescs)[0];
-		ctx.SetDataSet(&ds, accelType);
+		IntersectionDevice *idev = ctx.AddIntersectionDevices(devDescs)[0];
+		HIPIntersectionDevice &device = dynamic_cast<HIPIntersectionDevice &>(*idev);
+		ctx.SetDataSet(&ds);
 		ctx.Start();
 		std::cout << "device " << device.GetName() << " accel "
 				  << (device.GetAccelerator()->GetType() == ACCEL_BVH ? "BVH" : "MBVH") << " rays " << n << std::endl;
@@ -351,14 +357,12 @@ int main(int argc, char **argv) {
 				else if (o2[i].kind == 2)
 					static_cast<MotionTriangleMesh *>(objects[i])->SetMotionSystem(motion(o2[i]));
 			}
-			if (refit)
-				const_cast<Accelerator *>(device.GetAccelerator())->SetUpdateMode(Accelerator::UPDATE_REFIT);
 			ctx.UpdateDataSet();
 			traceAll("hits2.bin");
 		}
 
-		std::cout << "data-parallel rays " << device.GetDataParallelRayCount() << " serial rays "
-				  << device.GetSerialRayCount() << " used memory " << device.GetUsedMemory() << std::endl;
+		std::cout << "rays traced " << (unsigned long long)device.GetTotalRaysCount() << " used memory "
+				  << device.GetUsedMemory() << std::endl;
 		device.FreeBuffer(&rayBuff);
 		device.FreeBuffer(&hitBuff);
 		ctx.Stop();
