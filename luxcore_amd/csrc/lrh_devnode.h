@@ -7,41 +7,71 @@
 #ifndef LRH_DEVNODE_H
 #define LRH_DEVNODE_H
 
-#include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <cmath>
+#include <cstring>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define LRH_DN __host__ __device__
+#else
+#define LRH_DN
+#endif
 
 #include "lrh_layout.h"
 
 namespace lrhdev {
 
+// float <-> bits on either side (the host build is the CPU emulation of the
+// device tree code)
+LRH_DN inline uint32_t bitsOf(float f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+	return __float_as_uint(f);
+#else
+	uint32_t u;
+	std::memcpy(&u, &f, 4);
+	return u;
+#endif
+}
+LRH_DN inline float fromBits(uint32_t u) {
+#if defined(__HIP_DEVICE_COMPILE__)
+	return __uint_as_float(u);
+#else
+	float f;
+	std::memcpy(&f, &u, 4);
+	return f;
+#endif
+}
+
 struct GBox {
 	float lo[3], hi[3];
 };
 
-__device__ __forceinline__ float minT(float a, float b) { return a < b ? a : b; }  // utils.h Min
-__device__ __forceinline__ float maxT(float a, float b) { return a > b ? a : b; }  // utils.h Max
+LRH_DN inline float minT(float a, float b) { return a < b ? a : b; }  // utils.h Min
+LRH_DN inline float maxT(float a, float b) { return a > b ? a : b; }  // utils.h Max
 
 // MachineEpsilon::E (include/luxrays/core/epsilon.h:50-94)
-__device__ inline float machineEps(float v) {
-	const float n = __uint_as_float(__float_as_uint(v) + 0x80u);
+LRH_DN inline float machineEps(float v) {
+	const float n = fromBits(bitsOf(v) + 0x80u);
 	const float e = fabsf(n - v);
 	return e > 1e-5f ? (e < 1e-1f ? e : 1e-1f) : 1e-5f;
 }
 
 // ordered keys so unsigned min/max order floats
-__device__ __forceinline__ uint32_t fkey(float f) {
-	const uint32_t u = __float_as_uint(f);
+LRH_DN inline uint32_t fkey(float f) {
+	const uint32_t u = bitsOf(f);
 	return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
-__device__ __forceinline__ float unkey(uint32_t k) {
-	return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+LRH_DN inline float unkey(uint32_t k) {
+	return fromBits((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
 }
 
-__device__ __forceinline__ float expScale(uint32_t e) { return __uint_as_float(e << 23); }
-__device__ __forceinline__ float decodeQ(float org, uint32_t q, float s) { return org + (float)q * s; }
+LRH_DN inline float expScale(uint32_t e) { return fromBits(e << 23); }
+LRH_DN inline float decodeQ(float org, uint32_t q, float s) { return org + (float)q * s; }
 
 // outward-rounded 8-bit child bounds, the rule of the host builder's quantize
-__device__ inline void quantizeNode(const GBox *cb, int n, const uint32_t child[4], lrh_node4q *out) {
+LRH_DN inline void quantizeNode(const GBox *cb, int n, const uint32_t child[4], lrh_node4q *out) {
 	lrh_node4q q;
 	GBox pb;
 	for (int k = 0; k < 3; ++k) {
@@ -92,7 +122,7 @@ __device__ inline void quantizeNode(const GBox *cb, int n, const uint32_t child[
 	*out = q;
 }
 
-__device__ inline void floatNode(const GBox *cb, int n, const uint32_t child[4], lrh_node4f *out) {
+LRH_DN inline void floatNode(const GBox *cb, int n, const uint32_t child[4], lrh_node4f *out) {
 	lrh_node4f f;
 	for (int c = 0; c < 4; ++c) {
 		const bool v = c < n;

@@ -1,27 +1,24 @@
 /*
  * Linear BVH build on the device, written into the 4-wide node records of
- * lrh_layout.h.  Steps (one stream, no host round trip except one count per
- * level of the collapse):
- *   1. per triangle: the reference's leaf box (Triangle::WorldBound expanded
- *      by MachineEpsilon::E, src/luxrays/accelerators/bvhaccel.cpp:116-122),
- *      rounded exactly as the host builder rounds it, and the bounds of the
- *      box centres;
- *   2. 63-bit Morton codes of the centres, radix-sorted with the triangle id
- *      as value (stable, so equal codes keep id order);
- *   3. binary radix tree over the sorted codes (Karras, "Maximizing
- *      Parallelism in the Construction of BVHs, Octrees, and k-d Trees",
- *      HPG 2012), equal codes ordered by position;
+ * lrh_layout.h.  The per-element steps are lrh_lbvh.h's (shared with the
+ * CPU emulation); here they run as kernels:
+ *   1. leaf boxes and the bounds of their centres;
+ *   2. 63-bit Morton codes, sorted with the triangle id as value by a stable
+ *      LSD radix sort (8-bit digits: per-tile digit counts, one exclusive
+ *      scan of the [digit][tile] table, a scatter that ranks equal digits
+ *      in index order with wave ballots), then checked to be in order;
+ *   3. binary radix tree over the sorted codes;
  *   4. boxes bottom-up: the second child to finish computes its parent;
- *   5. breadth-first collapse into 4-wide nodes: each node takes a binary
- *      node's two children and keeps splitting its largest-area child until
- *      it has four; a subtree of at most maxLeaf triangles becomes a leaf
- *      (its triangles are consecutive in Morton order); child slots come
- *      from a prefix sum, so the node order is deterministic and the top of
- *      the tree comes first;
+ *   5. breadth-first collapse into 4-wide nodes, child slots from an
+ *      exclusive scan, so the node order is deterministic;
  *   6. triangle records in leaf order, bit for bit the host records.
+ * Every kernel runs a bounded amount of work: no block waits on another
+ * (the sort and the scans are sequences of kernels, not decoupled
+ * look-back), every search loop has a step bound, and a violated
+ * precondition sets a bit of an error word that the host reads after the
+ * sort and after the tree, reporting it as an error instead of going on.
  */
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <cmath>
 #include <stdexcept>
@@ -31,10 +28,11 @@
 #include "lrh_devnode.h"
 #include "lrh_gpubuild.h"
 #include "lrh_layout.h"
+#include "lrh_lbvh.h"
 
 namespace {
 
-using namespace lrhdev;
+using namespace lrhlbvh;
 
 #define GB_CHECK(x)                                                                                   \
 	do {                                                                                              \
@@ -42,26 +40,9 @@ using namespace lrhdev;
 		if (e_ != hipSuccess) throw std::runtime_error(std::string("device build: ") + hipGetErrorString(e_)); \
 	} while (0)
 
-constexpr uint32_t kLeafBit = 0x80000000u;
-constexpr uint32_t kNone = 0xffffffffu;
-
-__device__ __forceinline__ uint32_t meshOf(const uint32_t *base, uint32_t nMeshes, uint32_t i) {
-	uint32_t lo = 0, hi = nMeshes;  // last mesh whose base <= i
-	while (hi - lo > 1) {
-		const uint32_t mid = (lo + hi) >> 1;
-		if (base[mid] <= i) lo = mid; else hi = mid;
-	}
-	return lo;
-}
-
-// triangle i (global id: the triangle arrays are concatenated) of mesh m;
-// its vertex indices are local to the mesh
-__device__ __forceinline__ void loadTri(const float *verts, const uint32_t *tris, const uint32_t *vbase, uint32_t m,
-		uint32_t i, float p[3][3]) {
-	for (int c = 0; c < 3; ++c) {
-		const size_t v = (size_t)vbase[m] + tris[3 * (size_t)i + c];
-		for (int k = 0; k < 3; ++k) p[c][k] = verts[3 * v + k];
-	}
+__global__ void k_fill(uint32_t *p, uint32_t n, uint32_t v) {
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) p[i] = v;
 }
 
 __global__ void k_initBounds(uint32_t *cb) {
@@ -71,7 +52,7 @@ __global__ void k_initBounds(uint32_t *cb) {
 	}
 }
 
-// 1. leaf boxes (triangleBox of the host builder) and centre bounds
+// 1. leaf boxes and centre bounds
 __global__ void k_prims(const float *verts, const uint32_t *tris, const uint32_t *tbase, const uint32_t *vbase,
 		uint32_t nMeshes, uint32_t n, GBox *boxes, uint32_t *cb) {
 	__shared__ uint32_t smin[3], smax[3];
@@ -82,28 +63,10 @@ __global__ void k_prims(const float *verts, const uint32_t *tris, const uint32_t
 	__syncthreads();
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < n) {
-		const uint32_t m = meshOf(tbase, nMeshes, i);
-		float p[3][3];
-		loadTri(verts, tris, vbase, m, i, p);
-		GBox b;
-		for (int k = 0; k < 3; ++k) {
-			b.lo[k] = minT(p[0][k], p[1][k]);
-			b.hi[k] = maxT(p[0][k], p[1][k]);
-		}
-		for (int k = 0; k < 3; ++k) {
-			b.lo[k] = minT(b.lo[k], p[2][k]);
-			b.hi[k] = maxT(b.hi[k], p[2][k]);
-		}
-		const float ea = maxT(machineEps(b.lo[0]), maxT(machineEps(b.lo[1]), machineEps(b.lo[2])));
-		const float ec = maxT(machineEps(b.hi[0]), maxT(machineEps(b.hi[1]), machineEps(b.hi[2])));
-		const float e = maxT(ea, ec);
-		for (int k = 0; k < 3; ++k) {
-			b.lo[k] -= e;
-			b.hi[k] += e;
-		}
+		const GBox b = primBox(verts, tris, tbase, vbase, nMeshes, i);
 		boxes[i] = b;
 		for (int k = 0; k < 3; ++k) {
-			const uint32_t c = fkey(b.lo[k] + b.hi[k]);
+			const uint32_t c = lrhdev::fkey(b.lo[k] + b.hi[k]);
 			atomicMin(&smin[k], c);
 			atomicMax(&smax[k], c);
 		}
@@ -115,71 +78,142 @@ __global__ void k_prims(const float *verts, const uint32_t *tris, const uint32_t
 	}
 }
 
-__device__ __forceinline__ uint64_t spread21(uint32_t v) {
-	uint64_t x = v & 0x1fffffu;
-	x = (x | (x << 32)) & 0x1f00000000ffffull;
-	x = (x | (x << 16)) & 0x1f0000ff0000ffull;
-	x = (x | (x << 8)) & 0x100f00f00f00f00full;
-	x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
-	x = (x | (x << 2)) & 0x1249249249249249ull;
-	return x;
-}
-
-// 2. Morton codes of the box centres (21 bits per axis)
-__global__ void k_morton(const GBox *boxes, uint32_t n, const uint32_t *cb, uint64_t *keys, uint32_t *vals) {
+// 2. Morton codes of the box centres
+__global__ void k_morton(const GBox *boxes, uint32_t n, const uint32_t *cbIn, uint64_t *keys, uint32_t *vals) {
+	__shared__ uint32_t cb[6];
+	if (threadIdx.x < 6) cb[threadIdx.x] = cbIn[threadIdx.x];
+	__syncthreads();
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
-	const GBox b = boxes[i];
-	uint32_t q[3];
-	for (int k = 0; k < 3; ++k) {
-		const float lo = unkey(cb[k]), hi = unkey(cb[3 + k]);
-		const float ext = hi - lo;
-		const float u = ext > 0.f ? ((b.lo[k] + b.hi[k]) - lo) / ext : 0.f;
-		float s = u * 2097152.f;
-		s = s < 0.f ? 0.f : (s > 2097151.f ? 2097151.f : s);
-		q[k] = (uint32_t)s;
-	}
-	keys[i] = (spread21(q[0]) << 2) | (spread21(q[1]) << 1) | spread21(q[2]);
+	keys[i] = mortonKey(boxes[i], cb);
 	vals[i] = i;
 }
 
-__device__ __forceinline__ int delta(const uint64_t *k, int n, int i, int j) {
-	if (j < 0 || j >= n) return -1;
-	const uint64_t a = k[i], b = k[j];
-	if (a != b) return __clzll((long long)(a ^ b));
-	return 64 + __clz((int)((uint32_t)i ^ (uint32_t)j));
+// ---- radix sort pass: per-tile digit counts, table [digit][tile] ----
+__global__ __launch_bounds__(kSortBlock) void k_sortCount(const uint64_t *keys, uint32_t n, uint32_t shift,
+		uint32_t nTiles, uint32_t *table) {
+	__shared__ uint32_t cnt[256];
+	cnt[threadIdx.x] = 0;
+	__syncthreads();
+	const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+	for (uint32_t j = threadIdx.x; j < kSortTile; j += kSortBlock) {
+		const uint64_t idx = base + j;
+		if (idx < n) atomicAdd(&cnt[digitOf(keys[idx], shift)], 1u);
+	}
+	__syncthreads();
+	table[(size_t)threadIdx.x * nTiles + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// 3. binary radix tree: internal node i, children (kLeafBit | leaf) or internal index
-__global__ void k_radixTree(const uint64_t *keys, int n, uint32_t *left, uint32_t *right, uint32_t *first,
-		uint32_t *last, uint32_t *iparent, uint32_t *lparent) {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n - 1) return;
-	const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
-	const int dmin = delta(keys, n, i, i - d);
-	int lmax = 2;
-	while (delta(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
-	int l = 0;
-	for (int t = lmax / 2; t >= 1; t /= 2)
-		if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
-	const int j = i + l * d;
-	const int dnode = delta(keys, n, i, j);
-	int s = 0;
-	for (int t = (l + 1) / 2;; t = (t + 1) / 2) {
-		if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
-		if (t == 1) break;
+// ---- radix sort pass: stable scatter.  A tile is taken in rounds of 256
+// consecutive keys, one per thread in index order; a key's place is its
+// digit's table offset + the tile's earlier keys of that digit (earlier
+// rounds) + the earlier waves' keys of that digit in this round + its rank
+// among the lanes of its wave with the same digit (ballots over the eight
+// digit bits). ----
+__global__ __launch_bounds__(kSortBlock) void k_sortScatter(const uint64_t *keysIn, const uint32_t *valsIn, uint32_t n,
+		uint32_t shift, uint32_t nTiles, const uint32_t *offsets, uint64_t *keysOut, uint32_t *valsOut, uint32_t *err) {
+	constexpr uint32_t kWaves = kSortBlock / 64;
+	__shared__ uint32_t run[256];
+	__shared__ uint32_t wcnt[kWaves][256];
+	__shared__ uint32_t off[256];
+	const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
+	run[tid] = 0;
+	for (uint32_t w = 0; w < kWaves; ++w) wcnt[w][tid] = 0;
+	off[tid] = offsets[(size_t)tid * nTiles + blockIdx.x];
+	__syncthreads();
+	const uint64_t lanesBelow = (1ull << lane) - 1ull;
+	const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+	for (uint32_t r = 0; r < kSortTile / kSortBlock; ++r) {
+		const uint64_t idx = base + (uint64_t)r * kSortBlock + tid;
+		const bool valid = idx < n;
+		const uint64_t key = valid ? keysIn[idx] : 0ull;
+		const uint32_t dg = valid ? digitOf(key, shift) : 0u;
+		uint64_t same = __ballot(valid);
+		for (uint32_t b = 0; b < 8; ++b) {
+			const bool bit = (dg >> b) & 1u;
+			const uint64_t m = __ballot(bit);
+			same &= bit ? m : ~m;
+		}
+		const uint32_t rank = (uint32_t)__popcll(same & lanesBelow);
+		if (valid && rank == 0) wcnt[wave][dg] = (uint32_t)__popcll(same);
+		__syncthreads();
+		if (valid) {
+			uint32_t before = run[dg];
+			for (uint32_t w = 0; w < wave; ++w) before += wcnt[w][dg];
+			const uint32_t pos = off[dg] + before + rank;
+			if (pos < n) {
+				keysOut[pos] = key;
+				valsOut[pos] = valsIn[idx];
+			} else {
+				atomicOr(err, kErrUnsorted);  // a place outside the array: never written
+			}
+		}
+		__syncthreads();
+		uint32_t add = 0;
+		for (uint32_t w = 0; w < kWaves; ++w) {
+			add += wcnt[w][tid];
+			wcnt[w][tid] = 0;
+		}
+		run[tid] += add;
+		__syncthreads();
 	}
-	const int g = i + s * d + (d < 0 ? -1 : 0);
-	const int lo = i < j ? i : j, hi = i < j ? j : i;
-	const uint32_t lc = (lo == g) ? (kLeafBit | (uint32_t)g) : (uint32_t)g;
-	const uint32_t rc = (hi == g + 1) ? (kLeafBit | (uint32_t)(g + 1)) : (uint32_t)(g + 1);
+}
+
+// ---- exclusive scan of uint32 values: per-tile scans, then the tile
+// totals scanned the same way, then added ----
+__global__ __launch_bounds__(256) void k_scanTile(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *tileSums) {
+	constexpr uint32_t kPer = kScanTile / 256;
+	__shared__ uint32_t part[256];
+	const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kPer;
+	uint32_t v[kPer];
+	uint32_t sum = 0;
+	for (uint32_t k = 0; k < kPer; ++k) {
+		v[k] = (base + k < n) ? in[base + k] : 0u;
+		sum += v[k];
+	}
+	part[threadIdx.x] = sum;
+	__syncthreads();
+	// inclusive scan of the 256 partial sums (Hillis-Steele)
+	for (uint32_t o = 1; o < 256; o <<= 1) {
+		const uint32_t add = threadIdx.x >= o ? part[threadIdx.x - o] : 0u;
+		__syncthreads();
+		part[threadIdx.x] += add;
+		__syncthreads();
+	}
+	uint32_t run = part[threadIdx.x] - sum;
+	for (uint32_t k = 0; k < kPer; ++k) {
+		if (base + k < n) out[base + k] = run;
+		run += v[k];
+	}
+	if (threadIdx.x == 255) tileSums[blockIdx.x] = part[255];
+}
+
+__global__ void k_scanAdd(uint32_t *out, uint32_t n, const uint32_t *tileOffsets) {
+	const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) out[i] += tileOffsets[i / kScanTile];
+}
+
+__global__ void k_checkSorted(const uint64_t *keys, uint32_t n, uint32_t *err) {
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i + 1 < n && keys[i] > keys[i + 1]) atomicOr(err, kErrUnsorted);
+}
+
+// 3. binary radix tree
+__global__ void k_radixTree(const uint64_t *keys, uint32_t n, uint32_t *left, uint32_t *right, uint32_t *first,
+		uint32_t *last, uint32_t *iparent, uint32_t *lparent, uint32_t *err) {
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i + 1 >= n) return;
+	uint32_t lc, rc, lo, hi;
+	if (!radixNode(keys, n, i, lc, rc, lo, hi)) {
+		atomicOr(err, kErrRange);
+		return;
+	}
 	left[i] = lc;
 	right[i] = rc;
-	first[i] = (uint32_t)lo;
-	last[i] = (uint32_t)hi;
-	if (lc & kLeafBit) lparent[lc & ~kLeafBit] = (uint32_t)i; else iparent[lc] = (uint32_t)i;
-	if (rc & kLeafBit) lparent[rc & ~kLeafBit] = (uint32_t)i; else iparent[rc] = (uint32_t)i;
-	if (i == 0) iparent[0] = kNone;
+	first[i] = lo;
+	last[i] = hi;
+	if (lc & kLeafBit) lparent[lc & ~kLeafBit] = i; else iparent[lc] = i;
+	if (rc & kLeafBit) lparent[rc & ~kLeafBit] = i; else iparent[rc] = i;
 }
 
 __device__ __forceinline__ float loadF(const float *p) {
@@ -201,14 +235,19 @@ __device__ GBox childBox(uint32_t c, const GBox *leafBox, const GBox *nodeBox) {
 // 4. boxes bottom-up; also the leaf boxes in sorted order
 __global__ void k_boxesUp(const GBox *boxes, const uint32_t *vals, uint32_t n, const uint32_t *left,
 		const uint32_t *right, const uint32_t *iparent, const uint32_t *lparent, uint32_t *flags, GBox *leafBox,
-		GBox *nodeBox) {
+		GBox *nodeBox, uint32_t *err) {
 	const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
 	if (j >= n) return;
 	leafBox[j] = boxes[vals[j]];
 	uint32_t p = lparent[j];
 	// a binary radix tree over n keys is at most 96 levels deep (64 code bits
-	// + 32 position bits); the bound also stops the walk on a malformed tree
-	for (int steps = 0; p < n - 1 && steps < 128; ++steps) {
+	// + 32 position bits)
+	int steps = 0;
+	for (; p < n - 1; ++steps) {
+		if (steps >= 128) {
+			atomicOr(err, kErrClimb);
+			return;
+		}
 		__threadfence();
 		if (atomicAdd(&flags[p], 1u) == 0u) return;  // the sibling finishes this node
 		__threadfence();
@@ -223,89 +262,34 @@ __global__ void k_boxesUp(const GBox *boxes, const uint32_t *vals, uint32_t n, c
 	}
 }
 
-__device__ __forceinline__ uint32_t leafRef(uint32_t start, uint32_t count) {
-	return LRH_CHILD_LEAF | ((count - 1u) << LRH_LEAF_START_BITS) | start;
-}
-
-__device__ __forceinline__ uint32_t countOf(uint32_t c, const uint32_t *first, const uint32_t *last) {
-	return (c & kLeafBit) ? 1u : last[c] - first[c] + 1u;
-}
-
-__device__ __forceinline__ float areaOf(const GBox &b) {
-	const float dx = b.hi[0] - b.lo[0], dy = b.hi[1] - b.lo[1], dz = b.hi[2] - b.lo[2];
-	if (!(dx >= 0.f)) return 0.f;
-	return 2.f * (dx * dy + dy * dz + dz * dx);
-}
-
-// 5a. children of each frontier node (binary refs, Morton order) and how many become inner nodes
+// 5a. children of each frontier node and how many become inner nodes
 __global__ void k_expand(const uint32_t *frontier, uint32_t nf, uint32_t nInner, const uint32_t *left,
-		const uint32_t *right, const uint32_t *first, const uint32_t *last, const GBox *leafBox, const GBox *nodeBox,
-		uint32_t maxLeaf, uint32_t *kids, uint32_t *innerCount) {
+		const uint32_t *right, const uint32_t *first, const uint32_t *last, const GBox *nodeBox, uint32_t maxLeaf,
+		uint32_t *kids, uint32_t *innerCount, uint32_t *err) {
 	const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
 	if (e >= nf) return;
 	const uint32_t b = frontier[e];
-	if (b >= nInner) {  // not an internal node: nothing to expand (malformed input)
+	if (b >= nInner) {  // not an internal node
+		atomicOr(err, kErrExpand);
 		for (int k = 0; k < 4; ++k) kids[4 * e + k] = kNone;
 		innerCount[e] = 0;
 		return;
 	}
-	uint32_t ch[4] = {left[b], right[b], kNone, kNone};
-	int nch = 2;
-	while (nch < 4) {
-		int pick = -1;
-		float best = -1.f;
-		for (int k = 0; k < nch; ++k) {
-			if ((ch[k] & kLeafBit) || countOf(ch[k], first, last) <= maxLeaf) continue;
-			const float a = areaOf(nodeBox[ch[k]]);
-			if (a > best) {
-				best = a;
-				pick = k;
-			}
-		}
-		if (pick < 0) break;
-		const uint32_t c = ch[pick];
-		for (int k = nch; k > pick + 1; --k) ch[k] = ch[k - 1];
-		ch[pick] = left[c];
-		ch[pick + 1] = right[c];
-		++nch;
-	}
-	uint32_t inner = 0;
-	for (int k = 0; k < 4; ++k) {
-		kids[4 * e + k] = ch[k];
-		if (ch[k] != kNone && countOf(ch[k], first, last) > maxLeaf) ++inner;
-	}
-	innerCount[e] = inner;
+	uint32_t ch[4];
+	innerCount[e] = expandNode(b, left, right, first, last, nodeBox, maxLeaf, ch);
+	for (int k = 0; k < 4; ++k) kids[4 * e + k] = ch[k];
 }
 
-// 5b. write the frontier's wide nodes; inner children get slots after the level
-__global__ void k_emit(const uint32_t *frontier, uint32_t nf, const uint32_t *kids, const uint32_t *innerOff,
-		const uint32_t *first, const uint32_t *last, const GBox *leafBox, const GBox *nodeBox, uint32_t maxLeaf,
-		uint32_t levelBase, uint32_t layout, void *nodes, uint32_t *next) {
+// 5b. the frontier's wide nodes; inner children get slots after the level
+__global__ void k_emit(uint32_t nf, const uint32_t *kids, const uint32_t *innerOff, const uint32_t *first,
+		const uint32_t *last, const GBox *leafBox, const GBox *nodeBox, uint32_t maxLeaf, uint32_t levelBase,
+		uint32_t layout, void *nodes, uint32_t *next) {
 	const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
 	if (e >= nf) return;
-	GBox cb[4];
-	uint32_t ref[4] = {LRH_CHILD_EMPTY, LRH_CHILD_EMPTY, LRH_CHILD_EMPTY, LRH_CHILD_EMPTY};
-	int n = 0;
-	uint32_t slot = innerOff[e];
-	for (int k = 0; k < 4; ++k) {
-		const uint32_t c = kids[4 * e + k];
-		if (c == kNone) continue;
-		cb[n] = (c & kLeafBit) ? leafBox[c & ~kLeafBit] : nodeBox[c];
-		const uint32_t cnt = countOf(c, first, last);
-		if (cnt <= maxLeaf) {
-			const uint32_t start = (c & kLeafBit) ? (c & ~kLeafBit) : first[c];
-			ref[n] = leafRef(start, cnt);
-		} else {
-			ref[n] = levelBase + nf + slot;
-			next[slot] = c;
-			++slot;
-		}
-		++n;
-	}
-	if (layout == 0)
-		quantizeNode(cb, n, ref, reinterpret_cast<lrh_node4q *>(nodes) + levelBase + e);
-	else
-		floatNode(cb, n, ref, reinterpret_cast<lrh_node4f *>(nodes) + levelBase + e);
+	uint32_t ch[4];
+	for (int k = 0; k < 4; ++k) ch[k] = kids[4 * e + k];
+	emitNode(ch, innerOff[e], first, last, leafBox, nodeBox, maxLeaf, levelBase + nf, layout, nodes, levelBase + e,
+			next);
 }
 
 // a scene of at most maxLeaf triangles: one node with one leaf
@@ -323,9 +307,9 @@ __global__ void k_smallRoot(const GBox *leafBox, uint32_t n, uint32_t layout, vo
 		}
 	const uint32_t ref[4] = {leafRef(0, n), LRH_CHILD_EMPTY, LRH_CHILD_EMPTY, LRH_CHILD_EMPTY};
 	if (layout == 0)
-		quantizeNode(&b, 1, ref, reinterpret_cast<lrh_node4q *>(nodes));
+		lrhdev::quantizeNode(&b, 1, ref, reinterpret_cast<lrh_node4q *>(nodes));
 	else
-		floatNode(&b, 1, ref, reinterpret_cast<lrh_node4f *>(nodes));
+		lrhdev::floatNode(&b, 1, ref, reinterpret_cast<lrh_node4f *>(nodes));
 }
 
 __global__ void k_identity(const GBox *boxes, uint32_t n, uint32_t *vals, GBox *leafBox) {
@@ -335,25 +319,13 @@ __global__ void k_identity(const GBox *boxes, uint32_t n, uint32_t *vals, GBox *
 	leafBox[i] = boxes[i];
 }
 
-// 6. triangle records in leaf order, as the host builder writes them
+// 6. triangle records in leaf order
 __global__ void k_records(const float *verts, const uint32_t *tris, const uint32_t *tbase, const uint32_t *vbase,
 		uint32_t nMeshes, const uint32_t *vals, uint32_t n, lrh_tri *recs, uint32_t *invPerm) {
 	const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
 	if (j >= n) return;
 	const uint32_t i = vals[j];
-	const uint32_t m = meshOf(tbase, nMeshes, i);
-	float p[3][3];
-	loadTri(verts, tris, vbase, m, i, p);
-	lrh_tri r;
-	for (int k = 0; k < 3; ++k) {
-		r.p0[k] = p[0][k];
-		r.e1[k] = p[1][k] - p[0][k];
-		r.e2[k] = p[2][k] - p[0][k];
-	}
-	r.triIndex = i - tbase[m];
-	r.meshIndex = m;
-	r.pad = 0;
-	recs[j] = r;
+	recs[j] = triRecord(verts, tris, tbase, vbase, nMeshes, i);
 	invPerm[i] = j;
 }
 
@@ -375,11 +347,43 @@ struct Scratch {
 
 inline unsigned grid(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
+// exclusive scan of n values in place of out (in may equal out); the tile
+// sums are scanned by the same routine, so the depth is log_2048(n)
+void exclusiveScan(Scratch &S, const uint32_t *in, uint32_t *out, uint32_t n, hipStream_t st) {
+	if (n == 0) return;
+	const uint32_t tiles = grid(n, kScanTile);
+	uint32_t *sums = S.get<uint32_t>(tiles);
+	hipLaunchKernelGGL(k_scanTile, dim3(tiles), dim3(256), 0, st, in, out, n, sums);
+	GB_CHECK(hipGetLastError());
+	if (tiles > 1) {
+		exclusiveScan(S, sums, sums, tiles, st);
+		hipLaunchKernelGGL(k_scanAdd, dim3(grid(n, 256)), dim3(256), 0, st, out, n, sums);
+		GB_CHECK(hipGetLastError());
+	}
+}
+
+uint32_t readError(const uint32_t *err, hipStream_t st) {
+	uint32_t e = 0;
+	GB_CHECK(hipMemcpyAsync(&e, err, 4, hipMemcpyDeviceToHost, st));
+	GB_CHECK(hipStreamSynchronize(st));
+	return e;
+}
+
+std::string errorText(uint32_t e) {
+	std::string s = "device build: ";
+	if (e & kErrUnsorted) s += "sorted Morton codes out of order; ";
+	if (e & kErrRange) s += "radix tree node without a key range (codes not in order); ";
+	if (e & kErrClimb) s += "bottom-up box walk longer than any valid tree; ";
+	if (e & kErrExpand) s += "collapse reached a non-internal node; ";
+	return s;
+}
+
 }  // namespace
 
 void lrhGpuBuildBVH(const LrhGpuBuildIn &in, const LrhGpuAlloc &a, hipStream_t st, LrhGpuBuildOut *out) {
 	const uint32_t n = in.nTris;
 	if (n == 0) throw std::runtime_error("device build: empty scene");
+	if (n > 0x7fffffffu) throw std::runtime_error("device build: at most 2^31-1 triangles");
 	if (in.maxLeaf < 1 || in.maxLeaf > LRH_LEAF_MAX) throw std::runtime_error("device build: maxLeaf must be 1..8");
 	const size_t nodeSz = in.layout == 0 ? sizeof(lrh_node4q) : sizeof(lrh_node4f);
 	Scratch S(a);
@@ -388,6 +392,8 @@ void lrhGpuBuildBVH(const LrhGpuBuildIn &in, const LrhGpuAlloc &a, hipStream_t s
 	GB_CHECK(hipEventCreate(&e1));
 	GB_CHECK(hipEventRecord(e0, st));
 
+	uint32_t *err = S.get<uint32_t>(1);
+	hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, err, 1u, 0u);
 	uint32_t *tbase = S.get<uint32_t>(in.nMeshes + 1), *vbase = S.get<uint32_t>(in.nMeshes + 1);
 	GB_CHECK(hipMemcpyAsync(tbase, in.meshTriBase, (in.nMeshes + 1) * 4, hipMemcpyHostToDevice, st));
 	GB_CHECK(hipMemcpyAsync(vbase, in.meshVertBase, (in.nMeshes + 1) * 4, hipMemcpyHostToDevice, st));
@@ -411,51 +417,67 @@ void lrhGpuBuildBVH(const LrhGpuBuildIn &in, const LrhGpuAlloc &a, hipStream_t s
 			GB_CHECK(hipGetLastError());
 			nodeCount = 1;
 		} else {
-			uint64_t *keys = S.get<uint64_t>(n), *keysS = S.get<uint64_t>(n);
-			uint32_t *valsIn = S.get<uint32_t>(n);
-			hipLaunchKernelGGL(k_morton, dim3(grid(n, 256)), dim3(256), 0, st, boxes, n, cb, keys, valsIn);
+			uint64_t *keys = S.get<uint64_t>(n), *keysB = S.get<uint64_t>(n);
+			uint32_t *valsB = S.get<uint32_t>(n);
+			hipLaunchKernelGGL(k_morton, dim3(grid(n, 256)), dim3(256), 0, st, boxes, n, cb, keys, vals);
 			GB_CHECK(hipGetLastError());
-			size_t tmpBytes = 0;
-			GB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpBytes, keys, keysS, valsIn, vals, (int)n, 0, 63, st));
-			void *tmp = S.get<uint8_t>(tmpBytes);
-			GB_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tmpBytes, keys, keysS, valsIn, vals, (int)n, 0, 63, st));
+			// stable LSD radix sort, 8 passes of 8 bits; an even number of
+			// passes leaves the result in keys / vals
+			const uint32_t tiles = grid(n, kSortTile);
+			uint32_t *table = S.get<uint32_t>((size_t)256 * tiles);
+			uint64_t *kIn = keys, *kOut = keysB;
+			uint32_t *vIn = vals, *vOut = valsB;
+			for (uint32_t shift = 0; shift < 64; shift += 8) {
+				hipLaunchKernelGGL(k_sortCount, dim3(tiles), dim3(kSortBlock), 0, st, kIn, n, shift, tiles, table);
+				GB_CHECK(hipGetLastError());
+				exclusiveScan(S, table, table, 256u * tiles, st);
+				hipLaunchKernelGGL(k_sortScatter, dim3(tiles), dim3(kSortBlock), 0, st, kIn, vIn, n, shift, tiles, table,
+						kOut, vOut, err);
+				GB_CHECK(hipGetLastError());
+				std::swap(kIn, kOut);
+				std::swap(vIn, vOut);
+			}
+			hipLaunchKernelGGL(k_checkSorted, dim3(grid(n, 256)), dim3(256), 0, st, keys, n, err);
+			GB_CHECK(hipGetLastError());
+			if (const uint32_t e = readError(err, st)) throw std::runtime_error(errorText(e));
 
 			const uint32_t ni = n - 1;
 			uint32_t *left = S.get<uint32_t>(ni), *right = S.get<uint32_t>(ni), *first = S.get<uint32_t>(ni),
 					 *last = S.get<uint32_t>(ni), *iparent = S.get<uint32_t>(ni), *lparent = S.get<uint32_t>(n),
 					 *flags = S.get<uint32_t>(ni);
 			GBox *nodeBox = S.get<GBox>(ni);
-			GB_CHECK(hipMemsetAsync(flags, 0, (size_t)ni * 4, st));
-			GB_CHECK(hipMemsetAsync(iparent, 0xff, (size_t)ni * 4, st));  // unset parents end the climb
-			GB_CHECK(hipMemsetAsync(lparent, 0xff, (size_t)n * 4, st));
-			hipLaunchKernelGGL(k_radixTree, dim3(grid(ni, 256)), dim3(256), 0, st, keysS, (int)n, left, right, first,
-					last, iparent, lparent);
-			hipLaunchKernelGGL(k_boxesUp, dim3(grid(n, 256)), dim3(256), 0, st, boxes, vals, n, left, right, iparent,
-					lparent, flags, leafBox, nodeBox);
+			hipLaunchKernelGGL(k_fill, dim3(grid(ni, 256)), dim3(256), 0, st, flags, ni, 0u);
+			hipLaunchKernelGGL(k_fill, dim3(grid(ni, 256)), dim3(256), 0, st, iparent, ni, kNone);  // root: none
+			hipLaunchKernelGGL(k_fill, dim3(grid(n, 256)), dim3(256), 0, st, lparent, n, kNone);
+			hipLaunchKernelGGL(k_radixTree, dim3(grid(ni, 256)), dim3(256), 0, st, keys, n, left, right, first, last,
+					iparent, lparent, err);
 			GB_CHECK(hipGetLastError());
+			if (const uint32_t e = readError(err, st)) throw std::runtime_error(errorText(e));
+			hipLaunchKernelGGL(k_boxesUp, dim3(grid(n, 256)), dim3(256), 0, st, boxes, vals, n, left, right, iparent,
+					lparent, flags, leafBox, nodeBox, err);
+			GB_CHECK(hipGetLastError());
+			if (const uint32_t e = readError(err, st)) throw std::runtime_error(errorText(e));
 
 			// breadth-first collapse: level frontiers of binary node ids
 			uint32_t *fA = S.get<uint32_t>(ni), *fB = S.get<uint32_t>(ni);
 			uint32_t *kids = S.get<uint32_t>(4 * (size_t)ni), *cnt = S.get<uint32_t>(ni + 1),
 					 *off = S.get<uint32_t>(ni + 1);
-			size_t scanBytes = 0;
-			GB_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, scanBytes, cnt, off, (int)(ni + 1), st));
-			void *scanTmp = S.get<uint8_t>(scanBytes);
-			const uint32_t root = 0;
-			GB_CHECK(hipMemcpyAsync(fA, &root, 4, hipMemcpyHostToDevice, st));
+			hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, fA, 1u, 0u);  // the root
 			uint32_t nf = 1, levelBase = 0;
 			while (nf) {
 				if ((size_t)levelBase + nf > capNodes) throw std::runtime_error("device build: node capacity");
+				if (depth > 128) throw std::runtime_error("device build: collapse deeper than any valid tree");
 				hipLaunchKernelGGL(k_expand, dim3(grid(nf, 256)), dim3(256), 0, st, fA, nf, ni, left, right, first,
-						last, leafBox, nodeBox, in.maxLeaf, kids, cnt);
-				GB_CHECK(hipMemsetAsync(cnt + nf, 0, 4, st));
-				GB_CHECK(hipcub::DeviceScan::ExclusiveSum(scanTmp, scanBytes, cnt, off, (int)(nf + 1), st));
-				hipLaunchKernelGGL(k_emit, dim3(grid(nf, 256)), dim3(256), 0, st, fA, nf, kids, off, first, last,
-						leafBox, nodeBox, in.maxLeaf, levelBase, in.layout, nodes, fB);
+						last, nodeBox, in.maxLeaf, kids, cnt, err);
+				hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, cnt + nf, 1u, 0u);
+				GB_CHECK(hipGetLastError());
+				exclusiveScan(S, cnt, off, nf + 1, st);
+				hipLaunchKernelGGL(k_emit, dim3(grid(nf, 256)), dim3(256), 0, st, nf, kids, off, first, last, leafBox,
+						nodeBox, in.maxLeaf, levelBase, in.layout, nodes, fB);
 				GB_CHECK(hipGetLastError());
 				uint32_t nextN = 0;
 				GB_CHECK(hipMemcpyAsync(&nextN, off + nf, 4, hipMemcpyDeviceToHost, st));
-				GB_CHECK(hipStreamSynchronize(st));
+				if (const uint32_t e = readError(err, st)) throw std::runtime_error(errorText(e));
 				levelBase += nf;
 				nf = nextN;
 				std::swap(fA, fB);

@@ -1,0 +1,277 @@
+/*
+ * The per-element steps of the linear BVH build, shared by the device
+ * kernels (lrh_gpubuild.hip) and their sequential CPU emulation (the test
+ * suite's check of the build's algorithm and of its error paths):
+ *   1. a triangle's leaf box: Triangle::WorldBound expanded by
+ *      MachineEpsilon::E (src/luxrays/accelerators/bvhaccel.cpp:116-122),
+ *      rounded as the host builder rounds it;
+ *   2. the 63-bit Morton code of its centre (21 bits per axis);
+ *   3. a node of the binary radix tree over the sorted codes (Karras,
+ *      "Maximizing Parallelism in the Construction of BVHs, Octrees, and k-d
+ *      Trees", HPG 2012), equal codes ordered by position;
+ *   4. the expansion of a binary node into up to four children (largest
+ *      area first) and the 4-wide record it becomes;
+ *   5. the triangle record, bit for bit the host builder's.
+ * Every search loop is bounded: keys that are not in order make the range
+ * search fail (radixNode returns false) instead of looping.
+ */
+#ifndef LRH_LBVH_H
+#define LRH_LBVH_H
+
+#include <stdint.h>
+
+#include "lrh_devnode.h"
+#include "lrh_layout.h"
+
+namespace lrhlbvh {
+
+using lrhdev::GBox;
+using lrhdev::maxT;
+using lrhdev::minT;
+
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kNone = 0xffffffffu;
+
+// bits of the build's error word
+enum : uint32_t {
+	kErrUnsorted = 1u,   // sorted codes out of order
+	kErrRange = 2u,      // a radix-tree node found no range (keys not in order around it)
+	kErrClimb = 4u,      // a bottom-up box walk longer than any valid tree
+	kErrExpand = 8u,     // a frontier entry that is not an internal node
+};
+
+// radix sort: 8-bit digits, tiles of kSortTile keys per block
+constexpr uint32_t kSortTile = 2048u;
+constexpr uint32_t kSortBlock = 256u;
+// exclusive scan: tiles of kScanTile values per block
+constexpr uint32_t kScanTile = 2048u;
+
+LRH_DN inline uint32_t meshOf(const uint32_t *base, uint32_t nMeshes, uint32_t i) {
+	uint32_t lo = 0, hi = nMeshes;  // last mesh whose base <= i
+	while (hi - lo > 1) {
+		const uint32_t mid = (lo + hi) >> 1;
+		if (base[mid] <= i) lo = mid; else hi = mid;
+	}
+	return lo;
+}
+
+// triangle i (global id over the concatenated meshes) of mesh m; its vertex
+// indices are local to the mesh
+LRH_DN inline void loadTri(const float *verts, const uint32_t *tris, const uint32_t *vbase, uint32_t m, uint32_t i,
+		float p[3][3]) {
+	for (int c = 0; c < 3; ++c) {
+		const size_t v = (size_t)vbase[m] + tris[3 * (size_t)i + c];
+		for (int k = 0; k < 3; ++k) p[c][k] = verts[3 * v + k];
+	}
+}
+
+// 1. leaf box of triangle i (the host builder's triangleBox)
+LRH_DN inline GBox primBox(const float *verts, const uint32_t *tris, const uint32_t *tbase, const uint32_t *vbase,
+		uint32_t nMeshes, uint32_t i) {
+	const uint32_t m = meshOf(tbase, nMeshes, i);
+	float p[3][3];
+	loadTri(verts, tris, vbase, m, i, p);
+	GBox b;
+	for (int k = 0; k < 3; ++k) {
+		b.lo[k] = minT(p[0][k], p[1][k]);
+		b.hi[k] = maxT(p[0][k], p[1][k]);
+	}
+	for (int k = 0; k < 3; ++k) {
+		b.lo[k] = minT(b.lo[k], p[2][k]);
+		b.hi[k] = maxT(b.hi[k], p[2][k]);
+	}
+	const float ea = maxT(lrhdev::machineEps(b.lo[0]), maxT(lrhdev::machineEps(b.lo[1]), lrhdev::machineEps(b.lo[2])));
+	const float ec = maxT(lrhdev::machineEps(b.hi[0]), maxT(lrhdev::machineEps(b.hi[1]), lrhdev::machineEps(b.hi[2])));
+	const float e = maxT(ea, ec);
+	for (int k = 0; k < 3; ++k) {
+		b.lo[k] -= e;
+		b.hi[k] += e;
+	}
+	return b;
+}
+
+LRH_DN inline uint64_t spread21(uint32_t v) {
+	uint64_t x = v & 0x1fffffu;
+	x = (x | (x << 32)) & 0x1f00000000ffffull;
+	x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+	x = (x | (x << 8)) & 0x100f00f00f00f00full;
+	x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+	x = (x | (x << 2)) & 0x1249249249249249ull;
+	return x;
+}
+
+// 2. Morton code of the box centre within the centre bounds (ordered keys cb:
+// min x y z, max x y z)
+LRH_DN inline uint64_t mortonKey(const GBox &b, const uint32_t cb[6]) {
+	uint32_t q[3];
+	for (int k = 0; k < 3; ++k) {
+		const float lo = lrhdev::unkey(cb[k]), hi = lrhdev::unkey(cb[3 + k]);
+		const float ext = hi - lo;
+		const float u = ext > 0.f ? ((b.lo[k] + b.hi[k]) - lo) / ext : 0.f;
+		float s = u * 2097152.f;
+		s = s < 0.f ? 0.f : (s > 2097151.f ? 2097151.f : s);
+		q[k] = (uint32_t)s;
+	}
+	return (spread21(q[0]) << 2) | (spread21(q[1]) << 1) | spread21(q[2]);
+}
+
+LRH_DN inline int clz64(uint64_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+	return __clzll((long long)x);
+#else
+	return x ? __builtin_clzll(x) : 64;
+#endif
+}
+LRH_DN inline int clz32(uint32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+	return __clz((int)x);
+#else
+	return x ? __builtin_clz(x) : 32;
+#endif
+}
+
+// common prefix length of keys i and j (positions break ties); -1 outside [0, n)
+LRH_DN inline int delta(const uint64_t *k, int64_t n, int64_t i, int64_t j) {
+	if (j < 0 || j >= n) return -1;
+	const uint64_t a = k[i], b = k[j];
+	if (a != b) return clz64(a ^ b);
+	return 64 + clz32((uint32_t)i ^ (uint32_t)j);
+}
+
+// 3. internal node i of the radix tree: children lc / rc (kLeafBit | leaf, or
+// an internal index) over the key range [lo, hi].  With keys in order the
+// range search finds at least one more key and each loop ends within 33
+// steps; false when the keys around i are not in order.
+LRH_DN inline bool radixNode(const uint64_t *keys, int64_t n, int64_t i, uint32_t &lc, uint32_t &rc, uint32_t &lo,
+		uint32_t &hi) {
+	const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+	const int dmin = delta(keys, n, i, i - d);
+	int64_t lmax = 2;
+	for (int steps = 0; delta(keys, n, i, i + lmax * d) > dmin; ++steps) {
+		if (steps >= 33) return false;
+		lmax *= 2;
+	}
+	int64_t l = 0;
+	for (int64_t t = lmax / 2; t >= 1; t /= 2)
+		if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+	if (l < 1) return false;  // no key beyond i shares more than dmin bits: out of order
+	const int64_t j = i + l * d;
+	const int dnode = delta(keys, n, i, j);
+	int64_t s = 0;
+	// split: t = ceil(l/2), ceil(t/2), ..., 1
+	for (int64_t t = (l + 1) / 2, steps = 0;; t = (t + 1) / 2, ++steps) {
+		if (steps >= 40) return false;
+		if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+		if (t <= 1) break;
+	}
+	const int64_t g = i + s * d + (d < 0 ? -1 : 0);
+	const int64_t a = i < j ? i : j, b = i < j ? j : i;
+	if (g < a || g + 1 > b) return false;
+	lc = (a == g) ? (kLeafBit | (uint32_t)g) : (uint32_t)g;
+	rc = (b == g + 1) ? (kLeafBit | (uint32_t)(g + 1)) : (uint32_t)(g + 1);
+	lo = (uint32_t)a;
+	hi = (uint32_t)b;
+	return true;
+}
+
+LRH_DN inline uint32_t countOf(uint32_t c, const uint32_t *first, const uint32_t *last) {
+	return (c & kLeafBit) ? 1u : last[c] - first[c] + 1u;
+}
+
+LRH_DN inline float areaOf(const GBox &b) {
+	const float dx = b.hi[0] - b.lo[0], dy = b.hi[1] - b.lo[1], dz = b.hi[2] - b.lo[2];
+	if (!(dx >= 0.f)) return 0.f;
+	return 2.f * (dx * dy + dy * dz + dz * dx);
+}
+
+// 4a. the children of internal node b (Morton order): its two, then the
+// largest-area child above maxLeaf triangles split until there are four;
+// returns how many of them become inner nodes
+LRH_DN inline uint32_t expandNode(uint32_t b, const uint32_t *left, const uint32_t *right, const uint32_t *first,
+		const uint32_t *last, const GBox *nodeBox, uint32_t maxLeaf, uint32_t kids[4]) {
+	uint32_t ch[4] = {left[b], right[b], kNone, kNone};
+	int nch = 2;
+	while (nch < 4) {
+		int pick = -1;
+		float best = -1.f;
+		for (int k = 0; k < nch; ++k) {
+			if ((ch[k] & kLeafBit) || countOf(ch[k], first, last) <= maxLeaf) continue;
+			const float a = areaOf(nodeBox[ch[k]]);
+			if (a > best) {
+				best = a;
+				pick = k;
+			}
+		}
+		if (pick < 0) break;
+		const uint32_t c = ch[pick];
+		for (int k = nch; k > pick + 1; --k) ch[k] = ch[k - 1];
+		ch[pick] = left[c];
+		ch[pick + 1] = right[c];
+		++nch;
+	}
+	uint32_t inner = 0;
+	for (int k = 0; k < 4; ++k) {
+		kids[k] = ch[k];
+		if (ch[k] != kNone && countOf(ch[k], first, last) > maxLeaf) ++inner;
+	}
+	return inner;
+}
+
+LRH_DN inline uint32_t leafRef(uint32_t start, uint32_t count) {
+	return LRH_CHILD_LEAF | ((count - 1u) << LRH_LEAF_START_BITS) | start;
+}
+
+// 4b. the wide node of a frontier entry: leaf children reference their key
+// range, inner children the slots innerOff, innerOff+1, ... after the level
+// (their binary ids go to next[slot])
+LRH_DN inline void emitNode(const uint32_t kids[4], uint32_t innerOff, const uint32_t *first, const uint32_t *last,
+		const GBox *leafBox, const GBox *nodeBox, uint32_t maxLeaf, uint32_t childBase, uint32_t layout, void *nodes,
+		uint32_t nodeIndex, uint32_t *next) {
+	GBox cb[4];
+	uint32_t ref[4] = {LRH_CHILD_EMPTY, LRH_CHILD_EMPTY, LRH_CHILD_EMPTY, LRH_CHILD_EMPTY};
+	int n = 0;
+	uint32_t slot = innerOff;
+	for (int k = 0; k < 4; ++k) {
+		const uint32_t c = kids[k];
+		if (c == kNone) continue;
+		cb[n] = (c & kLeafBit) ? leafBox[c & ~kLeafBit] : nodeBox[c];
+		const uint32_t cnt = countOf(c, first, last);
+		if (cnt <= maxLeaf) {
+			const uint32_t start = (c & kLeafBit) ? (c & ~kLeafBit) : first[c];
+			ref[n] = leafRef(start, cnt);
+		} else {
+			ref[n] = childBase + slot;
+			next[slot] = c;
+			++slot;
+		}
+		++n;
+	}
+	if (layout == 0)
+		lrhdev::quantizeNode(cb, n, ref, reinterpret_cast<lrh_node4q *>(nodes) + nodeIndex);
+	else
+		lrhdev::floatNode(cb, n, ref, reinterpret_cast<lrh_node4f *>(nodes) + nodeIndex);
+}
+
+// 5. triangle record of global triangle i, as the host builder writes it
+LRH_DN inline lrh_tri triRecord(const float *verts, const uint32_t *tris, const uint32_t *tbase, const uint32_t *vbase,
+		uint32_t nMeshes, uint32_t i) {
+	const uint32_t m = meshOf(tbase, nMeshes, i);
+	float p[3][3];
+	loadTri(verts, tris, vbase, m, i, p);
+	lrh_tri r;
+	for (int k = 0; k < 3; ++k) {
+		r.p0[k] = p[0][k];
+		r.e1[k] = p[1][k] - p[0][k];
+		r.e2[k] = p[2][k] - p[0][k];
+	}
+	r.triIndex = i - tbase[m];
+	r.meshIndex = m;
+	r.pad = 0;
+	return r;
+}
+
+LRH_DN inline uint32_t digitOf(uint64_t key, uint32_t shift) { return (uint32_t)(key >> shift) & 0xffu; }
+
+}  // namespace lrhlbvh
+
+#endif
