@@ -133,6 +133,124 @@ def cpu_baseline(meshes, rays, target_s):
             "host": host_info()}
 
 
+def two_level_bytes(c):
+    """Algorithmic bytes per ray of a two-level trace from the counting kernel:
+    Ray + RayHit, 64-byte node visits, 48-byte triangle records, and per
+    object entry its 16-byte instance record plus the 64-byte matrix (static
+    instance) or the 32-byte motion record and the 112 bytes of the segment
+    a sample reads (motion)."""
+    rays = max(c["rays"], 1)
+    tris = c["prim_tests"] - c["object_entries"]
+    b = (64 * c["node_visits"] + 48 * tris + 16 * c["object_entries"] + 64 * c["matrix_entries"]
+         + (32 + 112) * c["motion_entries"])
+    return RAY_BYTES + HIT_BYTES + b / rays
+
+
+def _rate_line(n, wall, lat, steps, b_ray):
+    kernel_ms = float(lat.mean())
+    achieved = b_ray * n / (kernel_ms * 1e-3) / 1e9
+    return {"mrays": round(n * steps / wall / 1e6, 1), "kernel_ms": round(kernel_ms, 4),
+            "bytes_per_ray": round(b_ray, 1), "achieved_gbs": round(achieved, 1),
+            "frac": round(achieved / HBM_PEAK_GBS, 4)}
+
+
+def extra_workloads(args, dev):
+    """The other configurations in the same run, so their numbers are the
+    driver's measurement too: the two-level instance/motion scene (closest
+    and any hit), the batched workload (64 buffers over 8 resident scenes in
+    one graph replay) and the served workload (continuous batching of
+    512x512 path-tracing sessions, next to one plain batch of the same rays).
+    Each rate comes with its bytes per ray from the counting kernel and its
+    fraction of the 8 TB/s roof."""
+    from luxcore_amd import batch as B
+    from luxcore_amd import serve
+    steps = args.extra_steps
+    out = {}
+    # -- two level: 1,000 instances of a 50,000-triangle sphere, 200 with 2-key motion
+    t0 = time.time()
+    isc = scenes.instanced_scene()
+    sc = dev.scene_mbvh(isc)
+    rays = scenes.instanced_rays(1 << 22)
+    n = rays.shape[0]
+    rb = dev.upload(rays)
+    hb = dev.alloc(n * HIT_DTYPE.itemsize)
+    tl = {"workload": "1,000 instances of a 50,000-tri displaced sphere (T*R*S, scale 0.5-2, 200^3 box), 200 "
+                      "with 2-key motion; 4,194,304 rays uniform in the box, time ~ U[0,1]", "rays": n}
+    for name, q in (("closest", 0), ("any", 1)):
+        c = dev.trace_counters(sc, rb, hb, n, q)
+        wall, lat = _event_timed(dev, lambda: dev.trace(sc, rb, hb, n, q), steps, 3)
+        tl[name] = _rate_line(n, wall, lat, steps, two_level_bytes(c))
+        tl[name]["per_ray"] = {k: round(c[k] / max(c["rays"], 1), 3)
+                               for k in ("node_visits", "prim_tests", "object_entries", "motion_entries")}
+    out["two_level"] = tl
+    log(f"two level ({time.time() - t0:.1f}s): {tl}")
+    rb.free()
+    hb.free()
+    sc.close()
+    # -- batched: 8 resident scenes (10k..5M tris), 64 buffers, one persistent launch replayed from a graph
+    t0 = time.time()
+    meshes = scenes.batch_scenes()
+    bufs = scenes.batch_buffers()
+    wl = B.BatchedWorkload(dev, meshes, bufs)
+    items = wl.items()
+    total = wl.total_rays()
+    key = B.signature(dev, [m[0].ntris for m in meshes], [c for _, c, _ in bufs], "batched-v2")
+    opts, _ = B.autotune(dev, items, key=key)
+    cnt = {"node_visits": 0, "prim_tests": 0, "rays": 0}
+    for (s, r, h, cn, q) in items:
+        c = dev.trace_counters(s, r, h, cn, q, opts)
+        for k in cnt:
+            cnt[k] += c[k]
+    b_ray = RAY_BYTES + HIT_BYTES + (64 * cnt["node_visits"] + 48 * cnt["prim_tests"]) / max(cnt["rays"], 1)
+    bt = dev.batch(items, opts)
+    bt.capture()
+    wall, lat = _event_timed(dev, bt.replay, steps, 2)
+    bl = {"workload": "8 scenes 10k..5M tris (soups + sphere grids), 64 buffers log-uniform 4Ki..4Mi rays, "
+                      "mixed closest/any", "rays": total,
+          "shape": {"block": opts.blockSize, "lds": opts.ldsStack, "refill": opts.refillThreshold}}
+    bl.update(_rate_line(total, wall, lat, steps, b_ray))
+    bl["per_ray"] = {"node_visits": round(cnt["node_visits"] / max(cnt["rays"], 1), 3),
+                     "tri_tests": round(cnt["prim_tests"] / max(cnt["rays"], 1), 3)}
+    pwall, _ = _event_timed(dev, lambda: [dev.trace(s, r, h, cn, q, opts) for (s, r, h, cn, q) in items], steps, 1)
+    bl["plain_per_buffer_mrays"] = round(total * steps / pwall / 1e6, 1)
+    bt.close()
+    out["batched"] = bl
+    log(f"batched ({time.time() - t0:.1f}s): {bl}")
+    # -- served: Poisson arrivals of 512x512 sessions over the same 8 scenes
+    t0 = time.time()
+    scs = wl.scenes
+    for rbuf in wl.ray_bufs + wl.hit_bufs:
+        rbuf.free()
+    sopts = D.trace_opts(256, 8, 0, 12)
+    w = serve.Server(dev, scs, opts=sopts)
+    w.run(serve.poisson_specs(1000.0, 0.004, len(scs), seed=99))
+    w.close()
+    specs = serve.poisson_specs(600.0, 0.5, len(scs), seed=600)
+    srv = serve.Server(dev, scs, opts=sopts, max_rays_per_tick=3 << 19)
+    swall = srv.run(specs, time_limit=120.0)
+    row = serve.served_row(srv, swall, 600.0, 3 << 19)
+    srv.close()
+    plain = serve.plain_comparison(dev, scs, sopts, n_sessions=16)
+    out["served"] = {
+        "workload": "Poisson arrivals (600 sessions/s for 0.5 s) over the 8 batched scenes; 512x512 camera + <=8 "
+                    "bounces (cosine, RR 0.7 after 2) + 1 shadow ray per bounce, GPU shading (lrh_shade), 2 "
+                    "cohorts, 1.5Mi rays per tick, 4 GiB HBM budget",
+        "run": row,
+        "latency_ms": {"p50": row["p50_ms"], "p99": row["p99_ms"], "samples": row["wavefronts"],
+                       "per": "one session wavefront: ready -> completion of the tick that traced it"},
+        "busy_mrays": row["busy_mrays"], "trace_mrays": row["trace_mrays"],
+        "plain_single_batch": plain,
+        "trace_over_plain_batch": round(row["trace_mrays"] / plain["plain_batch_mrays"], 3),
+        "busy_over_plain_batch": round(row["busy_mrays"] / plain["plain_batch_mrays"], 3),
+        "note": "busy = GPU time of the ticks (batched trace + shading kernels); trace = the batched traces alone; "
+                "plain_single_batch = the first 16 sessions' wavefronts captured and traced as ONE lrh_trace_batch "
+                "(and as one lrh_trace each); live rays only"}
+    log(f"served ({time.time() - t0:.1f}s): {out['served']}")
+    for s in scs:
+        s.close()
+    return out
+
+
 def run_batched(args, dev):
     """64 ray buffers (log-uniform 4Ki..4Mi rays, mixed closest/any) over 8
     resident scenes (10k..5M tris): plain per-buffer traces vs one persistent
@@ -492,6 +610,9 @@ def main():
     ap.add_argument("--refill", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="headline only: skip the two-level, batched and served workloads of the JSON line")
+    ap.add_argument("--extra-steps", type=int, default=20, help="timed traces per extra workload")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_heightfield_bounce.json"),
                     help="rocprofv3 FETCH_SIZE/WRITE_SIZE measurement of this workload (bytes per ray)")
     ap.add_argument("--calib-copy", type=int, default=0,
@@ -558,8 +679,15 @@ def main():
     value = n * args.steps / wall / 1e6
     kernel_ms = float(lat.mean())
     achieved = b_ray * n / (kernel_ms * 1e-3) / 1e9
-    log(f"{value:.1f} Mrays/s wall, kernel {kernel_ms:.3f} ms avg, p50 {np.percentile(lat, 50):.3f} "
-        f"p99 {np.percentile(lat, 99):.3f} ms, hit rate {(hits['meshIndex'] != NULL_INDEX).mean():.3f}")
+    # wavefront latency percentiles over at least 1,000 wavefronts, whatever --steps is
+    n_lat = max(1000, args.steps)
+    if n_lat > args.steps:
+        _, lat_p = _event_timed(dev, lambda: dev.trace(scene, rb, hb, n, 0, use_opts), n_lat, 0)
+    else:
+        lat_p = lat
+    log(f"{value:.1f} Mrays/s wall, kernel {kernel_ms:.3f} ms avg; over {lat_p.size} wavefronts p50 "
+        f"{np.percentile(lat_p, 50):.3f} p99 {np.percentile(lat_p, 99):.3f} ms, hit rate "
+        f"{(hits['meshIndex'] != NULL_INDEX).mean():.3f}")
 
     if args.calib_copy:
         nb = args.calib_copy << 20
@@ -573,11 +701,16 @@ def main():
         cpu = cpu_baseline(meshes, rays, args.cpu_seconds)
         log(f"cpu baseline {cpu['value']:.3f} Mrays/s on {cpu['cores']} threads")
 
-    traffic = None
+    traffic, traffic_source = None, None
     if args.traffic_file and os.path.exists(args.traffic_file) and args.cells == 3240 and args.layout == 0:
         with open(args.traffic_file) as f:
             tr = json.load(f)
         traffic = round(tr["traffic_bytes_per_ray"] * n)
+        traffic_source = (f"not measured in this run: {os.path.relpath(args.traffic_file, ROOT)} "
+                          f"(rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel on this workload, scaled by a copy "
+                          f"kernel of known size) x the rays of this run")
+
+    extra = {} if args.no_extra else extra_workloads(args, dev)
 
     out = {
         "metric": METRIC,
@@ -597,11 +730,12 @@ def main():
                    "triangles": ntris, "rays_per_step": int(n), "node_layout": "bvh4 64B quantized"
                    if args.layout == 0 else "bvh4 128B fp32", "max_leaf": args.max_leaf,
                    "parallelism": "single GPU"},
-        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
-                       "p99": round(float(np.percentile(lat, 99)), 4),
+        "latency_ms": {"p50": round(float(np.percentile(lat_p, 50)), 4),
+                       "p99": round(float(np.percentile(lat_p, 99)), 4), "samples": int(lat_p.size),
                        "per": "one wavefront = one lrh_trace of the bounce buffer"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": traffic_source,
                      "bytes_per_ray": round(b_ray, 1),
                      "per_ray": {"node_visits": round(nodes / counted, 3),
                                  "tri_tests": round(prims / counted, 3)},
@@ -612,6 +746,7 @@ def main():
     }
     if cpu:
         out["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    out.update(extra)
     print(json.dumps(out), flush=True)
 
 

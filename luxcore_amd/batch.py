@@ -2,13 +2,14 @@
 persistent launch (optionally a captured hipGraph), and the launch-shape
 autotuner.
 
-The autotuner times candidate launch shapes (block size x LDS stack depth x
-refill threshold) of the batched kernel on the actual batch and keeps the
-fastest; results are cached on disk per (device, workload signature) so a
-serving process tunes once.  Node layout and leaf size are build-time choices:
+The autotuner (lrh_autotune_batch, in the library) times candidate launch
+shapes (block size x LDS stack depth x refill threshold) of the batched
+kernel on the actual batch and keeps the fastest; results are cached on disk
+per (device, workload signature) so a serving process tunes once.  Node layout and leaf size are build-time choices:
 ``tune_build`` rebuilds the scenes for each candidate and times the best
 launch shape on each.
 """
+import ctypes
 import hashlib
 import itertools
 import json
@@ -19,8 +20,9 @@ import numpy as np
 
 from . import device as D
 
-DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "lrhip", "autotune.json")
+DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "lrhip", "autotune.txt")
 
+# the library's built-in candidates (LRH_AUTOTUNE_SHAPES of them), in its order
 SHAPES = [(b, l, r) for b, l, r in itertools.product((128, 256), (4, 8, 16), (4, 8, 12, 16, 24))]
 
 
@@ -31,22 +33,6 @@ def signature(dev, scenes_info, counts, extra=""):
     h.update(np.asarray(counts, np.int64).tobytes())
     h.update(extra.encode())
     return h.hexdigest()[:16]
-
-
-def _load_cache(path):
-    try:
-        with open(path) as f:
-            return json.load(f)
-    except (OSError, ValueError):
-        return {}
-
-
-def _store_cache(path, cache):
-    os.makedirs(os.path.dirname(path), exist_ok=True)
-    tmp = path + ".tmp"
-    with open(tmp, "w") as f:
-        json.dump(cache, f, indent=1)
-    os.replace(tmp, path)
 
 
 def time_batch(dev, items, opts, reps=5):
@@ -66,32 +52,33 @@ def time_batch(dev, items, opts, reps=5):
     return ms
 
 
-def autotune(dev, items, shapes=SHAPES, reps=3, cache_path=DEFAULT_CACHE, key=None, log=None):
-    """Best lrh_trace_opts for this batch; (opts, table of results)."""
-    cache = _load_cache(cache_path) if cache_path else {}
-    if key and key in cache:
-        c = cache[key]
-        return D.trace_opts(c["block"], c["lds"], 0, c["refill"]), c.get("table", [])
-    table = []
-    for block, lds, refill in shapes:
-        opts = D.trace_opts(block, lds, 0, refill)
-        try:
-            ms = time_batch(dev, items, opts, reps)
-        except D.LrhError as e:  # a shape the device refuses (LDS limit) is skipped
-            if log:
-                log(f"autotune: block {block} lds {lds} refill {refill}: {e}")
-            continue
-        table.append({"block": block, "lds": lds, "refill": refill, "ms": ms})
-        if log:
-            log(f"autotune: block {block} lds {lds} refill {refill}: {ms:.3f} ms")
-    best = min(table, key=lambda r: r["ms"])
-    if key and cache_path:
-        cache[key] = dict(best, table=table, when=time.time())
-        _store_cache(cache_path, cache)
-    return D.trace_opts(best["block"], best["lds"], 0, best["refill"]), table
+def autotune(dev, items, shapes=None, cache_path=DEFAULT_CACHE, key=None, log=None):
+    """Best lrh_trace_opts for this batch from the library's autotuner
+    (lrh_autotune_batch); returns (opts, table of per-shape ms; empty when the
+    answer came from the cache)."""
+    cand = shapes if shapes is not None else SHAPES
+    arr = D._batch_items(items)
+    carr = (D.LrhTraceOpts * len(cand))()
+    for i, (b, l, r) in enumerate(cand):
+        carr[i] = D.trace_opts(b, l, 0, r)
+    best = D.LrhTraceOpts()
+    best_ms = ctypes.c_float()
+    ms = (ctypes.c_float * len(cand))()
+    if cache_path:
+        os.makedirs(os.path.dirname(os.path.abspath(cache_path)), exist_ok=True)
+    D.check(D.lib().lrh_autotune_batch(dev.h, arr, len(items), carr if shapes is not None else None,
+                                       len(cand) if shapes is not None else 0,
+                                       cache_path.encode() if cache_path else None, key.encode() if key else None,
+                                       ctypes.byref(best), ctypes.byref(best_ms), ms))
+    table = [{"block": b, "lds": l, "refill": r, "ms": float(ms[i])} for i, (b, l, r) in enumerate(cand)
+             if not np.isnan(ms[i])]
+    if log:
+        for row in table:
+            log(f"autotune: {row}")
+    return D.trace_opts(best.blockSize, best.ldsStack, 0, best.refillThreshold), table
 
 
-def tune_build(dev, scene_meshes, make_items, builds=((0, 4), (0, 2), (1, 4)), shapes=None, reps=3, log=None):
+def tune_build(dev, scene_meshes, make_items, builds=((0, 4), (0, 2), (1, 4)), shapes=None, log=None):
     """Times each (layout, max_leaf) build with its best launch shape;
     make_items(scenes) -> batch items.  Returns (best build, results)."""
     shapes = shapes or [(256, 8, 8), (256, 8, 12), (256, 4, 12), (128, 8, 12)]
@@ -99,7 +86,7 @@ def tune_build(dev, scene_meshes, make_items, builds=((0, 4), (0, 2), (1, 4)), s
     for layout, leaf in builds:
         sc = [dev.scene_bvh(m, layout=layout, max_leaf=leaf) for m in scene_meshes]
         items = make_items(sc)
-        opts, table = autotune(dev, items, shapes=shapes, reps=reps, cache_path=None)
+        opts, table = autotune(dev, items, shapes=shapes, cache_path=None)
         best = min(table, key=lambda r: r["ms"])
         res.append({"layout": layout, "max_leaf": leaf, **best})
         if log:

@@ -1,35 +1,20 @@
 """Served mode: continuous batching of path-tracing sessions.
 
+A thin driver over the library's C++ server (lrh_server_* in
+include/lrhip.h): the tick loop, admission by HBM budget, the batched
+traces, the GPU shading steps and the latency bookkeeping all run in
+liblrhip.so.  This module builds the session specs (Poisson arrivals),
+reads the results back, and holds the served benchmark.
+
 A session is a small path tracer bound to one resident scene: 512x512 camera
 rays, then up to ``max_bounces`` bounce wavefronts and one shadow ray per
-hit, traced as any-hit.  With ``shading`` (the default) each wavefront is
-shaded on the GPU by lrh_shade: grey Lambertian surfaces (kd 0.6) lit by a
-square area light above the scene through next-event estimation, cosine
-next vertices, Russian roulette from the third vertex with survival 0.7,
-radiance gathered per pixel; without it lrh_rays_bounce draws the same kind
-of directions without materials.  Each session has one wavefront in flight
-at a time.
-
-The server keeps the scenes and every admitted session's ray/hit buffers in
-HBM.  Each tick it admits queued sessions while the HBM budget and the
-per-tick ray budget allow, gathers the ready wavefront of every live session
-(closest-hit rays + the shadow rays of the previous hits) into ONE batched
-persistent launch (lrh_trace_batch), then generates every session's next
-wavefront on the GPU (lrh_shade or lrh_rays_bounce) and retires finished sessions, so
-sessions that arrive while others are on the GPU join at the next tick.
-
-Ticks are pipelined: the host needs no result of a tick to build the next
-one (a session's next wavefront is generated on the GPU and its step count is
-known on the host), so it enqueues up to `depth` ticks ahead and the GPU runs
-them back to back; a tick's completion is taken from its event.
-
-Per-wavefront latency = completion of the tick that traced it minus the
-moment it became ready (admission for a camera wavefront, the previous
-tick's completion otherwise).  Throughput counts live (unmasked) rays only;
-busy time is the GPU time of the ticks (events around each tick).
+path vertex, traced as any-hit.  With ``shading`` (the default) each
+wavefront is shaded on the GPU by lrh_shade (grey Lambertian surfaces lit by
+a square area light above the scene through next-event estimation, cosine
+next vertices, Russian roulette after bounce ``rr_start``); without it
+lrh_rays_bounce draws the same kind of directions without materials.
 """
-import collections
-import time
+import ctypes
 from dataclasses import dataclass
 
 import numpy as np
@@ -45,312 +30,126 @@ class SessionSpec:
     sid: int
     scene: int
     seed: int
-    arrival: float = 0.0
+    arrival: float = 0.0   # seconds after submission
 
 
-class Session:
-    """Device buffers and step counter of one path-tracing session."""
-
-    def __init__(self, server, spec, cohort=0):
-        self.spec = spec
-        self.server = server
-        self.cohort = cohort
-        self.stream = server.streams[cohort]
-        dev = server.dev
-        n = server.width * server.height
-        self.n = n
-        self.bufs = server._take_buffers()
-        self.rays = [self.bufs[0], self.bufs[1]]
-        self.shadow, self.hits, self.shadow_hits, self.paths = self.bufs[2], self.bufs[3], self.bufs[4], self.bufs[5]
-        self.radiance = None  # per-pixel radiance, kept for recorded sessions
-        self.cur = 0
-        self.step = 0
-        self.ready = None
-        self.admitted = None
-        self.recorded = []  # per step: (closest hits, shadow hits) when recording
-        rng = np.random.Generator(np.random.PCG64(spec.seed))
-        ext = server.extent
-        eye = rng.uniform(0.1 * ext, 0.9 * ext, 3)
-        eye[2] = ext * rng.uniform(0.6, 0.95)
-        look = np.array([ext / 2, ext / 2, ext * 0.3]) + rng.uniform(-0.1 * ext, 0.1 * ext, 3)
-        dev.camera_rays(self.rays[0], eye, look, (0.0, 0.0, 1.0), 50.0, server.width, server.height,
-                        stream=self.stream)
-        if server.shading:
-            dev.paths_init(self.paths, n, stream=self.stream)
-
-    @staticmethod
-    def bytes(width, height):
-        n = width * height
-        return 3 * n * RAY_B + 2 * n * HIT_B + n * D.PATH_DTYPE.itemsize
-
-    def items(self):
-        """Batch items of the current wavefront."""
-        sc = self.server.scenes[self.spec.scene]
-        last = self.server.max_bounces
-        out = []
-        if self.step <= last:
-            out.append((sc, self.rays[self.cur], self.hits, self.n, D.LRH_QUERY_CLOSEST))
-        if self.step >= 1:
-            out.append((sc, self.shadow, self.shadow_hits, self.n, D.LRH_QUERY_ANY))
-        return out
-
-    def advance(self):
-        """After the tick: generate the next wavefront on the GPU. Returns
-        False when the session is finished."""
-        srv = self.server
-        last = srv.max_bounces
-        sc = srv.scenes[self.spec.scene]
-        if srv.shading:
-            # vertex `step` (its shadow rays' hits come from the previous
-            # vertex); one more step after the last vertex only gathers the
-            # light its shadow rays found
-            nxt = 1 - self.cur
-            mats, nm = srv.materials[self.spec.scene]
-            srv.dev.shade(sc, mats, nm, srv.lights_buf, srv.n_lights, self.rays[self.cur], self.hits,
-                          self.shadow_hits if self.step >= 1 else None, self.paths, self.rays[nxt], self.shadow,
-                          self.n, max_depth=last + 1, rr_depth=srv.rr_start + 1, rr_survive=srv.rr_survive,
-                          seed=self.spec.seed, live_counter=srv.live_ptr, stream=self.stream)
-            self.cur = nxt
-        elif self.step <= last:
-            nxt = 1 - self.cur
-            srv.dev.bounce_rays(sc, self.rays[self.cur], self.hits, self.n, self.step,
-                                self.spec.seed, self.rays[nxt], self.shadow, 0, srv.rr_start, srv.rr_survive,
-                                srv.light, srv.live_ptr, stream=self.stream)
-            self.cur = nxt
-        self.step += 1
-        return self.step <= last + 1
-
-    def release(self):
-        """Hand the buffers back to the server's pool (freeing would wait for
-        the whole device, including the other cohorts' work)."""
-        self.server.pool.append(self.bufs)
-        self.bufs = None
+def session_bytes(width, height):
+    """Device bytes one admitted session holds (the admission unit)."""
+    n = width * height
+    return 3 * n * RAY_B + 2 * n * HIT_B + n * D.PATH_DTYPE.itemsize
 
 
 class Server:
-    """Continuous batching of sessions.  Sessions are split into `cohorts`,
-    each with its own stream; the host serves the cohorts in turn, so while
-    one cohort's tick drains on the GPU the next cohort's tick is already
-    running, and a new session joins the least loaded cohort at its next
-    tick."""
+    """lrh_server over resident scenes; record: session ids whose every
+    wavefront (rays + hits) and final radiance are kept for checking;
+    capture_bytes: device arena for lrh_server_capture (0 = off)."""
 
     def __init__(self, dev, scenes, width=512, height=512, max_bounces=8, rr_start=2, rr_survive=0.7,
-                 hbm_budget=4 << 30, max_rays_per_tick=3 << 20, opts=None, extent=100.0,
-                 light=(50.0, 50.0, 150.0), record=(), cohorts=2, shading=True):
+                 hbm_budget=4 << 30, max_rays_per_tick=3 << 20, opts=None, extent=100.0, light=None, record=(),
+                 cohorts=2, shading=True, capture_bytes=0):
         self.dev = dev
-        self.shading = shading
-        self.scenes = scenes
+        self.scenes = list(scenes)  # keeps the scenes alive
         self.width, self.height = width, height
+        self.n = width * height
         self.max_bounces = max_bounces
-        self.rr_start, self.rr_survive = rr_start, rr_survive
-        self.hbm_budget = hbm_budget
-        self.max_rays_per_tick = max_rays_per_tick
-        self.opts = opts
-        self.extent = extent
-        self.light = light
+        c = D.LrhServerConfig()
+        c.width, c.height, c.maxBounces, c.rrStart, c.rrSurvive = width, height, max_bounces, rr_start, rr_survive
+        c.hbmBudget, c.maxRaysPerTick, c.cohorts = hbm_budget, max_rays_per_tick, cohorts
+        c.mode = D.SERVE_SHADE if shading else D.SERVE_BOUNCE
+        c.extent = extent
+        if light is not None:
+            c.light[:] = list(light)
+        if opts is not None:
+            c.opts = opts
+        arr = (ctypes.c_void_p * len(self.scenes))(*[s.h for s in self.scenes])
+        p = ctypes.c_void_p()
+        D.check(D.lib().lrh_server_create(dev.h, arr, len(self.scenes), ctypes.byref(c), ctypes.byref(p)))
+        self.h = p.value
         self.record = set(record)
-        ncoh = 1 if self.record else max(1, cohorts)
-        self.streams = [None] + [dev.stream() for _ in range(ncoh - 1)]
-        self.cohorts = [[] for _ in range(ncoh)]
-        self.inflight = [None] * ncoh  # per cohort: (start event, end event, sessions traced, sessions finished)
-        self.turn = 0
-        self.queue = collections.deque()
-        self.done = []
-        self.latencies = []
-        self.queue_waits = []
-        self.ticks = 0
-        self.intervals = []        # GPU (start, end) of each tick, ms after the reference event
-        self.pool = []             # free session buffer sets
-        self.allocated = 0         # session buffer sets allocated
-        self.counter = dev.alloc(8)
-        self.live_ptr = self.counter.ptr
-        dev.upload(np.zeros(2, np.uint32), self.counter)
-        self.session_bytes = Session.bytes(width, height)
-        if shading:
-            # grey Lambertian surfaces; a square light (two triangles, 0.3 x extent wide,
-            # radiance 30) facing down from above the scene at light[2]
-            # (one table for every scene: entry i is mesh i of whichever scene)
-            m = np.zeros(1024, D.MATERIAL_DTYPE)
-            m["kd"] = 0.6
-            table = (dev.upload(m), len(m))
-            self.materials = [table] * len(scenes)
-            c, h, w = np.array(light[:2], np.float32), np.float32(light[2]), np.float32(0.15 * extent)
-            q = np.array([[c[0] - w, c[1] - w, h], [c[0] + w, c[1] - w, h], [c[0] + w, c[1] + w, h],
-                          [c[0] - w, c[1] + w, h]], np.float32)
-            lt = np.zeros(2, D.LIGHT_DTYPE)
-            for k, (a, b, d) in enumerate(((0, 2, 1), (0, 3, 2))):
-                lt[k]["p0"], lt[k]["e1"], lt[k]["e2"] = q[a], q[b] - q[a], q[d] - q[a]
-                lt[k]["le"] = 30.0
-            self.lights_buf, self.n_lights = dev.upload(lt), 2
-        self.max_active_seen = 0
-        dev.sync()
-        self.ref = dev.event()
-        self.ref.record()
+        for sid in self.record:
+            D.check(D.lib().lrh_server_record(self.h, sid))
+        if capture_bytes:
+            D.check(D.lib().lrh_server_capture(self.h, capture_bytes))
 
-    @property
-    def active(self):
-        return [s for c in self.cohorts for s in c]
+    def submit(self, specs):
+        arr = (D.LrhSessionSpec * max(1, len(specs)))()
+        for i, sp in enumerate(specs):
+            arr[i].sid, arr[i].scene, arr[i].seed, arr[i].arrival = sp.sid, sp.scene, sp.seed, sp.arrival
+        D.check(D.lib().lrh_server_submit(self.h, arr, len(specs)))
 
-    def _take_buffers(self):
-        if self.pool:
-            return self.pool.pop()
-        n = self.width * self.height
-        d = self.dev
-        self.allocated += 1
-        return (d.alloc(n * RAY_B), d.alloc(n * RAY_B), d.alloc(n * RAY_B), d.alloc(n * HIT_B), d.alloc(n * HIT_B),
-                d.alloc(n * D.PATH_DTYPE.itemsize))
+    def run(self, specs=(), time_limit=60.0):
+        """Submit specs (arrivals relative to now) and tick until every
+        session has finished; returns the wall seconds."""
+        if specs:
+            self.submit(list(specs))
+        wall = ctypes.c_double()
+        D.check(D.lib().lrh_server_run(self.h, time_limit, ctypes.byref(wall)))
+        return wall.value
+
+    def tick(self):
+        t = ctypes.c_int()
+        D.check(D.lib().lrh_server_tick(self.h, ctypes.byref(t)))
+        return bool(t.value)
+
+    def stats(self):
+        st = D.LrhServerStats()
+        D.check(D.lib().lrh_server_get_stats(self.h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in D.LrhServerStats._fields_}
+
+    def latencies_ms(self):
+        n = ctypes.c_size_t()
+        D.check(D.lib().lrh_server_latencies(self.h, None, 0, ctypes.byref(n)))
+        out = np.zeros(n.value, np.float64)
+        D.check(D.lib().lrh_server_latencies(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), out.size,
+                                             ctypes.byref(n)))
+        return out
+
+    def recorded(self, sid, step, kind):
+        """(rays, hits) of a recorded wavefront (kind 0 closest, 1 shadow), or
+        None when the session traced no such wavefront at that step."""
+        rays = np.zeros(self.n, RAY_DTYPE)
+        hits = np.zeros(self.n, HIT_DTYPE)
+        rc = D.lib().lrh_server_recorded(self.h, sid, step, kind, rays.ctypes.data, hits.ctypes.data, None)
+        if rc != D.LRH_OK:
+            return None
+        return rays, hits
+
+    def recorded_steps(self, sid):
+        """Per step: ((closest rays, hits) or None, (shadow rays, hits) or None)."""
+        out = []
+        for step in range(self.max_bounces + 2):
+            out.append((self.recorded(sid, step, 0), self.recorded(sid, step, 1)))
+        return out
+
+    def radiance(self, sid):
+        out = np.zeros((self.n, 3), np.float32)
+        D.check(D.lib().lrh_server_radiance(self.h, sid, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), self.n))
+        return out
+
+    def plain_batch(self, opts=None, reps=3):
+        """The captured wavefronts traced as ONE plain lrh_trace_batch and as
+        one lrh_trace per buffer (ms each, plus ray counts)."""
+        r = D.LrhPlainBatchResult()
+        D.check(D.lib().lrh_server_plain_batch(self.h, ctypes.byref(opts) if opts else None, reps, ctypes.byref(r)))
+        return {f: getattr(r, f) for f, _ in D.LrhPlainBatchResult._fields_}
 
     def close(self):
-        self.drain()
-        for bufs in self.pool:
-            for b in bufs:
-                b.free()
-        self.pool = []
-        for st in self.streams[1:]:
-            self.dev.destroy_stream(st)
-        self.streams = self.streams[:1]
+        if self.h and self.dev.h:
+            D.check(D.lib().lrh_server_destroy(self.h))
+        self.h = None
 
-    # ---- admission ----
-    def _resident(self):
-        return self.allocated - len(self.pool)
-
-    def _can_admit(self, cohort):
-        if (self._resident() + 1) * self.session_bytes > self.hbm_budget:
-            return False
-        rays = (len(self.cohorts[cohort]) + 1) * 2 * self.width * self.height
-        return rays <= max(self.max_rays_per_tick, 2 * self.width * self.height)
-
-    def submit(self, spec):
-        self.queue.append(spec)
-
-    def _admit(self, now):
-        while self.queue and self.queue[0].arrival <= now:
-            c = min(range(len(self.cohorts)), key=lambda k: (len(self.cohorts[k]), k))
-            if not self._can_admit(c):
-                return
-            spec = self.queue.popleft()
-            s = Session(self, spec, c)
-            s.admitted = now
-            s.ready = now
-            self.queue_waits.append(now - spec.arrival)
-            self.cohorts[c].append(s)
-
-    # ---- completion of a cohort's tick ----
-    def _retire(self, c, block):
-        f = self.inflight[c]
-        if f is None:
-            return
-        e0, e1, traced, finished = f
-        if not block and not e1.query():
-            return
-        e1.sync()
-        t = time.perf_counter()
-        self.inflight[c] = None
-        self.intervals.append((self.ref.elapsed_ms(e0), self.ref.elapsed_ms(e1)))
-        for s in traced:
-            self.latencies.append(t - s.ready)
-            s.ready = t
-        for s in finished:
-            if s.spec.sid in self.record and self.shading:
-                s.radiance = self.dev.download(s.paths, D.PATH_DTYPE, s.n)["rad"]
-            s.release()
-            self.done.append(s)
-
-    def busy_seconds(self):
-        """Length of the union of the ticks' GPU intervals."""
-        total, end = 0.0, -1e30
-        for a, b in sorted(self.intervals):
-            if b <= end:
-                continue
-            total += b - max(a, end)
-            end = b
-        return total / 1e3
-
-    # ---- one tick of the cohort whose turn it is ----
-    def tick(self, now=None):
-        c = self.turn
-        self.turn = (c + 1) % len(self.cohorts)
-        self._retire(c, block=True)
-        for k in range(len(self.cohorts)):
-            self._retire(k, block=False)
-        now = time.perf_counter() if now is None else now
-        self._admit(now)
-        coh = self.cohorts[c]
-        if not coh:
-            return False
-        self.max_active_seen = max(self.max_active_seen, sum(len(x) for x in self.cohorts))
-        st = self.streams[c]
-        items = []
-        for s in coh:
-            items += s.items()
-        if self.record:
-            # masked rays leave their hit records untouched: clear the recorded
-            # sessions' hit buffers so what is compared does not depend on the
-            # buffers' previous contents
-            for s in coh:
-                if s.spec.sid in self.record:
-                    z = np.zeros(s.n, HIT_DTYPE)
-                    self.dev.upload(z, s.hits)
-                    self.dev.upload(z, s.shadow_hits)
-            self.dev.sync()
-        e0, e1 = self.dev.event(), self.dev.event()
-        e0.record(st)
-        self.dev.trace_batch(items, self.opts, stream=st)
-        if self.record:
-            self.dev.sync(st)
-            for s in coh:
-                if s.spec.sid in self.record:
-                    ch = self.dev.download(s.hits, HIT_DTYPE, s.n) if s.step <= self.max_bounces else None
-                    sh = self.dev.download(s.shadow_hits, HIT_DTYPE, s.n) if s.step >= 1 else None
-                    s.recorded.append((ch, sh))
-        still = []
-        finished = []
-        for s in coh:
-            (still if s.advance() else finished).append(s)
-        e1.record(st)
-        self.inflight[c] = (e0, e1, list(coh), finished)
-        self.cohorts[c] = still
-        self.ticks += 1
-        return True
-
-    def drain(self):
-        for c in range(len(self.cohorts)):
-            self._retire(c, block=True)
-
-    def live_rays(self):
-        self.drain()
-        c = self.dev.download(self.counter, np.uint32, 2)
-        return int(c[0])
-
-    def run(self, specs, time_limit=60.0):
-        """Run until every spec has finished (arrival times relative to now)."""
-        t_start = time.perf_counter()
-        for sp in sorted(specs, key=lambda x: x.arrival):
-            self.submit(SessionSpec(sp.sid, sp.scene, sp.seed, t_start + sp.arrival))
-        idle = 0
-        while (self.queue or any(self.cohorts) or any(f is not None for f in self.inflight)) and \
-                time.perf_counter() - t_start < time_limit:
-            if self.tick():
-                idle = 0
-                continue
-            idle += 1
-            if idle >= len(self.cohorts):
-                idle = 0
-                if any(f is not None for f in self.inflight):
-                    self.drain()
-                elif self.queue:
-                    # nothing admitted yet: wait for the next arrival
-                    dt = self.queue[0].arrival - time.perf_counter()
-                    if dt > 0:
-                        time.sleep(min(dt, 0.001))
-        self.drain()
-        wall = time.perf_counter() - t_start
-        return wall
+    def __del__(self):
+        try:
+            if self.h and self.dev.h:
+                D.lib().lrh_server_destroy(self.h)
+        except Exception:
+            pass
+        self.h = None
 
 
-def poisson_specs(rate, duration, n_scenes, seed=0):
+def poisson_specs(rate, duration, n_scenes, seed=0, first_sid=0):
     rng = np.random.Generator(np.random.PCG64(seed))
-    t, out, sid = 0.0, [], 0
+    t, out, sid = 0.0, [], first_sid
     while True:
         t += rng.exponential(1.0 / rate)
         if t > duration:
@@ -360,58 +159,60 @@ def poisson_specs(rate, duration, n_scenes, seed=0):
     return out
 
 
-def plain_throughput(dev, scenes, specs, opts=None, width=512, height=512, max_bounces=8):
-    """The same sessions one at a time with one plain lrh_trace per buffer:
-    live Mrays/s of the un-batched path."""
-    srv = Server(dev, scenes, width, height, max_bounces, opts=opts)
-    t0 = time.perf_counter()
-    for sp in specs:
-        s = Session(srv, sp)
-        while True:
-            for (sc, rb, hb, n, q) in s.items():
-                dev.trace(sc, rb, hb, n, q, opts)
-            if not s.advance():
-                break
-        dev.sync()
-        s.release()
-    wall = time.perf_counter() - t0
-    cam = len(specs) * width * height
-    rate = (srv.live_rays() + cam) / wall / 1e6
+def served_row(srv, wall, rate=None, tick_rays=None):
+    st = srv.stats()
+    return {"rate_per_s": rate, "max_rays_per_tick": tick_rays, "sessions": st["sessionsDone"],
+            "wall_s": round(wall, 3), "live_mrays": round(st["liveRays"] / wall / 1e6, 1),
+            "busy_mrays": round(st["liveRays"] / max(st["busySeconds"], 1e-9) / 1e6, 1),
+            "trace_mrays": round(st["liveRays"] / max(st["traceSeconds"], 1e-9) / 1e6, 1),
+            "p50_ms": round(st["p50Ms"], 3), "p99_ms": round(st["p99Ms"], 3),
+            "wavefronts": st["latencySamples"], "queue_p99_ms": round(st["queueP99Ms"], 3),
+            "ticks": st["ticks"], "max_active": st["maxActive"]}
+
+
+def plain_comparison(dev, scenes, opts, n_sessions=16, seed=7, log=None):
+    """The served form next to a single plain batch of the same rays: the
+    first n_sessions sessions of a served run are replayed with every traced
+    wavefront captured, then those buffers are traced as one plain
+    lrh_trace_batch and as one lrh_trace each."""
+    specs = poisson_specs(1000.0, 1.0, len(scenes), seed=seed)[:n_sessions]
+    n = 512 * 512
+    cap = n_sessions * 20 * n * RAY_B  # up to 9 closest + 9 shadow wavefronts per session
+    srv = Server(dev, scenes, opts=opts, capture_bytes=cap)
+    wall = srv.run(specs, time_limit=120.0)
+    row = served_row(srv, wall)
+    pb = srv.plain_batch(opts)
     srv.close()
-    return rate
+    live = pb["liveRays"]
+    out = {"sessions": n_sessions, "buffers": pb["buffers"], "rays": pb["rays"], "live_rays": live,
+           "batch_ms": round(pb["batchMs"], 3), "per_buffer_ms": round(pb["perBufferMs"], 3),
+           "plain_batch_mrays": round(live / pb["batchMs"] / 1e3, 1),
+           "plain_per_buffer_mrays": round(live / pb["perBufferMs"] / 1e3, 1),
+           "served_trace_mrays_same_sessions": row["trace_mrays"]}
+    if log:
+        log(f"plain single batch of the captured wavefronts: {out}")
+    return out
 
 
 def bench(args, dev, log):
     """bench.py --mode served: Poisson arrivals at several rates over the 8
     resident batched-workload scenes; p50/p99 wavefront latency and live
-    Mrays/s per rate, against the plain one-session-at-a-time path."""
+    Mrays/s per rate, next to a single plain batch of the same kind of rays."""
     from . import scenes as S
     meshes = S.batch_scenes()
     scenes = [dev.scene_bvh(m) for m in meshes]
     opts = D.trace_opts(256, 8, 0, 12)
-    warm = poisson_specs(1000.0, 0.004, len(scenes), seed=99)
     w = Server(dev, scenes, opts=opts)
-    w.run(warm)
+    w.run(poisson_specs(1000.0, 0.004, len(scenes), seed=99))
     w.close()
-    base_specs = poisson_specs(1000.0, 0.016, len(scenes), seed=7)[:12]
-    plain = plain_throughput(dev, scenes, base_specs, opts)
-    log(f"plain (one session at a time, per-buffer traces): {plain:.1f} live Mrays/s")
+    plain = plain_comparison(dev, scenes, opts, log=log)
     rows = []
     for tick_rays in (3 << 20, 3 << 19, 1 << 20):
-        for rate in (50.0, 150.0, 250.0, 300.0, 600.0):
+        for rate in (50.0, 150.0, 300.0, 600.0):
             specs = poisson_specs(rate, 1.0, len(scenes), seed=int(rate))
             srv = Server(dev, scenes, opts=opts, max_rays_per_tick=tick_rays)
             wall = srv.run(specs, time_limit=120.0)
-            live = srv.live_rays() + len(srv.done) * srv.width * srv.height
-            lat = np.array(srv.latencies) * 1e3
-            qw = np.array(srv.queue_waits) * 1e3
-            busy = srv.busy_seconds()
-            row = {"rate_per_s": rate, "max_rays_per_tick": tick_rays, "sessions": len(srv.done),
-                   "wall_s": round(wall, 3), "live_mrays": round(live / wall / 1e6, 1),
-                   "busy_mrays": round(live / busy / 1e6, 1),
-                   "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
-                   "queue_p99_ms": round(float(np.percentile(qw, 99)), 3) if qw.size else 0.0,
-                   "ticks": srv.ticks, "max_active": srv.max_active_seen}
+            row = served_row(srv, wall, rate, tick_rays)
             log(f"served: {row}")
             rows.append(row)
             srv.close()
@@ -423,8 +224,10 @@ def bench(args, dev, log):
             "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": "Poisson session arrivals over 8 resident scenes (10k..5M tris), 512x512 camera "
                                    "+ <=8 bounces (RR 0.7 after 2) + 1 shadow ray per bounce; value = live rays per "
-                                   "second of GPU time of the best setting whose p99 wavefront latency is under 5 ms",
-                       "rates": rows, "hbm_budget_bytes": 4 << 30, "cohorts": 2},
-            "plain_mrays": round(plain, 1), "served_over_plain": round(best["busy_mrays"] / plain, 3),
+                                   "second of GPU time (trace + shading) of the best setting whose p99 wavefront "
+                                   "latency is under 5 ms", "rates": rows, "hbm_budget_bytes": 4 << 30, "cohorts": 2},
+            "plain": plain,
+            "served_trace_over_plain_batch": round(best["trace_mrays"] / plain["plain_batch_mrays"], 3),
             "latency_ms": {"p50": best["p50_ms"], "p99": best["p99_ms"], "per": "one session wavefront",
-                           "rate_per_s": best["rate_per_s"], "max_rays_per_tick": best["max_rays_per_tick"]}}
+                           "samples": best["wavefronts"], "rate_per_s": best["rate_per_s"],
+                           "max_rays_per_tick": best["max_rays_per_tick"]}}

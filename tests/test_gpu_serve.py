@@ -81,27 +81,52 @@ def test_session_same_alone_and_in_mix(gpu_device, serve_scenes):
     mix = serve.Server(dev, scs, record={0}, max_rays_per_tick=6 * 2 * 96 * 96, **kw)
     others = [serve.SessionSpec(i, i % 4, 1000 + i, 0.0005 * i) for i in range(1, 8)]
     mix.run([target] + others)
-    sa_, sb_ = [s for s in alone.done if s.spec.sid == 0][0], [s for s in mix.done if s.spec.sid == 0][0]
-    a, b = sa_.recorded, sb_.recorded
-    assert len(a) == len(b) == 6
+    a, b = alone.recorded_steps(0), mix.recorded_steps(0)
+    # 5 closest-hit wavefronts (camera + 4 bounces), shadow rays from step 1 to 5
+    assert [x is not None for x, _ in a] == [True] * 5 + [False]
+    assert [y is not None for _, y in a] == [False] + [True] * 5
     # the shaded image is the same alone and in the mix, and something was lit
-    assert sa_.radiance is not None and np.array_equal(sa_.radiance, sb_.radiance)
-    assert (sa_.radiance.sum(axis=1) > 0).mean() > 0.05
-    for k, ((ca, sa), (cb, sb)) in enumerate(zip(a, b)):
-        for x, y in ((ca, cb), (sa, sb)):
+    ra, rb = alone.radiance(0), mix.radiance(0)
+    assert np.array_equal(ra, rb)
+    assert (ra.sum(axis=1) > 0).mean() > 0.05
+    for k, (pa, pb) in enumerate(zip(a, b)):
+        for x, y in ((pa[0], pb[0]), (pa[1], pb[1])):
             assert (x is None) == (y is None), k
             if x is not None:
-                assert np.array_equal(x, y), k
-    assert len(mix.done) == 8 and mix.max_active_seen >= 2
+                assert np.array_equal(x[0], y[0]), k   # rays
+                assert np.array_equal(x[1], y[1]), k   # hits
+    st = mix.stats()
+    assert st["sessionsDone"] == 8 and st["maxActive"] >= 2
+    alone.close()
+    mix.close()
 
 
 def test_admission_respects_hbm_budget(gpu_device, serve_scenes):
     _case("admission")
     dev = gpu_device
     _, scs = serve_scenes
-    budget = 2 * serve.Session.bytes(64, 64) + 100
+    budget = 2 * serve.session_bytes(64, 64) + 100
     srv = serve.Server(dev, scs, width=64, height=64, max_bounces=3, hbm_budget=budget)
     specs = [serve.SessionSpec(i, i % 4, 50 + i, 0.0) for i in range(6)]
     srv.run(specs)
-    assert len(srv.done) == 6 and srv.max_active_seen == 2
-    assert len(srv.latencies) == 6 * 5
+    st = srv.stats()
+    assert st["sessionsDone"] == 6 and st["maxActive"] == 2 and st["sessionBuffersAllocated"] == 2
+    # each session: camera wavefront + 3 bounce wavefronts + the last shadow wavefront
+    assert st["latencySamples"] == 6 * 5 and len(srv.latencies_ms()) == 6 * 5
+    srv.close()
+
+
+def test_server_rejects_bad_arguments(gpu_device, serve_scenes):
+    _case("server_errors")
+    _, scs = serve_scenes
+    srv = serve.Server(gpu_device, scs, width=32, height=32, max_bounces=1)
+    with pytest.raises(D.LrhError):
+        srv.submit([serve.SessionSpec(0, 99, 1, 0.0)])        # no such scene
+    srv.submit([serve.SessionSpec(1, 0, 1, 0.0)])
+    with pytest.raises(D.LrhError):
+        srv.submit([serve.SessionSpec(1, 0, 2, 0.0)])         # id already used
+    srv.run()
+    assert srv.recorded(1, 0, 0) is None                       # not recorded
+    with pytest.raises(D.LrhError):
+        srv.plain_batch()                                      # nothing captured
+    srv.close()
