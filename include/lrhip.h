@@ -176,7 +176,13 @@ int lrh_scene_create_bvh(lrh_device *dev, const lrh_mesh *meshes, unsigned int n
 int lrh_scene_create_mbvh(lrh_device *dev, const lrh_mesh *meshes, unsigned int nMeshes,
 		const lrh_object *objects, unsigned int nObjects, const lrh_build_opts *opts, lrh_scene **out);
 /* MBVHAccel::Update: new instance matrices / motion keys for the same objects
- * (same count, kinds and meshes); root tree rebuilt, leaf trees kept. */
+ * (same count, kinds and meshes; key counts may change); root tree rebuilt,
+ * leaf trees kept.  Synchronous on the device's stream; no trace of this
+ * scene may run on another stream meanwhile (the root nodes, instance
+ * records and motion arrays are rewritten in place, and arrays that must grow
+ * are reallocated).  An edit that reallocates an array or deepens the tree
+ * invalidates resident batches over the scene (lrh_batch_launch/replay then
+ * return LRH_EINVAL: recreate the batch). */
 int lrh_scene_update(lrh_scene *scene, const lrh_object *objects, unsigned int nObjects);
 /* The same edit without a rebuild (MBVHAccel::Update, mbvhaccel.cpp:237-250,
  * rebuilds the root tree on the host; MBVHKernel::Update,
@@ -213,8 +219,11 @@ int lrh_trace_ex(lrh_scene *scene, const void *rays, void *hits, unsigned int co
  * bytes-per-ray roofline accounting; and the SIMD-efficiency counters, summed
  * over waves: out[3] = traversal passes, out[4] = live lanes summed over those
  * passes, out[5] = passes with a node test, out[6] = passes with a leaf test,
- * out[7] = refills. */
-#define LRH_TRACE_STATS 8
+ * out[7] = refills; two-level scenes: out[8] = object entries (root leaves
+ * entered, each an instance record read; out[1] counts them too), out[9] =
+ * of those, entries through a static matrix, out[10] = through a motion
+ * system. */
+#define LRH_TRACE_STATS 11
 int lrh_trace_stats(lrh_scene *scene, const void *rays, void *hits, unsigned int count,
 		unsigned int query, const lrh_trace_opts *opts, void *stream, unsigned long long out[LRH_TRACE_STATS]);
 /* Many ray buffers over one or more resident scenes of the same device in one
@@ -228,7 +237,11 @@ int lrh_trace_batch(lrh_device *dev, const lrh_batch_item *items, unsigned int n
 /* A resident batch: descriptor table and scratch uploaded once, then launched
  * many times, directly or as a captured hipGraph (one kernel node).  Replays
  * of one batch must not overlap; the kernel rewinds its own ticket counter, so
- * the graph holds no memset node. */
+ * the graph holds no memset node.  The batch holds its scenes' device
+ * addresses and spill depth: launch, capture and replay return LRH_EINVAL once
+ * a scene was destroyed or edited in a way that moved its arrays or deepened
+ * its tree (lrh_scene_update / lrh_scene_refit with more motion keys, or a
+ * deeper rebuilt root); a refit that keeps the key counts leaves it valid. */
 int lrh_batch_create(lrh_device *dev, const lrh_batch_item *items, unsigned int nItems,
 		const lrh_trace_opts *opts, lrh_batch **out);
 int lrh_batch_launch(lrh_batch *batch, void *stream);
@@ -305,6 +318,102 @@ int lrh_shade(lrh_scene *scene, const lrh_material *materials, unsigned int nMat
 		const void *shadowHits /* hits of the previous step's shadow rays, or NULL */, void *paths,
 		void *nextRays, void *shadowRays, const lrh_shade_params *params,
 		unsigned int *liveCount /* device counter or NULL: += live rays written */, void *stream);
+/* ---- served path-tracing sessions (continuous batching) ----
+ * No reference counterpart as a component: it batches what SLG's path tracer
+ * does one device at a time, one ray buffer per wavefront traced and its hits
+ * consumed (src/slg/engines/pathocl/pathoclopenclthread.cpp:183-189).
+ * A session = width x height camera rays, then up to maxBounces bounce
+ * wavefronts, one shadow ray per path vertex (any-hit), shaded on the GPU by
+ * lrh_shade (mode LRH_SERVE_SHADE: grey Lambertian surfaces, a square area
+ * light at `light`, Russian roulette after bounce rrStart) or lrh_rays_bounce
+ * (LRH_SERVE_BOUNCE).  Sessions are admitted while their buffers fit
+ * hbmBudget and their cohort's tick stays within maxRaysPerTick; each tick
+ * traces one cohort's ready wavefronts in one lrh_trace_batch launch.
+ * A session's rays and results do not depend on what else is in flight.
+ * Host-side calls of one server must not overlap (one driving thread). */
+#define LRH_SERVE_SHADE 0u
+#define LRH_SERVE_BOUNCE 1u
+typedef struct {
+	unsigned int width, height;       /* session image, default 512 x 512 */
+	unsigned int maxBounces;          /* bounce wavefronts after the camera one, default 8 */
+	unsigned int rrStart;             /* Russian roulette after this bounce, default 2 */
+	float rrSurvive;                  /* survival probability, default 0.7 */
+	unsigned long long hbmBudget;     /* bytes of session buffers resident at once, default 4 GiB */
+	unsigned int maxRaysPerTick;      /* rays one cohort's tick may carry, default 3 Mi */
+	unsigned int cohorts;             /* streams the sessions are split over, 1..8, default 2 */
+	unsigned int mode;                /* LRH_SERVE_SHADE (default) or LRH_SERVE_BOUNCE */
+	float extent;                     /* scene box [0, extent]^3 for camera placement, default 100 */
+	float light[3];                   /* light centre, default (extent/2, extent/2, 1.5 extent) */
+	lrh_trace_opts opts;              /* launch shape of the batched trace (zeros: defaults) */
+} lrh_server_config;
+typedef struct {
+	unsigned int sid;                 /* caller's session id (unique per server) */
+	unsigned int scene;               /* index into the server's scenes */
+	unsigned long long seed;          /* camera placement and path random numbers */
+	double arrival;                   /* seconds after the lrh_server_submit call */
+} lrh_session_spec;
+typedef struct {
+	unsigned long long sessionsDone, sessionsQueued, ticks, wavefronts;
+	unsigned long long liveRays;      /* camera rays + live rays the shading steps wrote */
+	unsigned int maxActive;           /* most sessions live at once */
+	unsigned int sessionBuffersAllocated;
+	double busySeconds;               /* union of the ticks' GPU intervals (trace + shading) */
+	double traceSeconds;              /* union of the batched traces' GPU intervals */
+	unsigned long long latencySamples;
+	double p50Ms, p99Ms, maxMs;       /* wavefront latency: ready -> completion of its tick */
+	double queueP99Ms;                /* arrival -> admission */
+} lrh_server_stats;
+typedef struct {
+	unsigned int buffers;             /* captured ray buffers */
+	unsigned long long rays, liveRays;
+	double batchMs;                   /* one lrh_trace_batch over every captured buffer */
+	double perBufferMs;               /* one lrh_trace per captured buffer */
+} lrh_plain_batch_result;
+typedef struct lrh_server lrh_server;
+int lrh_server_create(lrh_device *dev, lrh_scene *const *scenes, unsigned int nScenes, const lrh_server_config *cfg,
+		lrh_server **out);
+int lrh_server_submit(lrh_server *srv, const lrh_session_spec *specs, unsigned int n);
+/* Keep every wavefront's rays and hits (and, in shade mode, the final
+ * per-pixel radiance) of session `sid`, for checking; call before it is admitted. */
+int lrh_server_record(lrh_server *srv, unsigned int sid);
+/* One tick of the next cohort; *traced = 1 if it traced anything. */
+int lrh_server_tick(lrh_server *srv, int *traced);
+/* Ticks until every submitted session has finished (or timeLimit seconds). */
+int lrh_server_run(lrh_server *srv, double timeLimit, double *wallSeconds);
+int lrh_server_get_stats(lrh_server *srv, lrh_server_stats *stats);
+/* Every wavefront latency in ms, in completion order; *n = how many there are. */
+int lrh_server_latencies(lrh_server *srv, double *ms, size_t cap, size_t *n);
+/* Recorded wavefront of step `step` (0 = camera): kind 0 the closest-hit
+ * rays and hits, kind 1 the shadow rays (traced from step 1 on); host buffers
+ * of width*height records (either may be NULL). */
+int lrh_server_recorded(lrh_server *srv, unsigned int sid, unsigned int step, unsigned int kind, void *rays, void *hits,
+		unsigned int *count);
+/* Final per-pixel radiance (3 floats per pixel) of a recorded session. */
+int lrh_server_radiance(lrh_server *srv, unsigned int sid, float *rgb, unsigned int count);
+/* Copy every ray buffer the server traces into a device arena of maxBytes
+ * (call before the first session is admitted), then time the same buffers as
+ * one plain lrh_trace_batch and as one lrh_trace each: the served form's
+ * throughput next to a single plain batch of the same rays. */
+int lrh_server_capture(lrh_server *srv, unsigned long long maxBytes);
+int lrh_server_plain_batch(lrh_server *srv, const lrh_trace_opts *opts, unsigned int reps,
+		lrh_plain_batch_result *result);
+int lrh_server_destroy(lrh_server *srv);
+/* Launch-shape autotuner of a batched trace: times each candidate shape
+ * (NULL: the LRH_AUTOTUNE_SHAPES built-in ones, block size {128, 256} x LDS
+ * stack {4, 8, 16} x refill threshold {4, 8, 12, 16, 24}) on the actual items
+ * (resident batches, median of 3 launches after 2 warm-ups) and keeps the
+ * fastest; candidateMs (optional, one per candidate) gets each time, NaN for
+ * a shape the device refuses.  With cachePath and key (no spaces), the result
+ * is read from / stored in a text cache ("key block lds refill ms" per line),
+ * so a serving process tunes once per workload signature; a cached answer
+ * times nothing (candidateMs all NaN).  (No reference counterpart: the
+ * reference's work-group size is a fixed setting,
+ * src/slg/engines/oclrenderengine.cpp:193.) */
+#define LRH_AUTOTUNE_SHAPES 30
+int lrh_autotune_batch(lrh_device *dev, const lrh_batch_item *items, unsigned int nItems,
+		const lrh_trace_opts *candidates, unsigned int nCandidates, const char *cachePath, const char *key,
+		lrh_trace_opts *best, float *bestMs, float *candidateMs);
+
 /* 1 if all work before the event has completed, 0 if not yet, < 0 on error. */
 int lrh_event_query(lrh_event *ev);
 

@@ -28,6 +28,7 @@
 #include "lrh_paths.h"
 #include "lrh_refit.h"
 #include "lrh_layout.h"
+#include "lrh_internal.h"
 
 using lrh::Box;
 using lrh::Error;
@@ -74,6 +75,13 @@ int guarded(F fn) {
 double now() {
 	return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
+
+}  // namespace
+
+// the message of lrh_last_error for the library's other host files
+int lrhSetError(int code, const std::string &msg) { return fail(code, msg); }
+
+namespace {
 
 }  // namespace
 
@@ -146,6 +154,13 @@ struct DevBuf {
 	~DevBuf() { release(); }
 };
 
+namespace {
+uint64_t nextSceneSerial() {
+	static std::atomic<uint64_t> n{0};
+	return ++n;
+}
+}  // namespace
+
 struct lrh_event {
 	hipEvent_t ev = nullptr;
 	lrh_device *dev = nullptr;
@@ -170,8 +185,14 @@ struct lrh_scene {
 	// root topology kept for lrh_scene_refit: leaf slot -> object, node
 	// indices by level (deepest first), object boxes of the last host build
 	std::vector<uint32_t> rootOrder, levelNodes, levelStart;
-	std::vector<Box> objBoxes;
-	bool boxesOnDevice = false;  // a refit has replaced objBoxes on the device
+	std::vector<Box> objBoxes;  // object boxes of the last accepted build, update or refit
+	// bumped whenever an edit moves one of the scene's device arrays or deepens
+	// its tree: a resident batch built before then would read freed memory or
+	// overflow its spill stack, so it refuses to launch
+	uint64_t generation = 0;
+	// process-wide creation number: a new scene at a destroyed one's address
+	// is told apart from it
+	uint64_t serial = nextSceneSerial();
 	DevBuf rSlotBoxes, rNodeBoxes, rLevelNodes, rMeshBoxes, rMotionObjs, rSampleBase, rKeys, rError, rMotions, rSegs;
 	// host-only scenes (created with dev == NULL) keep their arrays here so
 	// the builder can be inspected without a GPU; they cannot be traced
@@ -437,7 +458,6 @@ void buildRoot(lrh_scene *s, ObjectData &od, HostTree &ht, std::vector<lrh_inst>
 	// refit topology: slot order and the root nodes grouped by depth
 	s->rootOrder = w.order;
 	s->objBoxes = od.boxes;
-	s->boxesOnDevice = false;
 	const size_t nn = w.nodeCount();
 	std::vector<uint32_t> depthOf(nn, 0), bfs(1, w.root);
 	uint32_t maxD = 0;
@@ -467,6 +487,25 @@ void keepHost(lrh_scene *s, const HostTree &ht, const void *tris, size_t triByte
 	s->hostPrims.resize(triBytes + instBytes);
 	if (triBytes) std::memcpy(s->hostPrims.data(), tris, triBytes);
 	if (instBytes) std::memcpy(s->hostPrims.data() + triBytes, insts, instBytes);
+}
+
+// uploads an edit's instance matrices and motion systems; true if one of the
+// arrays had to grow (a new device address)
+bool uploadEdit(lrh_scene *s, const ObjectData &od, hipStream_t st) {
+	lrh_device *dev = s->dev;
+	bool moved = false;
+	auto put = [&](DevBuf &b, const void *src, size_t bytes) {
+		if (bytes > b.bytes) {
+			b.alloc(dev, bytes);
+			moved = true;
+		}
+		b.upload(src, bytes, st);
+	};
+	put(s->xforms, od.xf.data(), od.xf.size() * 4);
+	put(s->motions, od.mot.data(), od.mot.size() * sizeof(lrh_motion));
+	put(s->segs, od.segs.data(), od.segs.size() * sizeof(lrh_motion_seg));
+	put(s->times, od.times.data(), od.times.size() * 4);
+	return moved;
 }
 
 void finishInfo(lrh_scene *s, size_t nNodes, size_t nPrims, size_t primBytes, double tb, double tu) {
@@ -944,17 +983,11 @@ int lrh_scene_update(lrh_scene *s, const lrh_object *objects, unsigned int nObje
 		HIPCHK(hipMemcpyAsync(s->nodes.p, ht.data(s->layout), nb, hipMemcpyHostToDevice, dev->stream));
 		HIPCHK(hipMemcpyAsync(s->insts.p, instOut.data(), instOut.size() * sizeof(lrh_inst), hipMemcpyHostToDevice,
 				dev->stream));
-		if (od.xf.size() * 4 > s->xforms.bytes) s->xforms.alloc(dev, od.xf.size() * 4);
-		s->xforms.upload(od.xf.data(), od.xf.size() * 4, dev->stream);
-		if (od.mot.size() * sizeof(lrh_motion) > s->motions.bytes) s->motions.alloc(dev, od.mot.size() * sizeof(lrh_motion));
-		s->motions.upload(od.mot.data(), od.mot.size() * sizeof(lrh_motion), dev->stream);
-		if (od.segs.size() * sizeof(lrh_motion_seg) > s->segs.bytes)
-			s->segs.alloc(dev, od.segs.size() * sizeof(lrh_motion_seg));
-		s->segs.upload(od.segs.data(), od.segs.size() * sizeof(lrh_motion_seg), dev->stream);
-		if (od.times.size() * 4 > s->times.bytes) s->times.alloc(dev, od.times.size() * 4);
-		s->times.upload(od.times.data(), od.times.size() * 4, dev->stream);
+		const bool moved = uploadEdit(s, od, dev->stream);
 		HIPCHK(hipStreamSynchronize(dev->stream));
-		s->maxDepth = rootDepth + s->leafDepth + 1;
+		const uint32_t depth = rootDepth + s->leafDepth + 1;
+		if (moved || depth > s->maxDepth) ++s->generation;
+		s->maxDepth = depth;
 		s->info.maxDepth = s->maxDepth;
 	});
 }
@@ -1047,18 +1080,20 @@ int lrh_scene_refit(lrh_scene *s, const lrh_object *objects, unsigned int nObjec
 			if (err) throw Error(LRH_EINVAL, "singular motion sample");
 		}
 		// accepted: the edit's matrices and motion systems, as lrh_scene_update uploads them
-		if (od.xf.size() * 4 > s->xforms.bytes) s->xforms.alloc(dev, od.xf.size() * 4);
-		s->xforms.upload(od.xf.data(), od.xf.size() * 4, st);
-		if (od.mot.size() * sizeof(lrh_motion) > s->motions.bytes) s->motions.alloc(dev, od.mot.size() * sizeof(lrh_motion));
-		s->motions.upload(od.mot.data(), od.mot.size() * sizeof(lrh_motion), st);
-		if (od.segs.size() * sizeof(lrh_motion_seg) > s->segs.bytes)
-			s->segs.alloc(dev, od.segs.size() * sizeof(lrh_motion_seg));
-		s->segs.upload(od.segs.data(), od.segs.size() * sizeof(lrh_motion_seg), st);
-		if (od.times.size() * 4 > s->times.bytes) s->times.alloc(dev, od.times.size() * 4);
-		s->times.upload(od.times.data(), od.times.size() * 4, st);
+		if (uploadEdit(s, od, st)) ++s->generation;
 		HIPCHK(lrhLaunchRefitLevels(a, st));
+		// the accepted object boxes (a rejected edit never gets here, so the
+		// scene's boxes stay those of the last accepted edit)
+		std::vector<float> out(6 * (size_t)nObjects);
+		HIPCHK(hipMemcpyAsync(out.data(), s->rSlotBoxes.p, out.size() * 4, hipMemcpyDeviceToHost, st));
 		HIPCHK(hipStreamSynchronize(st));
-		s->boxesOnDevice = true;
+		for (size_t j = 0; j < s->rootOrder.size(); ++j) {
+			Box &b = s->objBoxes[s->rootOrder[j]];
+			for (int k = 0; k < 3; ++k) {
+				b.lo[k] = out[6 * j + k];
+				b.hi[k] = out[6 * j + 3 + k];
+			}
+		}
 	});
 }
 
@@ -1067,20 +1102,11 @@ int lrh_scene_object_boxes(lrh_scene *s, float *boxes, unsigned int nObjects) {
 	if (!s->twoLevel) return fail(LRH_EINVAL, "object boxes exist for two-level scenes only");
 	if (nObjects != s->objMesh.size()) return fail(LRH_EINVAL, "object count differs from the scene");
 	return guarded([&]() {
-		if (!s->boxesOnDevice) {
-			for (unsigned o = 0; o < nObjects; ++o)
-				for (int k = 0; k < 3; ++k) {
-					boxes[6 * o + k] = s->objBoxes[o].lo[k];
-					boxes[6 * o + 3 + k] = s->objBoxes[o].hi[k];
-				}
-			return;
-		}
-		std::vector<float> slot(6 * (size_t)nObjects);
-		s->dev->activate();
-		HIPCHK(hipMemcpyAsync(slot.data(), s->rSlotBoxes.p, slot.size() * 4, hipMemcpyDeviceToHost, s->dev->stream));
-		HIPCHK(hipStreamSynchronize(s->dev->stream));
-		for (size_t j = 0; j < s->rootOrder.size(); ++j)
-			std::memcpy(boxes + 6 * (size_t)s->rootOrder[j], slot.data() + 6 * j, 24);
+		for (unsigned o = 0; o < nObjects; ++o)
+			for (int k = 0; k < 3; ++k) {
+				boxes[6 * o + k] = s->objBoxes[o].lo[k];
+				boxes[6 * o + 3 + k] = s->objBoxes[o].hi[k];
+			}
 	});
 }
 
@@ -1204,8 +1230,8 @@ void fillSceneParams(LrhTraceParams &P, const lrh_scene *s) {
 	P.root = s->root;
 }
 
-int traceImpl(lrh_scene *s, const void *rays, void *hits, unsigned int count, unsigned int query,
-		const lrh_trace_opts *opts, void *stream, unsigned long long *statsOut) {
+// argument checks shared by every single-scene trace entry point
+int checkTrace(const lrh_scene *s, const void *rays, const void *hits, unsigned int count, unsigned int query) {
 	if (!s) return fail(LRH_EINVAL, "null scene");
 	if (count && (!rays || !hits)) return fail(LRH_EINVAL, "null ray or hit buffer");
 	if (query > 1) return fail(LRH_EINVAL, "query must be LRH_QUERY_CLOSEST or LRH_QUERY_ANY");
@@ -1213,6 +1239,13 @@ int traceImpl(lrh_scene *s, const void *rays, void *hits, unsigned int count, un
 	if (!s->dev) return fail(LRH_EINVAL, "host-only scene cannot be traced");
 	if (((uintptr_t)rays & 15u) || ((uintptr_t)hits & 3u))
 		return fail(LRH_EINVAL, "ray buffer must be 16-byte aligned, hit buffer 4-byte aligned");
+	return LRH_OK;
+}
+
+int traceImpl(lrh_scene *s, const void *rays, void *hits, unsigned int count, unsigned int query,
+		const lrh_trace_opts *opts, void *stream, unsigned long long *statsOut) {
+	const int rc = checkTrace(s, rays, hits, count, query);
+	if (rc != LRH_OK) return rc;
 	if (count == 0) return LRH_OK;
 	lrh_device *dev = s->dev;
 	return guarded([&]() {
@@ -1256,6 +1289,12 @@ struct BatchTable {
 	bool two = false;
 	unsigned maxDepth = 0;
 	uint64_t total = 0;
+	// the distinct scenes of the batch with their generation at creation
+	struct SceneStamp {
+		const lrh_scene *scene;
+		uint64_t serial, generation;
+	};
+	std::vector<SceneStamp> sceneGen;
 };
 
 BatchTable makeTable(lrh_device *dev, const lrh_batch_item *items, unsigned n) {
@@ -1276,6 +1315,9 @@ BatchTable makeTable(lrh_device *dev, const lrh_batch_item *items, unsigned n) {
 		else if (s->layout != T.layout) throw Error(LRH_EINVAL, "all scenes of a batch must share the node layout");
 		T.two = T.two || s->twoLevel;
 		T.maxDepth = std::max(T.maxDepth, s->maxDepth);
+		bool seen = false;
+		for (const auto &sg : T.sceneGen) seen = seen || sg.scene == s;
+		if (!seen) T.sceneGen.push_back({s, s->serial, s->generation});
 		LrhBatchDesc &d = T.descs[i];
 		std::memset(&d, 0, sizeof(d));
 		d.nodes = s->nodes.p;
@@ -1309,6 +1351,22 @@ struct lrh_batch {
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
 };
+
+namespace {
+// a resident batch holds its scenes' device addresses and spill depth: it may
+// launch only while every scene is alive and unchanged in those respects
+void checkBatchScenes(const lrh_batch *b) {
+	std::lock_guard<std::mutex> g(b->dev->mu);
+	for (const auto &sg : b->T.sceneGen) {
+		if (!b->dev->scenes.count(const_cast<lrh_scene *>(sg.scene)) || sg.scene->serial != sg.serial)
+			throw Error(LRH_EINVAL, "a scene of the batch was destroyed: recreate the batch");
+		if (sg.scene->generation != sg.generation)
+			throw Error(LRH_EINVAL,
+					"a scene of the batch changed since the batch was created (an edit moved its device arrays "
+					"or deepened its tree): recreate the batch");
+	}
+}
+}  // namespace
 
 extern "C" {
 
@@ -1375,6 +1433,7 @@ int lrh_batch_launch(lrh_batch *b, void *stream) {
 	return guarded([&]() {
 		b->dev->activate();
 		if (b->T.total == 0) return;
+		checkBatchScenes(b);
 		hipStream_t st = stream ? (hipStream_t)stream : b->dev->stream;
 		HIPCHK(lrhLaunchTrace(b->T.layout, b->T.two, false, false, true, dim3(b->L.blocks), dim3(b->L.block),
 				b->L.ldsBytes, st, b->P));
@@ -1386,6 +1445,7 @@ int lrh_batch_capture(lrh_batch *b, void *stream) {
 	if (!b) return fail(LRH_EINVAL, "null batch");
 	return guarded([&]() {
 		b->dev->activate();
+		checkBatchScenes(b);
 		hipStream_t st = stream ? (hipStream_t)stream : b->dev->stream;
 		if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
 		if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
@@ -1406,6 +1466,7 @@ int lrh_batch_replay(lrh_batch *b, void *stream) {
 	if (!b->exec) return fail(LRH_EINVAL, "batch has no captured graph (lrh_batch_capture)");
 	return guarded([&]() {
 		b->dev->activate();
+		checkBatchScenes(b);
 		HIPCHK(hipGraphLaunch(b->exec, stream ? (hipStream_t)stream : b->dev->stream));
 		b->dev->rays += b->T.total;
 	});

@@ -282,6 +282,7 @@ __device__ __forceinline__ void traceBody(const LrhTraceParams &P) {
 	// wave passes, live lanes summed over passes, passes with a node test,
 	// passes with a leaf test, refills
 	uint32_t cNode = 0, cPrim = 0, cRay = 0, cPass = 0, cLive = 0, cNodePass = 0, cLeafPass = 0, cRefill = 0;
+	uint32_t cObj = 0, cXf = 0, cMot = 0;  // per lane: object entries, through a matrix, through a motion system
 
 
 	for (;;) {
@@ -463,6 +464,11 @@ __device__ __forceinline__ void traceBody(const LrhTraceParams &P) {
 			if (two && !inInst) {
 				// root leaf: enter the object's tree in its local space
 				const lrh_inst in = v.insts[start];
+				if (COUNT) {
+					++cObj;
+					cXf += in.kind == 1u ? 1u : 0u;
+					cMot += in.kind == 2u ? 1u : 0u;
+				}
 				if (in.kind == 1u) {
 					xfRay(v.xforms + 16 * (size_t)in.index, wo, wd, r);
 				} else if (in.kind == 2u) {
@@ -527,11 +533,14 @@ __device__ __forceinline__ void traceBody(const LrhTraceParams &P) {
 		} while ((uint32_t)__popcll(__ballot(rid != kNoRay)) >= minLive);
 	}
 	if (COUNT) {
-		unsigned long long a = cNode, b = cPrim, c = cRay;
+		unsigned long long a = cNode, b = cPrim, c = cRay, e = cObj, f = cXf, g = cMot;
 		for (int o = 32; o > 0; o >>= 1) {
 			a += __shfl_xor(a, o);
 			b += __shfl_xor(b, o);
 			c += __shfl_xor(c, o);
+			e += __shfl_xor(e, o);
+			f += __shfl_xor(f, o);
+			g += __shfl_xor(g, o);
 		}
 		if (lane == 0) {
 			// wave counters are uniform: lane 0 adds them once per wave
@@ -543,6 +552,11 @@ __device__ __forceinline__ void traceBody(const LrhTraceParams &P) {
 			atomicAdd(P.stats + 5, (unsigned long long)cNodePass);
 			atomicAdd(P.stats + 6, (unsigned long long)cLeafPass);
 			atomicAdd(P.stats + 7, (unsigned long long)cRefill);
+			if (TWO) {
+				atomicAdd(P.stats + 8, e);
+				atomicAdd(P.stats + 9, f);
+				atomicAdd(P.stats + 10, g);
+			}
 		}
 	}
 	// the last block to finish rewinds the ticket counter for the next launch

@@ -21,7 +21,7 @@ LRH_OK = 0
 LRH_EINVAL, LRH_EHIP, LRH_ENOMEM, LRH_ERANGE, LRH_ENODEV, LRH_EDEVICE = -1, -2, -3, -4, -5, -6
 LRH_H2D, LRH_D2H, LRH_D2D = 1, 2, 3
 LRH_QUERY_CLOSEST, LRH_QUERY_ANY = 0, 1
-TRACE_STATS = 8  # LRH_TRACE_STATS
+TRACE_STATS = 11  # LRH_TRACE_STATS
 ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "ERANGE", -5: "ENODEV", -6: "EDEVICE"}
 
 
@@ -81,6 +81,35 @@ LIGHT_DTYPE = np.dtype([("p0", "<f4", 3), ("e1", "<f4", 3), ("e2", "<f4", 3), ("
 PATH_DTYPE = np.dtype([("thr", "<f4", 3), ("rad", "<f4", 3), ("dl", "<f4", 3), ("depth", "<u4"),
                        ("alive", "<u4"), ("pad", "<u4")])
 assert MATERIAL_DTYPE.itemsize == 24 and LIGHT_DTYPE.itemsize == 48 and PATH_DTYPE.itemsize == 48
+
+
+class LrhServerConfig(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint), ("height", ctypes.c_uint), ("maxBounces", ctypes.c_uint),
+                ("rrStart", ctypes.c_uint), ("rrSurvive", ctypes.c_float), ("hbmBudget", ctypes.c_ulonglong),
+                ("maxRaysPerTick", ctypes.c_uint), ("cohorts", ctypes.c_uint), ("mode", ctypes.c_uint),
+                ("extent", ctypes.c_float), ("light", ctypes.c_float * 3), ("opts", LrhTraceOpts)]
+
+
+class LrhSessionSpec(ctypes.Structure):
+    _fields_ = [("sid", ctypes.c_uint), ("scene", ctypes.c_uint), ("seed", ctypes.c_ulonglong),
+                ("arrival", ctypes.c_double)]
+
+
+class LrhServerStats(ctypes.Structure):
+    _fields_ = [("sessionsDone", ctypes.c_ulonglong), ("sessionsQueued", ctypes.c_ulonglong),
+                ("ticks", ctypes.c_ulonglong), ("wavefronts", ctypes.c_ulonglong), ("liveRays", ctypes.c_ulonglong),
+                ("maxActive", ctypes.c_uint), ("sessionBuffersAllocated", ctypes.c_uint),
+                ("busySeconds", ctypes.c_double), ("traceSeconds", ctypes.c_double),
+                ("latencySamples", ctypes.c_ulonglong), ("p50Ms", ctypes.c_double), ("p99Ms", ctypes.c_double),
+                ("maxMs", ctypes.c_double), ("queueP99Ms", ctypes.c_double)]
+
+
+class LrhPlainBatchResult(ctypes.Structure):
+    _fields_ = [("buffers", ctypes.c_uint), ("rays", ctypes.c_ulonglong), ("liveRays", ctypes.c_ulonglong),
+                ("batchMs", ctypes.c_double), ("perBufferMs", ctypes.c_double)]
+
+
+SERVE_SHADE, SERVE_BOUNCE = 0, 1
 
 
 class LrhBatchItem(ctypes.Structure):
@@ -147,6 +176,24 @@ SIGNATURES = {
     "lrh_shade": ([_vp, _vp, ctypes.c_uint, _vp, ctypes.c_uint, _vp, _vp, _vp, _vp, _vp, _vp,
                    ctypes.POINTER(LrhShadeParams), _vp, _vp], ctypes.c_int),
     "lrh_event_query": ([_vp], ctypes.c_int),
+    "lrh_server_create": ([_vp, _pp, ctypes.c_uint, ctypes.POINTER(LrhServerConfig), _pp], ctypes.c_int),
+    "lrh_server_submit": ([_vp, ctypes.POINTER(LrhSessionSpec), ctypes.c_uint], ctypes.c_int),
+    "lrh_server_record": ([_vp, ctypes.c_uint], ctypes.c_int),
+    "lrh_server_tick": ([_vp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "lrh_server_run": ([_vp, ctypes.c_double, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "lrh_server_get_stats": ([_vp, ctypes.POINTER(LrhServerStats)], ctypes.c_int),
+    "lrh_server_latencies": ([_vp, ctypes.POINTER(ctypes.c_double), ctypes.c_size_t,
+                              ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
+    "lrh_server_recorded": ([_vp, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, _vp, _vp,
+                             ctypes.POINTER(ctypes.c_uint)], ctypes.c_int),
+    "lrh_server_radiance": ([_vp, ctypes.c_uint, ctypes.POINTER(ctypes.c_float), ctypes.c_uint], ctypes.c_int),
+    "lrh_server_capture": ([_vp, ctypes.c_ulonglong], ctypes.c_int),
+    "lrh_server_plain_batch": ([_vp, ctypes.POINTER(LrhTraceOpts), ctypes.c_uint, ctypes.POINTER(LrhPlainBatchResult)],
+                               ctypes.c_int),
+    "lrh_server_destroy": ([_vp], ctypes.c_int),
+    "lrh_autotune_batch": ([_vp, ctypes.POINTER(LrhBatchItem), ctypes.c_uint, ctypes.POINTER(LrhTraceOpts),
+                            ctypes.c_uint, ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(LrhTraceOpts),
+                            ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
     "lrh_device_stats": ([_vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_double)],
                          ctypes.c_int),
 }
@@ -521,7 +568,8 @@ class Device:
         check(lib().lrh_trace_stats(scene.h, rays_buf.ptr, hits_buf.ptr, count, query,
                                     ctypes.byref(opts) if opts else None, stream, out))
         return dict(zip(("node_visits", "prim_tests", "rays", "passes", "live_lane_passes", "node_passes",
-                         "leaf_passes", "refills"), (int(x) for x in out)))
+                         "leaf_passes", "refills", "object_entries", "matrix_entries", "motion_entries"),
+                        (int(x) for x in out)))
 
     def trace_batch(self, items, opts=None, stream=None):
         """One persistent launch over items (scene, rays, hits, count, query)."""

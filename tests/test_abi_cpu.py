@@ -281,3 +281,30 @@ def test_documented_limits_fail_cleanly():
         two.object_boxes(5)  # wrong object count
     arr, keep = D._object_structs(sc)
     assert L.lrh_scene_refit(two.h, arr, len(sc.obj_kind)) == -1  # host-only
+
+
+def _bound_probe_scene(keys, p):
+    """One motion object whose mesh is a single degenerate triangle at p (its
+    local box is the point p)."""
+    mesh = scenes.Mesh(np.repeat(p[None, :], 3, axis=0).astype(np.float32), np.array([[0, 1, 2]], np.uint32))
+    return scenes.InstancedScene([mesh], np.zeros(1, np.uint32), np.array([2], np.uint32),
+                                 np.zeros((1, 4, 4), np.float32), np.array([0, 2], np.uint32),
+                                 np.array([0.0, 1.0], np.float32), keys)
+
+
+@pytest.mark.parametrize("probe", ["A", "B"])
+def test_host_builder_motion_bound_closed_form(probe):
+    """The host builder's culling box of a moving object is MotionSystem::Bound
+    (1,025 samples per segment, motionsystem.cpp:75-87) expanded by
+    MachineEpsilon (mbvhaccel.cpp:192-193): checked against the closed form
+    of a point rotated about z (tests/test_oracle_golden.py)."""
+    from tests.test_oracle_golden import BOUND_PROBES, BOUND_TOL, _bound_err, bound_probe
+    keys, p, expected = bound_probe(BOUND_PROBES[probe])
+    h = D.host_scene_mbvh(_bound_probe_scene(keys, p))
+    box = h.object_boxes(1)[0]
+    lo, hi = oracle.motion_bound([0, 1], keys, p, p)
+    e = oracle.epsilon_bbox(lo, hi)
+    assert np.array_equal(box[0], (lo - e).astype(np.float32)) and np.array_equal(box[1], (hi + e).astype(np.float32))
+    elo, ehi = expected(1024)
+    assert _bound_err(box[0] + e, box[1] - e, (elo, ehi)) <= BOUND_TOL + 1e-6
+    assert _bound_err(box[0] + e, box[1] - e, expected(1023)) > 2 * BOUND_TOL

@@ -137,11 +137,87 @@ def test_batch_masked_empty_and_two_level(gpu_device):
 def test_autotuner_picks_and_caches(gpu_device, small_batch, tmp_path):
     _, wl = small_batch
     _case("autotune")
-    path = str(tmp_path / "tune.json")
+    path = str(tmp_path / "tune.txt")
     shapes = [(256, 8, 12), (128, 8, 8), (256, 4, 16)]
-    opts, table = B.autotune(gpu_device, wl.items(), shapes=shapes, reps=2, cache_path=path, key="k1")
+    opts, table = B.autotune(gpu_device, wl.items(), shapes=shapes, cache_path=path, key="k1")
     assert len(table) == 3 and os.path.exists(path)
     best = min(table, key=lambda r: r["ms"])
     assert (opts.blockSize, opts.ldsStack, opts.refillThreshold) == (best["block"], best["lds"], best["refill"])
-    opts2, _ = B.autotune(gpu_device, wl.items(), shapes=shapes, reps=2, cache_path=path, key="k1")
+    # the second call answers from the cache and times nothing
+    opts2, table2 = B.autotune(gpu_device, wl.items(), shapes=shapes, cache_path=path, key="k1")
+    assert table2 == []
     assert (opts2.blockSize, opts2.ldsStack, opts2.refillThreshold) == (best["block"], best["lds"], best["refill"])
+    # another key is tuned and stored beside the first
+    B.autotune(gpu_device, wl.items(), shapes=shapes[:1], cache_path=path, key="k2")
+    with open(path) as f:
+        assert sorted(ln.split()[0] for ln in f if ln.strip()) == ["k1", "k2"]
+
+
+def _more_keys(sc, n_keys=4, seed=71):
+    """The same objects with n_keys motion keys each (the arrays of motion
+    systems grow, so an edit with them moves the scene's device arrays)."""
+    import copy
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = copy.deepcopy(sc)
+    times, mats, off = [], [], [0]
+    for o in range(len(sc.obj_kind)):
+        if sc.obj_kind[o] == 2:
+            k0 = int(sc.key_off[o])
+            M0 = np.linalg.inv(sc.key_mats[k0].astype(np.float64))
+            for k in range(n_keys):
+                Mk = M0.copy()
+                Mk[:3, 3] += rng.uniform(-2.0, 2.0, size=3)
+                times.append(k / (n_keys - 1))
+                mats.append(np.linalg.inv(Mk).astype(np.float32))
+            off.append(off[-1] + n_keys)
+        else:
+            off.append(off[-1])
+    out.key_off = np.asarray(off, np.uint32)
+    out.key_times = np.asarray(times, np.float32)
+    out.key_mats = np.asarray(mats, np.float32).reshape(-1, 4, 4)
+    return out
+
+
+def test_resident_batch_refuses_changed_or_destroyed_scene(gpu_device):
+    """A resident batch keeps its scenes' device addresses: after an edit that
+    moves them (more motion keys) it refuses to launch or replay, as it does
+    after a scene is destroyed; a refit that moves nothing leaves it valid."""
+    _case("batch_stale_scene")
+    dev = gpu_device
+    sc = scenes.instanced_scene(n_inst=40, motion_frac=0.5, freq=8, extent=20.0, seed=70)
+    dsc = dev.scene_mbvh(sc)
+    flat = dev.scene_bvh(scenes.plain_scene(seed=1, n_random=100))
+    rays = scenes.instanced_rays(4096, extent=20.0, seed=72)
+    rb = dev.upload(rays)
+    hb = dev.alloc(rays.shape[0] * 20)
+    hb2 = dev.alloc(rays.shape[0] * 20)
+    b = dev.batch([(dsc, rb, hb, rays.shape[0], 0), (flat, rb, hb2, rays.shape[0], 0)])
+    b.launch()
+    b.capture()
+    b.replay()
+    dev.sync()
+    # a refit with the same key counts: arrays rewritten in place, the tree
+    # keeps its topology and depth, so the batch stays valid
+    same = _more_keys(sc, n_keys=2, seed=73)
+    dsc.refit(same)
+    b.launch()
+    dev.sync()
+    got = dev.download(hb, HIT_DTYPE, rays.shape[0])
+    ref = oracle.OracleMBVH(same).intersect(rays)
+    assert_parity(compare(ref, got))
+    # more keys: the motion arrays grow and move
+    dsc.update(_more_keys(sc, n_keys=5, seed=74))
+    with pytest.raises(D.LrhError, match="recreate the batch"):
+        b.launch()
+    with pytest.raises(D.LrhError, match="recreate the batch"):
+        b.replay()
+    b.close()
+    # a fresh batch over the edited scene traces again
+    b2 = dev.batch([(flat, rb, hb2, rays.shape[0], 0)])
+    b2.launch()
+    dev.sync()
+    flat.close()
+    with pytest.raises(D.LrhError, match="destroyed"):
+        b2.launch()
+    b2.close()
+    dsc.close()

@@ -51,7 +51,7 @@ def _check_trace(dev, dev_scene, sc, rays):
     ob = oracle.OracleMBVH(sc)
     ref = ob.intersect(rays)
     got = dev.trace_numpy(dev_scene, rays)
-    st = compare(ref, got, mint=rays["mint"])
+    st = compare(ref, got)
     print(st, flush=True)
     assert_parity(st)
     return st
@@ -120,4 +120,47 @@ def test_refit_rejects_bad_edits(gpu_device):
     dsc.refit(ok)
     _check_boxes(dsc, ok)
     _check_trace(gpu_device, dsc, ok, scenes.instanced_rays(50000, extent=10.0, seed=53))
+    dsc.close()
+
+
+def test_rejected_refit_keeps_last_accepted_boxes(gpu_device):
+    """A good refit, then one with a singular motion sample: the rejected
+    edit leaves the object boxes (and traces) of the good one."""
+    _case("refit_rejected_keeps_boxes")
+    sc = scenes.instanced_scene(n_inst=30, motion_frac=0.5, freq=8, extent=10.0, seed=61)
+    dsc = gpu_device.scene_mbvh(sc)
+    good = moved(sc, 62, extent=10.0)
+    dsc.refit(good)
+    before = dsc.object_boxes(len(sc.obj_kind)).copy()
+    _check_boxes(dsc, good)
+    sing = moved(sc, 63, extent=10.0)
+    j = int(np.nonzero(sing.obj_kind == 2)[0][0])
+    sing.key_mats[int(sing.key_off[j])] = np.zeros((4, 4), np.float32)
+    with pytest.raises(D.LrhError):
+        dsc.refit(sing)
+    after = dsc.object_boxes(len(sc.obj_kind))
+    assert np.array_equal(before, after)
+    _check_boxes(dsc, good)
+    _check_trace(gpu_device, dsc, good, scenes.instanced_rays(50000, extent=10.0, seed=64))
+    dsc.close()
+
+
+@pytest.mark.parametrize("probe", ["A", "B"])
+def test_refit_motion_bound_closed_form(gpu_device, probe):
+    """The GPU refit's box of a moving object against the closed form of a
+    point rotated about z (MotionSystem::Bound, 1,025 samples, then
+    MachineEpsilon), and equal to the host builder's bit for bit."""
+    from tests.test_abi_cpu import _bound_probe_scene
+    from tests.test_oracle_golden import BOUND_PROBES, BOUND_TOL, _bound_err, bound_probe
+    _case("refit_bound_closed_form", probe=probe)
+    keys, p, expected = bound_probe(BOUND_PROBES[probe])
+    sc = _bound_probe_scene(keys, p)
+    dsc = gpu_device.scene_mbvh(sc)
+    dsc.refit(sc)
+    box = dsc.object_boxes(1)[0]
+    assert np.array_equal(box, D.host_scene_mbvh(sc).object_boxes(1)[0])
+    lo, hi = oracle.motion_bound([0, 1], keys, p, p)
+    e = oracle.epsilon_bbox(lo, hi)
+    assert _bound_err(box[0] + e, box[1] - e, expected(1024)) <= BOUND_TOL + 1e-6
+    assert _bound_err(box[0] + e, box[1] - e, expected(1023)) > 2 * BOUND_TOL
     dsc.close()
