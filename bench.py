@@ -448,7 +448,6 @@ def run_configs(args, dev):
                    light (any-hit);
       instanced    1,000 instances of a 50,000-triangle sphere, 200 with 2-key
                    motion, 4,194,304 rays with time ~ U[0,1]."""
-    import oracle  # the checker, timed here only as the CPU baseline
     threads = os.cpu_count() or 1
     out = {}
 
@@ -473,7 +472,26 @@ def run_configs(args, dev):
         v, passes, spent = _timed_passes(ob, sample, threads, seconds, min_passes=3)
         return {"mrays": round(v, 3), "sample_rays": int(sample.shape[0]), "passes": passes, "threads": threads}
 
-    # plain
+    sets = {x.strip() for x in args.sets.split(",") if x.strip()}
+    if not sets <= {"plain", "heightfield", "instanced"}:
+        raise SystemExit(f"--sets: unknown workload in {args.sets}")
+    if "plain" in sets:
+        _configs_plain(args, dev, rate, cpu, out)
+    if "heightfield" in sets:
+        _configs_heightfield(dev, rate, out)
+    if "instanced" in sets:
+        _configs_instanced(args, dev, rate, cpu, out)
+    value = out["heightfield"]["bounce_closest"]["mrays"] if "heightfield" in out else None
+    return {"metric": "Mrays/s per workload (closest / any hit)", "value": value,
+            "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "plain 101,932 tris / heightfield 26.2M tris / 1,000 instances",
+                       "sets": sorted(sets)}, "configs": out}
+
+
+def _configs_plain(args, dev, rate, cpu, out):
+    import oracle  # the checker, timed here only as the CPU baseline
+    threads = os.cpu_count() or 1
     meshes = scenes.plain_scene()
     sc = dev.scene_bvh(meshes)
     rays = scenes.plain_rays(1 << 20)
@@ -483,7 +501,9 @@ def run_configs(args, dev):
         out["plain"]["cpu_closest"] = cpu(oracle.OracleBVH(meshes, threads=threads), rays)
     sc.close()
     log(f"plain: {out['plain']}")
-    # heightfield
+
+
+def _configs_heightfield(dev, rate, out):
     meshes = scenes.heightfield_scene()
     sc = dev.scene_bvh(meshes)
     prim = scenes.pinhole_rays(1920, 1080)
@@ -495,7 +515,11 @@ def run_configs(args, dev):
                           "shadow_closest": rate(sc, shadow, 0)}
     sc.close()
     log(f"heightfield: {out['heightfield']}")
-    # instanced
+
+
+def _configs_instanced(args, dev, rate, cpu, out):
+    import oracle  # the checker, timed here only as the CPU baseline
+    threads = os.cpu_count() or 1
     isc = scenes.instanced_scene()
     sc = dev.scene_mbvh(isc)
     rays = scenes.instanced_rays(1 << 22)
@@ -505,10 +529,6 @@ def run_configs(args, dev):
         out["instanced"]["cpu_closest"] = cpu(oracle.OracleMBVH(isc, threads=threads), rays)
     sc.close()
     log(f"instanced: {out['instanced']}")
-    return {"metric": "Mrays/s per workload (closest / any hit)", "value": out["heightfield"]["bounce_closest"]["mrays"],
-            "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
-            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-            "config": {"workload": "plain 101,932 tris / heightfield 26.2M tris / 1,000 instances"}, "configs": out}
 
 
 def run_render(args, dev):
@@ -584,6 +604,21 @@ def run_render(args, dev):
     return out
 
 
+def calibration_copy(args, dev):
+    """A device-to-device copy of --calib-copy MiB after the timed work: a
+    kernel of known traffic that scales rocprofv3's FETCH_SIZE / WRITE_SIZE."""
+    if not args.calib_copy:
+        return
+    nb = args.calib_copy << 20
+    src = dev.alloc(nb)
+    dst = dev.alloc(nb)
+    D.check(D.lib().lrh_memcpy(dev.h, dst.ptr, src.ptr, nb, D.LRH_D2D, None))
+    dev.sync()
+    src.free()
+    dst.free()
+    log(f"calibration copy: {args.calib_copy} MiB device-to-device")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -595,6 +630,8 @@ def main():
                          "refit: two-level edit latency, host rebuild (lrh_scene_update) vs GPU refit; "
                          "configs: closest/any-hit rates of the plain, heightfield and instanced workloads; "
                          "render: path tracing of a scene file with lrh_shade (samples per second)")
+    ap.add_argument("--sets", default="plain,heightfield,instanced",
+                    help="--mode configs: which workloads (comma list of plain, heightfield, instanced)")
     ap.add_argument("--scene", default=None, help="scene file for --mode scene (.scn/.cfg)")
     ap.add_argument("--scene-rays", type=int, default=1 << 22, help="incoherent rays for --mode scene")
     ap.add_argument("--steps", type=int, default=None,
@@ -629,24 +666,15 @@ def main():
         args.steps = 1000 if args.mode == "headline" else 20
     dev = D.Device(0)
     log(f"device {dev.name}, {dev.cu_count} CUs")
-    if args.mode == "batched":
-        print(json.dumps(run_batched(args, dev)), flush=True)
-        return
+    modes = {"batched": run_batched, "scene": run_scene, "refit": run_refit, "configs": run_configs,
+             "render": run_render}
     if args.mode == "served":
         from luxcore_amd import serve
-        print(json.dumps(serve.bench(args, dev, log)), flush=True)
-        return
-    if args.mode == "scene":
-        print(json.dumps(run_scene(args, dev)), flush=True)
-        return
-    if args.mode == "refit":
-        print(json.dumps(run_refit(args, dev)), flush=True)
-        return
-    if args.mode == "configs":
-        print(json.dumps(run_configs(args, dev)), flush=True)
-        return
-    if args.mode == "render":
-        print(json.dumps(run_render(args, dev)), flush=True)
+        modes["served"] = lambda a, d: serve.bench(a, d, log)
+    if args.mode in modes:
+        out = modes[args.mode](args, dev)
+        calibration_copy(args, dev)
+        print(json.dumps(out), flush=True)
         return
     meshes, scene, info, rays, ntris = build_workload(args, dev)
     n = rays.shape[0]
@@ -689,12 +717,7 @@ def main():
         f"{np.percentile(lat_p, 50):.3f} p99 {np.percentile(lat_p, 99):.3f} ms, hit rate "
         f"{(hits['meshIndex'] != NULL_INDEX).mean():.3f}")
 
-    if args.calib_copy:
-        nb = args.calib_copy << 20
-        src = dev.alloc(nb)
-        dst = dev.alloc(nb)
-        D.check(D.lib().lrh_memcpy(dev.h, dst.ptr, src.ptr, nb, D.LRH_D2D, None))
-        log(f"calibration copy: {args.calib_copy} MiB device-to-device")
+    calibration_copy(args, dev)
 
     cpu = None
     if not args.no_cpu:
