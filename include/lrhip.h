@@ -226,6 +226,22 @@ int lrh_trace_ex(lrh_scene *scene, const void *rays, void *hits, unsigned int co
 #define LRH_TRACE_STATS 11
 int lrh_trace_stats(lrh_scene *scene, const void *rays, void *hits, unsigned int count,
 		unsigned int query, const lrh_trace_opts *opts, void *stream, unsigned long long out[LRH_TRACE_STATS]);
+/* lrh_trace that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory while they trace (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and asynchrony as lrh_trace with the default launch
+ * shape; the ticket counter and the spill stack live in one static scratch
+ * array of LRH_NOALLOC_SCRATCH_BYTES per GPU (held from the library's first
+ * use on that GPU until the process ends), whose counter each call clears
+ * with a fill kernel.  The persistent grid is capped at the lanes whose spill
+ * stacks fit the array (3·(treeDepth+2)+4-8 entries of 8 bytes per lane), so
+ * deep trees run fewer lanes than lrh_trace; a tree too deep for even one
+ * block of 256 lanes returns LRH_ERANGE.  Every call on one GPU shares the
+ * array: calls that overlap on a GPU (other streams, other threads) race and
+ * must be serialised by the caller. */
+#define LRH_NOALLOC_SCRATCH_BYTES (128u * 1024u * 1024u)
+int lrh_trace_noalloc(lrh_scene *scene, const void *rays, void *hits, unsigned int count,
+		unsigned int query, void *stream);
 /* Many ray buffers over one or more resident scenes of the same device in one
  * persistent launch: the rays of all items form one ticket space, every lane
  * takes the next ticket, finds its item and traces in that item's scene with

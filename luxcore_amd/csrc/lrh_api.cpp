@@ -78,6 +78,36 @@ double now() {
 
 }  // namespace
 
+// Scratch memory of lrh_trace_noalloc: the ray-ticket counter (first 256
+// bytes) and the spill stack, in one statically sized array that every GPU
+// the library runs on holds from the loading of this code object until the
+// process ends.  In host code `scratch` is only the host-side stand-in of the
+// array; its device address on each GPU comes from hipGetSymbolAddress,
+// taken once per device in lrh_device_open with that device current.
+static __device__ __align__(256) unsigned char scratch[LRH_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr size_t kScratchCounterBytes = 256;
+constexpr int kMaxOrdinals = 64;
+std::mutex g_scratchMu;
+void *g_scratchPtr[kMaxOrdinals] = {};
+
+// device address of `scratch` on the current device (ordinal), resolved once
+int resolveScratch(int ordinal, void **out) {
+	if (ordinal < 0 || ordinal >= kMaxOrdinals) return fail(LRH_ERANGE, "device ordinal beyond the scratch table");
+	std::lock_guard<std::mutex> g(g_scratchMu);
+	if (!g_scratchPtr[ordinal]) {
+		void *p = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (e != hipSuccess || !p)
+			return fail(LRH_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+		g_scratchPtr[ordinal] = p;
+	}
+	*out = g_scratchPtr[ordinal];
+	return LRH_OK;
+}
+}  // namespace
+
 // the message of lrh_last_error for the library's other host files
 int lrhSetError(int code, const std::string &msg) { return fail(code, msg); }
 
@@ -110,6 +140,8 @@ struct lrh_device {
 	std::set<lrh_scene *> scenes;     // live scenes (destroyed with the device)
 	std::atomic<unsigned long long> rays{0};
 	double t0 = 0;
+	void *scratch = nullptr;  // device address of the no-alloc scratch array on this GPU
+	std::string scratchError;  // why it could not be resolved
 
 	void activate() const { HIPCHK(hipSetDevice(ordinal)); }
 	void *devAlloc(size_t bytes) {
@@ -563,6 +595,9 @@ int lrh_device_open(int ordinal, lrh_device **out) {
 		d->dError = (uint32_t *)d->devAlloc(sizeof(uint32_t));
 		HIPCHK(hipMemsetAsync(d->dError, 0, sizeof(uint32_t), d->stream));
 		HIPCHK(hipStreamSynchronize(d->stream));
+		// the no-alloc scratch of this GPU (device current: its own copy); a
+		// failure here only disables lrh_trace_noalloc, which reports it
+		if (resolveScratch(ordinal, &d->scratch) != LRH_OK) d->scratchError = g_lastError;
 		d->t0 = now();
 	});
 	if (rc != LRH_OK) return rc;
@@ -1384,6 +1419,52 @@ int lrh_trace_stats(lrh_scene *s, const void *rays, void *hits, unsigned int cou
 	if (!out) return fail(LRH_EINVAL, "null stats output");
 	for (int i = 0; i < LRH_TRACE_STATS; ++i) out[i] = 0;
 	return traceImpl(s, rays, hits, count, query, opts, stream, out);
+}
+
+int lrh_trace_noalloc(lrh_scene *s, const void *rays, void *hits, unsigned int count, unsigned int query,
+		void *stream) {
+	const int rc = checkTrace(s, rays, hits, count, query);
+	if (rc != LRH_OK) return rc;
+	if (count == 0) return LRH_OK;
+	lrh_device *dev = s->dev;
+	if (!dev->scratch) return fail(LRH_EHIP, "the no-alloc scratch is unavailable on this device: " + dev->scratchError);
+	return guarded([&]() {
+		dev->activate();
+		hipStream_t st = stream ? (hipStream_t)stream : dev->stream;
+		const bool any = query == LRH_QUERY_ANY;
+		LaunchShape L = chooseShape(dev, s->layout, s->twoLevel, any, false, count, s->maxDepth, 0, nullptr);
+		// the persistent grid is capped at the lanes whose spill stacks fit
+		const size_t blockBytes = (size_t)L.spillDepth * sizeof(uint2) * L.block;
+		const size_t maxBlocks = (LRH_NOALLOC_SCRATCH_BYTES - kScratchCounterBytes) / blockBytes;
+		if (maxBlocks == 0)
+			throw Error(LRH_ERANGE, "tree depth " + std::to_string(s->maxDepth) + " needs " +
+					std::to_string(blockBytes) + " bytes of spill stack per block, beyond the " +
+					std::to_string((unsigned long long)LRH_NOALLOC_SCRATCH_BYTES) + "-byte no-alloc scratch");
+		L.blocks = (unsigned)std::min<size_t>(L.blocks, maxBlocks);
+		uint32_t *counter = static_cast<uint32_t *>(dev->scratch);
+		uint2 *spill = reinterpret_cast<uint2 *>(static_cast<unsigned char *>(dev->scratch) + kScratchCounterBytes);
+		// the ticket counters start at zero whatever the last call left (a fill
+		// kernel, not a memset node, so the call may be graph-captured)
+		HIPCHK(lrhLaunchReset(counter, 2, st));
+		LrhTraceParams P;
+		std::memset(&P, 0, sizeof(P));
+		fillSceneParams(P, s);
+		P.rays = (const lrh_ray_rec *)rays;
+		P.hits = (lrh_hit_rec *)hits;
+		P.nRays = count;
+		P.counter = counter;
+		P.spill = spill;
+		P.spillDepth = L.spillDepth;
+		P.nLanes = L.blocks * L.block;
+		P.ldsDepth = L.lds;
+		P.refill = L.refill;
+		P.leafTris = L.leafTris;
+		P.error = dev->dError;
+		P.stats = nullptr;
+		HIPCHK(lrhLaunchTrace(s->layout, s->twoLevel, any, false, false, dim3(L.blocks), dim3(L.block), L.ldsBytes, st,
+				P));
+		dev->rays += count;
+	});
 }
 
 int lrh_batch_create(lrh_device *dev, const lrh_batch_item *items, unsigned int nItems, const lrh_trace_opts *opts,

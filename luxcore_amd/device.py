@@ -158,6 +158,7 @@ SIGNATURES = {
                      ctypes.c_int),
     "lrh_trace_stats": ([_vp, _vp, _vp, ctypes.c_uint, ctypes.c_uint, ctypes.POINTER(LrhTraceOpts), _vp,
                          ctypes.POINTER(ctypes.c_ulonglong)], ctypes.c_int),
+    "lrh_trace_noalloc": ([_vp, _vp, _vp, ctypes.c_uint, ctypes.c_uint, _vp], ctypes.c_int),
     "lrh_trace_batch": ([_vp, ctypes.POINTER(LrhBatchItem), ctypes.c_uint, ctypes.POINTER(LrhTraceOpts), _vp],
                         ctypes.c_int),
     "lrh_batch_create": ([_vp, ctypes.POINTER(LrhBatchItem), ctypes.c_uint, ctypes.POINTER(LrhTraceOpts), _pp],
