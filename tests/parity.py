@@ -7,6 +7,8 @@ an exact tie in t (both triangles accept the ray at the same t; which one is
 kept depends on visit order).  Where the indices agree, t, b1 and b2 must
 agree within 1e-5 relative.
 """
+import hashlib
+
 import numpy as np
 
 from luxcore_amd.raytypes import NULL_INDEX
@@ -95,3 +97,14 @@ def tri_tie_checker(oracle_mod, meshes, rays, ref, got):
         hit, tt, _, _ = oracle_mod.triangle_intersect(rays[i], v[tv[0]], v[tv[1]], v[tv[2]])
         return hit and tt == ref["t"][i]
     return check
+
+
+def chunk_digests(hits, chunk):
+    """SHA-1 of each run of `chunk` hit records, normalized: a miss keeps its
+    t and null indices, its barycentrics count as zero (the oracle and the
+    device may leave different values there)."""
+    h = np.array(hits, copy=True)
+    miss = h["meshIndex"] == NULL_INDEX
+    h["b1"][miss] = 0.0
+    h["b2"][miss] = 0.0
+    return [hashlib.sha1(h[i:i + chunk].tobytes()).hexdigest() for i in range(0, h.shape[0], chunk)]

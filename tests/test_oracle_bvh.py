@@ -195,3 +195,21 @@ def test_mbvh_equals_bruteforce_with_motion(builder):
     assert st["mismatch"] == st["ties"] + st["edge_band"] and st["bad"] == 0, st
     assert_parity(st, rel_tol=0.0)
     assert st["hits"] > 300
+
+
+def test_plain_full_digest_fixture_matches_oracle():
+    """The committed digests of the oracle's hits on the full plain workload
+    (used by the GPU test of all 1,048,576 rays) are the oracle's: checked on
+    the first, a middle and the last chunk."""
+    import json
+    import os
+    from tests.parity import chunk_digests
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "plain_full_digest.json")) as f:
+        gold = json.load(f)
+    meshes = scenes.plain_scene(seed=1)
+    rays = scenes.plain_rays(gold["n_rays"], seed=2)
+    ob = oracle.OracleBVH(meshes)
+    c = gold["chunk"]
+    for i in (0, 97, len(gold["sha1"]) - 1):
+        hits = ob.intersect(rays[i * c:(i + 1) * c])
+        assert chunk_digests(hits, c) == [gold["sha1"][i]], i
