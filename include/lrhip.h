@@ -94,8 +94,11 @@ typedef struct {
  * builder: LRH_BUILD_HOST_SAH (default) builds on the host (binned SAH, the
  * stand-in for the reference's EMBREE_BINNED_SAH, bvhaccel.cpp:144-156);
  * LRH_BUILD_DEVICE_LBVH builds on the GPU (Morton order + radix tree,
- * collapsed to the same 4-wide records; single-level scenes with a device);
- * it is switched off in this build and returns LRH_EINVAL. */
+ * collapsed to the same 4-wide records, the triangle records bit for bit the
+ * host's; single-level scenes with a device; no early split clipping, so
+ * splitBudget is ignored; the replacement of the reference's EMBREE_MORTON
+ * builder).  A violated internal precondition of the device build (codes out
+ * of order, a malformed radix tree) returns LRH_EHIP with the reason. */
 #define LRH_BUILD_HOST_SAH 0u
 #define LRH_BUILD_DEVICE_LBVH 1u
 typedef struct {

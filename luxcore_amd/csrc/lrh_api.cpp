@@ -779,8 +779,6 @@ int lrh_event_elapsed(lrh_event *a, lrh_event *b, float *ms) {
 
 namespace {
 
-constexpr bool kDeviceBuilderEnabled = false;
-
 void *gbAlloc(void *ctx, size_t bytes) { return static_cast<lrh_device *>(ctx)->devAlloc(bytes); }
 void gbRelease(void *ctx, void *p, size_t bytes) { static_cast<lrh_device *>(ctx)->devFree(p, bytes); }
 
@@ -866,14 +864,8 @@ int lrh_scene_create_bvh(lrh_device *dev, const lrh_mesh *meshes, unsigned int n
 		if (builder > LRH_BUILD_DEVICE_LBVH) throw Error(LRH_EINVAL, "builder must be LRH_BUILD_HOST_SAH or LRH_BUILD_DEVICE_LBVH");
 		if (builder == LRH_BUILD_DEVICE_LBVH) {
 			if (!dev) throw Error(LRH_EINVAL, "the device builder needs a device");
-			// switched off: its first device run stopped making progress inside
-			// the build and the cause has not been shown (the code is kept and
-			// compiled; deviceBuild is not called)
-			if (kDeviceBuilderEnabled) {
-				deviceBuild(s.get(), meshes, nMeshes);
-				return;
-			}
-			throw Error(LRH_EINVAL, "the device builder is disabled in this build");
+			deviceBuild(s.get(), meshes, nMeshes);
+			return;
 		}
 		const double t0 = now();
 		HostTree ht;
@@ -1338,22 +1330,36 @@ BatchTable makeTable(lrh_device *dev, const lrh_batch_item *items, unsigned n) {
 	if (n > 4096) throw Error(LRH_ERANGE, "at most 4096 items per batch");
 	T.descs.resize(n);
 	T.starts.resize(n + 1);
-	for (unsigned i = 0; i < n; ++i) {
+	for (unsigned i = 0; i < n; ++i)
+		if (!items[i].scene || items[i].scene->dev != dev)
+			throw Error(LRH_EINVAL, "batch item " + std::to_string(i) + ": scene of another device");
+	// ticket order: closest-hit items before any-hit ones, larger scenes first
+	// and the items of one scene together, so the costliest rays start first
+	// (the launch ends on short rays) and the grid works on one tree at a time
+	std::vector<unsigned> order(n);
+	for (unsigned i = 0; i < n; ++i) order[i] = i;
+	std::stable_sort(order.begin(), order.end(), [items](unsigned a, unsigned b) {
+		const lrh_batch_item &x = items[a], &y = items[b];
+		if ((x.query == LRH_QUERY_ANY) != (y.query == LRH_QUERY_ANY)) return y.query == LRH_QUERY_ANY;
+		if (x.scene->info.primCount != y.scene->info.primCount) return x.scene->info.primCount > y.scene->info.primCount;
+		return x.scene->serial < y.scene->serial;
+	});
+	for (unsigned j = 0; j < n; ++j) {
+		const unsigned i = order[j];
 		const lrh_batch_item &it = items[i];
 		const lrh_scene *s = it.scene;
-		if (!s || s->dev != dev) throw Error(LRH_EINVAL, "batch item " + std::to_string(i) + ": scene of another device");
 		if (it.count && (!it.rays || !it.hits)) throw Error(LRH_EINVAL, "batch item " + std::to_string(i) + ": null buffer");
 		if (((uintptr_t)it.rays & 15u) || ((uintptr_t)it.hits & 3u))
 			throw Error(LRH_EINVAL, "batch item " + std::to_string(i) + ": misaligned buffer");
 		if (it.query > 1) throw Error(LRH_EINVAL, "batch item " + std::to_string(i) + ": bad query");
-		if (i == 0) T.layout = s->layout;
+		if (j == 0) T.layout = s->layout;
 		else if (s->layout != T.layout) throw Error(LRH_EINVAL, "all scenes of a batch must share the node layout");
 		T.two = T.two || s->twoLevel;
 		T.maxDepth = std::max(T.maxDepth, s->maxDepth);
 		bool seen = false;
 		for (const auto &sg : T.sceneGen) seen = seen || sg.scene == s;
 		if (!seen) T.sceneGen.push_back({s, s->serial, s->generation});
-		LrhBatchDesc &d = T.descs[i];
+		LrhBatchDesc &d = T.descs[j];
 		std::memset(&d, 0, sizeof(d));
 		d.nodes = s->nodes.p;
 		d.tris = (const lrh_tri *)s->tris.p;
@@ -1367,7 +1373,7 @@ BatchTable makeTable(lrh_device *dev, const lrh_batch_item *items, unsigned n) {
 		d.root = s->root;
 		d.count = it.count;
 		d.flags = (it.query == LRH_QUERY_ANY ? LRH_ITEM_ANY : 0u) | (s->twoLevel ? LRH_ITEM_TWO : 0u);
-		T.starts[i] = (uint32_t)T.total;
+		T.starts[j] = (uint32_t)T.total;
 		T.total += it.count;
 		if (T.total > 0x7fffffffull) throw Error(LRH_ERANGE, "at most 2^31-1 rays per batch");
 	}

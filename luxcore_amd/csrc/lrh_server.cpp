@@ -148,6 +148,7 @@ double percentile(std::vector<double> v, double q) {
 struct lrh_server {
 	lrh_device *dev = nullptr;
 	std::vector<lrh_scene *> scenes;
+	std::map<const lrh_scene *, unsigned long long> weight;  // records per scene: the cost order of a tick's items
 	lrh_server_config cfg{};
 	unsigned n = 0;                       // rays per wavefront (width * height)
 	size_t sessionBytes = 0;
@@ -358,6 +359,13 @@ struct lrh_server {
 		void *st = streams[c];
 		items.clear();
 		for (Session *s : coh) sessionItems(s);
+		// the costliest rays take the first tickets (closest-hit rays before
+		// shadow rays, larger scenes first), so the end of the launch is
+		// spent on short rays and its tail is short
+		std::stable_sort(items.begin(), items.end(), [this](const lrh_batch_item &a, const lrh_batch_item &b) {
+			if (a.query != b.query) return a.query < b.query;
+			return weight.at(a.scene) > weight.at(b.scene);
+		});
 		bool anyRecord = false;
 		for (Session *s : coh) {
 			if (!s->record) continue;
@@ -468,6 +476,9 @@ int lrh_server_create(lrh_device *dev, lrh_scene *const *scenes, unsigned int nS
 		for (unsigned i = 0; i < nScenes; ++i) {
 			if (!scenes[i]) throw SrvError(LRH_EINVAL, "null scene " + std::to_string(i));
 			s->scenes.push_back(scenes[i]);
+			lrh_scene_info info{};
+			chk(lrh_scene_get_info(scenes[i], &info), "lrh_scene_get_info");
+			s->weight[scenes[i]] = info.primCount;
 		}
 		lrh_server_config c{};
 		if (cfg) c = *cfg;
