@@ -729,7 +729,8 @@ def main():
         log(f"cpu baseline {cpu['value']:.3f} Mrays/s on {cpu['cores']} threads")
 
     traffic, traffic_source = None, None
-    if args.traffic_file and os.path.exists(args.traffic_file) and args.cells == 3240 and args.layout == 0:
+    if (args.traffic_file and os.path.exists(args.traffic_file) and args.cells == 3240 and args.layout == 0
+            and args.builder == "sah"):
         with open(args.traffic_file) as f:
             tr = json.load(f)
         traffic = round(tr["traffic_bytes_per_ray"] * n)
