@@ -41,7 +41,7 @@ def build_workload(args, dev):
     meshes = scenes.heightfield_scene(cells=args.cells, n_spheres=64, seed=3)
     ntris = sum(m.ntris for m in meshes)
     log(f"scene: {ntris} tris in {len(meshes)} meshes ({time.time() - t0:.1f}s)")
-    builder = D.BUILD_DEVICE_LBVH if args.builder == "lbvh" else D.BUILD_HOST_SAH
+    builder = {"sah": D.BUILD_HOST_SAH, "lbvh": D.BUILD_DEVICE_LBVH, "ploc": D.BUILD_DEVICE_PLOC}[args.builder]
     scene = dev.scene_bvh(meshes, layout=args.layout, max_leaf=args.max_leaf, builder=builder)
     info = scene.info()
     log(f"device tree ({args.builder}): {info['nodeCount']} nodes, {info['nodeBytes'] / 2**20:.0f} MiB nodes, "
@@ -643,8 +643,9 @@ def main():
     ap.add_argument("--cells", type=int, default=3240)
     ap.add_argument("--layout", type=int, default=0)
     ap.add_argument("--max-leaf", type=int, default=4)
-    ap.add_argument("--builder", choices=("sah", "lbvh"), default="sah",
-                    help="headline tree: host binned SAH (default) or the device LBVH build")
+    ap.add_argument("--builder", choices=("sah", "lbvh", "ploc"), default="sah",
+                    help="headline tree: host binned SAH (default), or a device build: LBVH (radix tree) or "
+                         "PLOC (clustering)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--lds", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
@@ -756,7 +757,7 @@ def main():
         "config": {"workload": "heightfield 3240x3240 fBm + 64 L6 icospheres, 1920x1080 primaries -> 1 cosine "
                                "bounce per hit (incoherent closest-hit)",
                    "triangles": ntris, "rays_per_step": int(n),
-                   "builder": {"sah": "host binned SAH", "lbvh": "device LBVH"}[args.builder],
+                   "builder": {"sah": "host binned SAH", "lbvh": "device LBVH", "ploc": "device PLOC"}[args.builder],
                    "build_s": round(info["buildSeconds"], 3), "node_layout": "bvh4 64B quantized"
                    if args.layout == 0 else "bvh4 128B fp32", "max_leaf": args.max_leaf,
                    "parallelism": "single GPU"},

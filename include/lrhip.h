@@ -97,16 +97,21 @@ typedef struct {
  * collapsed to the same 4-wide records, the triangle records bit for bit the
  * host's; single-level scenes with a device; no early split clipping, so
  * splitBudget is ignored; the replacement of the reference's EMBREE_MORTON
- * builder).  A violated internal precondition of the device build (codes out
- * of order, a malformed radix tree) returns LRH_EHIP with the reason. */
+ * builder).  LRH_BUILD_DEVICE_PLOC builds on the GPU too, with the radix
+ * tree replaced by agglomerative clustering of the Morton-ordered boxes
+ * (rounds of mutual nearest neighbours within 16 places): a tree of lower
+ * surface-area cost for a somewhat longer build, same records and limits.
+ * A violated internal precondition of a device build (codes out of order, a
+ * malformed tree) returns LRH_EHIP with the reason. */
 #define LRH_BUILD_HOST_SAH 0u
 #define LRH_BUILD_DEVICE_LBVH 1u
+#define LRH_BUILD_DEVICE_PLOC 2u
 typedef struct {
 	unsigned int layout;
 	unsigned int maxLeafSize;  /* 1..8, default 4 */
 	unsigned int nThreads;     /* host build threads, default = hardware threads */
 	float travCost;            /* SAH traversal cost relative to one triangle test, default 1.0 */
-	unsigned int builder;      /* LRH_BUILD_HOST_SAH or LRH_BUILD_DEVICE_LBVH */
+	unsigned int builder;      /* LRH_BUILD_HOST_SAH, LRH_BUILD_DEVICE_LBVH or LRH_BUILD_DEVICE_PLOC */
 	float splitBudget;         /* early split clipping: triangles whose box area is above twice the
 	                              median and above 2.5x the triangle's two-sided area (a poor fit) are
 	                              cut into up to 64 references with tighter boxes, at most

@@ -782,9 +782,10 @@ namespace {
 void *gbAlloc(void *ctx, size_t bytes) { return static_cast<lrh_device *>(ctx)->devAlloc(bytes); }
 void gbRelease(void *ctx, void *p, size_t bytes) { static_cast<lrh_device *>(ctx)->devFree(p, bytes); }
 
-// single-level scene built on the device (LRH_BUILD_DEVICE_LBVH): vertices and
-// triangles are uploaded mesh by mesh, the tree and the records never leave HBM
-void deviceBuild(lrh_scene *s, const lrh_mesh *meshes, unsigned nMeshes) {
+// single-level scene built on the device (LRH_BUILD_DEVICE_LBVH / _PLOC):
+// vertices and triangles are uploaded mesh by mesh, the tree and the records
+// never leave HBM
+void deviceBuild(lrh_scene *s, const lrh_mesh *meshes, unsigned nMeshes, unsigned builder) {
 	lrh_device *dev = s->dev;
 	dev->activate();
 	const double t0 = now();
@@ -832,6 +833,7 @@ void deviceBuild(lrh_scene *s, const lrh_mesh *meshes, unsigned nMeshes) {
 	in.nTris = n;
 	in.maxLeaf = s->params.maxLeaf;
 	in.layout = s->layout;
+	in.method = builder == LRH_BUILD_DEVICE_PLOC ? 1u : 0u;
 	LrhGpuAlloc a{gbAlloc, gbRelease, dev};
 	LrhGpuBuildOut out;
 	try {
@@ -861,10 +863,12 @@ int lrh_scene_create_bvh(lrh_device *dev, const lrh_mesh *meshes, unsigned int n
 		s->layout = s->params.layout;
 		s->twoLevel = false;
 		const unsigned builder = opts ? opts->builder : LRH_BUILD_HOST_SAH;
-		if (builder > LRH_BUILD_DEVICE_LBVH) throw Error(LRH_EINVAL, "builder must be LRH_BUILD_HOST_SAH or LRH_BUILD_DEVICE_LBVH");
-		if (builder == LRH_BUILD_DEVICE_LBVH) {
+		if (builder > LRH_BUILD_DEVICE_PLOC)
+			throw Error(LRH_EINVAL,
+					"builder must be LRH_BUILD_HOST_SAH, LRH_BUILD_DEVICE_LBVH or LRH_BUILD_DEVICE_PLOC");
+		if (builder != LRH_BUILD_HOST_SAH) {
 			if (!dev) throw Error(LRH_EINVAL, "the device builder needs a device");
-			deviceBuild(s.get(), meshes, nMeshes);
+			deviceBuild(s.get(), meshes, nMeshes, builder);
 			return;
 		}
 		const double t0 = now();

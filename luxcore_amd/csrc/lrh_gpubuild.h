@@ -1,7 +1,8 @@
 /*
- * Device-side BVH build (linear BVH): Morton order of the triangle boxes,
- * binary radix tree, bottom-up boxes, then a breadth-first collapse into the
- * same 4-wide node records the host builder writes.  It stands in for the
+ * Device-side BVH build: Morton order of the triangle boxes, a binary tree
+ * over them (the radix tree of a linear BVH, or agglomerative clustering of
+ * neighbours in that order), then a breadth-first collapse into the same
+ * 4-wide node records the host builder writes.  It stands in for the
  * host build of BVHAccel::Init (src/luxrays/accelerators/bvhaccel.cpp:72-168)
  * when the scene is built for a device.
  */
@@ -27,6 +28,7 @@ struct LrhGpuBuildIn {
 	uint32_t nTris;
 	uint32_t maxLeaf;            // 1..8
 	uint32_t layout;             // 0 quantized 64 B nodes, 1 fp32 128 B nodes
+	uint32_t method = 0;         // binary tree over the sorted codes: 0 radix tree (LBVH), 1 clustering (PLOC)
 };
 
 struct LrhGpuBuildOut {
@@ -39,6 +41,7 @@ struct LrhGpuBuildOut {
 	size_t invPermBytes = 0;
 	uint32_t root = 0xffffffffu;
 	uint32_t maxDepth = 0;
+	uint32_t plocRounds = 0;     // clustering rounds (method 1)
 	double kernelMs = 0.0;
 };
 

@@ -262,6 +262,98 @@ __global__ void k_boxesUp(const GBox *boxes, const uint32_t *vals, uint32_t n, c
 	}
 }
 
+// 3'. clustering (PLOC): clusters start as the sorted leaves
+__global__ void k_plocInit(const GBox *boxes, const uint32_t *vals, uint32_t n, uint32_t *cid, GBox *cbox,
+		GBox *leafBox) {
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const GBox b = boxes[vals[i]];
+	leafBox[i] = b;
+	cid[i] = kLeafBit | i;
+	cbox[i] = b;
+}
+
+__global__ void k_plocNearest(const GBox *cbox, uint32_t nc, uint32_t *nn) {
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < nc) nn[i] = plocNearest(cbox, nc, i, kPlocRadius);
+}
+
+// lead: the lower place of a merging pair (a new node); keep: the place
+// survives the round
+__global__ void k_plocFlags(const uint32_t *nn, uint32_t nc, uint32_t forced, uint32_t *lead, uint32_t *keep) {
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i > nc) return;
+	if (i == nc) {  // the scan's total slot
+		lead[i] = 0;
+		keep[i] = 0;
+		return;
+	}
+	const uint32_t j = plocPartner(nn, nc, i, forced != 0);
+	lead[i] = (j != kNone && i < j) ? 1u : 0u;
+	keep[i] = (j != kNone && i > j) ? 0u : 1u;
+}
+
+__global__ void k_plocApply(const uint32_t *nn, uint32_t nc, uint32_t forced, const uint32_t *cid, const GBox *cbox,
+		const uint32_t *leadOff, const uint32_t *keepOff, uint32_t nodeBase, uint32_t *outCid, GBox *outBox,
+		uint32_t *left, uint32_t *right, GBox *nodeBox) {
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= nc) return;
+	const uint32_t j = plocPartner(nn, nc, i, forced != 0);
+	if (j != kNone && i > j) return;  // merged into its lower partner
+	const uint32_t o = keepOff[i];
+	if (j != kNone) {
+		const uint32_t node = nodeBase + leadOff[i];
+		const GBox u = unionBox(cbox[i], cbox[j]);
+		left[node] = cid[i];
+		right[node] = cid[j];
+		nodeBox[node] = u;
+		outCid[o] = node;
+		outBox[o] = u;
+	} else {
+		outCid[o] = cid[i];
+		outBox[o] = cbox[i];
+	}
+}
+
+// leaves per subtree, the nodes of one round (their children are older)
+__global__ void k_plocCount(uint32_t b, uint32_t e, const uint32_t *left, const uint32_t *right, uint32_t *count) {
+	const uint32_t x = b + blockIdx.x * blockDim.x + threadIdx.x;
+	if (x < e) count[x] = plocSize(left[x], count) + plocSize(right[x], count);
+}
+
+// first leaf place of each subtree, top down (the nodes of one round; their
+// parents are younger); leafFirst: the place of each sorted leaf
+__global__ void k_plocFirst(uint32_t b, uint32_t e, const uint32_t *left, const uint32_t *right, const uint32_t *count,
+		uint32_t *first, uint32_t *leafFirst) {
+	const uint32_t x = b + blockIdx.x * blockDim.x + threadIdx.x;
+	if (x >= e) return;
+	const uint32_t f = first[x], l = left[x], r = right[x];
+	const uint32_t fr = f + plocSize(l, count);
+	if (l & kLeafBit) leafFirst[l & ~kLeafBit] = f; else first[l] = f;
+	if (r & kLeafBit) leafFirst[r & ~kLeafBit] = fr; else first[r] = fr;
+}
+
+// the clustering's tree in the radix tree's form: internal node x becomes
+// n-2-x (root 0), leaves their places; leaf values and boxes in place order
+__global__ void k_plocFinish(uint32_t n, const uint32_t *left, const uint32_t *right, const uint32_t *count,
+		const uint32_t *first, const GBox *nodeBox, const uint32_t *leafFirst, const uint32_t *vals,
+		const GBox *leafBoxIn, uint32_t *left2, uint32_t *right2, uint32_t *first2, uint32_t *last2, GBox *nodeBox2,
+		uint32_t *vals2, GBox *leafBox2) {
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const uint32_t p = leafFirst[i];
+	vals2[p] = vals[i];
+	leafBox2[p] = leafBoxIn[i];
+	if (i + 1 >= n) return;  // n - 1 internal nodes
+	const uint32_t y = n - 2 - i;
+	auto remap = [&](uint32_t c) { return (c & kLeafBit) ? (kLeafBit | leafFirst[c & ~kLeafBit]) : n - 2 - c; };
+	left2[y] = remap(left[i]);
+	right2[y] = remap(right[i]);
+	first2[y] = first[i];
+	last2[y] = first[i] + count[i] - 1;
+	nodeBox2[y] = nodeBox[i];
+}
+
 // 5a. children of each frontier node and how many become inner nodes
 __global__ void k_expand(const uint32_t *frontier, uint32_t nf, uint32_t nInner, const uint32_t *left,
 		const uint32_t *right, const uint32_t *first, const uint32_t *last, const GBox *nodeBox, uint32_t maxLeaf,
@@ -378,6 +470,89 @@ std::string errorText(uint32_t e) {
 	return s;
 }
 
+// the clustering's binary tree over the n >= 2 sorted leaves, written in the
+// radix tree's form (left / right / first / last / nodeBox, root 0, leaf
+// places contiguous per subtree); vals and leafBox are repointed to the
+// leaves in place order.  Each round: nearest neighbours, the merge and keep
+// flags scanned, then the pairs merged and the clusters compacted.  The host
+// reads the round's merge count; a round that merges fewer than 1/64 of the
+// clusters is followed by a forced round that pairs places 2k, 2k+1, so the
+// number of rounds is bounded by the size alone.  (Natural rounds merge a
+// third of the clusters at first and a tenth or less near the top, where
+// boxes overlap; forcing there would cost tree quality.)
+void plocTree(Scratch &S, const GBox *boxes, uint32_t n, hipStream_t st, uint32_t *&vals, GBox *&leafBox,
+		uint32_t *left, uint32_t *right, uint32_t *first, uint32_t *last, GBox *nodeBox, uint32_t *roundsOut) {
+	const uint32_t ni = n - 1;
+	uint32_t *cidA = S.get<uint32_t>(n), *cidB = S.get<uint32_t>(n), *nn = S.get<uint32_t>(n);
+	GBox *cboxA = S.get<GBox>(n), *cboxB = S.get<GBox>(n);
+	uint32_t *lead = S.get<uint32_t>((size_t)n + 1), *keep = S.get<uint32_t>((size_t)n + 1),
+			 *leadOff = S.get<uint32_t>((size_t)n + 1), *keepOff = S.get<uint32_t>((size_t)n + 1);
+	uint32_t back[2] = {0, 0};
+	hipLaunchKernelGGL(k_plocInit, dim3(grid(n, 256)), dim3(256), 0, st, boxes, vals, n, cidA, cboxA, leafBox);
+	GB_CHECK(hipGetLastError());
+	std::vector<uint32_t> roundStart{0};
+	uint32_t nc = n, nodeBase = 0;
+	bool forced = false;
+	// fewer than 1/64 merged forces the next round, which halves: at most
+	// 2 * 31 slow rounds, 63 single merges below 64 clusters, and
+	// log(2^31) / log(64/63) others
+	const uint32_t kMaxRounds = 4096;
+	while (nc > 1) {
+		if (roundStart.size() > kMaxRounds) throw std::runtime_error("device build: clustering did not converge");
+		if (!forced)
+			hipLaunchKernelGGL(k_plocNearest, dim3(grid(nc, 256)), dim3(256), 0, st, cboxA, nc, nn);
+		hipLaunchKernelGGL(k_plocFlags, dim3(grid((size_t)nc + 1, 256)), dim3(256), 0, st, nn, nc, forced ? 1u : 0u,
+				lead, keep);
+		GB_CHECK(hipGetLastError());
+		exclusiveScan(S, lead, leadOff, nc + 1, st);
+		exclusiveScan(S, keep, keepOff, nc + 1, st);
+		hipLaunchKernelGGL(k_plocApply, dim3(grid(nc, 256)), dim3(256), 0, st, nn, nc, forced ? 1u : 0u, cidA, cboxA,
+				leadOff, keepOff, nodeBase, cidB, cboxB, left, right, nodeBox);
+		GB_CHECK(hipGetLastError());
+		GB_CHECK(hipMemcpyAsync(back, leadOff + nc, 4, hipMemcpyDeviceToHost, st));
+		GB_CHECK(hipMemcpyAsync(back + 1, keepOff + nc, 4, hipMemcpyDeviceToHost, st));
+		GB_CHECK(hipStreamSynchronize(st));
+		const uint32_t merged = back[0], kept = back[1];
+		if (merged == 0 || merged > nc / 2 || kept != nc - merged || nodeBase + merged > ni)
+			throw std::runtime_error("device build: clustering round without a consistent merge count");
+		nodeBase += merged;
+		roundStart.push_back(nodeBase);
+		forced = !forced && (uint64_t)merged * 64u < nc;
+		nc = kept;
+		std::swap(cidA, cidB);
+		std::swap(cboxA, cboxB);
+	}
+	if (nodeBase != ni) throw std::runtime_error("device build: clustering left nodes unmade");
+	*roundsOut = (uint32_t)roundStart.size() - 1;
+	// leaves per subtree, oldest round first; first places, newest first
+	uint32_t *count = S.get<uint32_t>(ni), *firstP = S.get<uint32_t>(ni), *leafFirst = S.get<uint32_t>(n);
+	for (size_t r = 0; r + 1 < roundStart.size(); ++r) {
+		const uint32_t b = roundStart[r], e = roundStart[r + 1];
+		hipLaunchKernelGGL(k_plocCount, dim3(grid(e - b, 256)), dim3(256), 0, st, b, e, left, right, count);
+	}
+	hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, firstP + (ni - 1), 1u, 0u);  // the root, node n-2
+	for (size_t r = roundStart.size() - 1; r-- > 0;) {
+		const uint32_t b = roundStart[r], e = roundStart[r + 1];
+		hipLaunchKernelGGL(k_plocFirst, dim3(grid(e - b, 256)), dim3(256), 0, st, b, e, left, right, count, firstP,
+				leafFirst);
+	}
+	GB_CHECK(hipGetLastError());
+	// the radix tree's form, written into the caller's arrays; the binary
+	// tree's own are copied aside first
+	uint32_t *leftP = S.get<uint32_t>(ni), *rightP = S.get<uint32_t>(ni);
+	GBox *boxP = S.get<GBox>(ni);
+	GB_CHECK(hipMemcpyAsync(leftP, left, (size_t)ni * 4, hipMemcpyDeviceToDevice, st));
+	GB_CHECK(hipMemcpyAsync(rightP, right, (size_t)ni * 4, hipMemcpyDeviceToDevice, st));
+	GB_CHECK(hipMemcpyAsync(boxP, nodeBox, (size_t)ni * sizeof(GBox), hipMemcpyDeviceToDevice, st));
+	uint32_t *vals2 = S.get<uint32_t>(n);
+	GBox *leafBox2 = S.get<GBox>(n);
+	hipLaunchKernelGGL(k_plocFinish, dim3(grid(n, 256)), dim3(256), 0, st, n, leftP, rightP, count, firstP, boxP,
+			leafFirst, vals, leafBox, left, right, first, last, nodeBox, vals2, leafBox2);
+	GB_CHECK(hipGetLastError());
+	vals = vals2;
+	leafBox = leafBox2;
+}
+
 }  // namespace
 
 void lrhGpuBuildBVH(const LrhGpuBuildIn &in, const LrhGpuAlloc &a, hipStream_t st, LrhGpuBuildOut *out) {
@@ -443,20 +618,24 @@ void lrhGpuBuildBVH(const LrhGpuBuildIn &in, const LrhGpuAlloc &a, hipStream_t s
 
 			const uint32_t ni = n - 1;
 			uint32_t *left = S.get<uint32_t>(ni), *right = S.get<uint32_t>(ni), *first = S.get<uint32_t>(ni),
-					 *last = S.get<uint32_t>(ni), *iparent = S.get<uint32_t>(ni), *lparent = S.get<uint32_t>(n),
-					 *flags = S.get<uint32_t>(ni);
+					 *last = S.get<uint32_t>(ni);
 			GBox *nodeBox = S.get<GBox>(ni);
-			hipLaunchKernelGGL(k_fill, dim3(grid(ni, 256)), dim3(256), 0, st, flags, ni, 0u);
-			hipLaunchKernelGGL(k_fill, dim3(grid(ni, 256)), dim3(256), 0, st, iparent, ni, kNone);  // root: none
-			hipLaunchKernelGGL(k_fill, dim3(grid(n, 256)), dim3(256), 0, st, lparent, n, kNone);
-			hipLaunchKernelGGL(k_radixTree, dim3(grid(ni, 256)), dim3(256), 0, st, keys, n, left, right, first, last,
-					iparent, lparent, err);
-			GB_CHECK(hipGetLastError());
-			if (const uint32_t e = readError(err, st)) throw std::runtime_error(errorText(e));
-			hipLaunchKernelGGL(k_boxesUp, dim3(grid(n, 256)), dim3(256), 0, st, boxes, vals, n, left, right, iparent,
-					lparent, flags, leafBox, nodeBox, err);
-			GB_CHECK(hipGetLastError());
-			if (const uint32_t e = readError(err, st)) throw std::runtime_error(errorText(e));
+			if (in.method == 1) {
+				plocTree(S, boxes, n, st, vals, leafBox, left, right, first, last, nodeBox, &out->plocRounds);
+			} else {
+				uint32_t *iparent = S.get<uint32_t>(ni), *lparent = S.get<uint32_t>(n), *flags = S.get<uint32_t>(ni);
+				hipLaunchKernelGGL(k_fill, dim3(grid(ni, 256)), dim3(256), 0, st, flags, ni, 0u);
+				hipLaunchKernelGGL(k_fill, dim3(grid(ni, 256)), dim3(256), 0, st, iparent, ni, kNone);  // root: none
+				hipLaunchKernelGGL(k_fill, dim3(grid(n, 256)), dim3(256), 0, st, lparent, n, kNone);
+				hipLaunchKernelGGL(k_radixTree, dim3(grid(ni, 256)), dim3(256), 0, st, keys, n, left, right, first,
+						last, iparent, lparent, err);
+				GB_CHECK(hipGetLastError());
+				if (const uint32_t e = readError(err, st)) throw std::runtime_error(errorText(e));
+				hipLaunchKernelGGL(k_boxesUp, dim3(grid(n, 256)), dim3(256), 0, st, boxes, vals, n, left, right,
+						iparent, lparent, flags, leafBox, nodeBox, err);
+				GB_CHECK(hipGetLastError());
+				if (const uint32_t e = readError(err, st)) throw std::runtime_error(errorText(e));
+			}
 
 			// breadth-first collapse: level frontiers of binary node ids
 			uint32_t *fA = S.get<uint32_t>(ni), *fB = S.get<uint32_t>(ni);

@@ -272,6 +272,70 @@ LRH_DN inline lrh_tri triRecord(const float *verts, const uint32_t *tris, const 
 
 LRH_DN inline uint32_t digitOf(uint64_t key, uint32_t shift) { return (uint32_t)(key >> shift) & 0xffu; }
 
+// ---- agglomerative clustering of the Morton-ordered boxes (PLOC: Meister &
+// Bittner, "Parallel Locally-Ordered Clustering for Bounding Volume
+// Hierarchy Construction", IEEE TVCG 2018).  Each round every cluster finds
+// its nearest neighbour within kPlocRadius places (the smallest merged box
+// area); mutual pairs merge into a new binary node at the lower place and the
+// array is compacted in order.  The binary tree then gets contiguous leaf
+// ranges (subtree sizes bottom-up, first places top-down) and is collapsed
+// like the radix tree. ----
+constexpr uint32_t kPlocRadius = 16;
+
+LRH_DN inline GBox unionBox(const GBox &a, const GBox &b) {
+	GBox u;
+	for (int k = 0; k < 3; ++k) {
+		u.lo[k] = minT(a.lo[k], b.lo[k]);
+		u.hi[k] = maxT(a.hi[k], b.hi[k]);
+	}
+	return u;
+}
+
+// area of the merged box, a NaN (boxes of NaN vertices) as +inf
+LRH_DN inline float mergedArea(const GBox &a, const GBox &b) {
+	const float x = areaOf(unionBox(a, b));
+	return x == x ? x : INFINITY;
+}
+
+// nearest neighbour of cluster i among places [i - r, i + r]: the smallest
+// merged area, equal areas by the pair's (lower, upper) place, so the pair
+// with the smallest such key of all is mutual and every round merges
+LRH_DN inline uint32_t plocNearest(const GBox *c, uint32_t nc, uint32_t i, uint32_t r) {
+	const uint32_t lo = i > r ? i - r : 0u;
+	const uint32_t hi = (nc - 1u - i > r) ? i + r : nc - 1u;
+	float best = INFINITY;
+	uint32_t arg = i;
+	for (uint32_t j = lo; j <= hi; ++j) {
+		if (j == i) continue;
+		const float a = mergedArea(c[i], c[j]);
+		bool better = arg == i || a < best;
+		if (arg != i && a == best) {
+			const uint32_t p0 = i < j ? i : j, q0 = i < j ? j : i;
+			const uint32_t p1 = i < arg ? i : arg, q1 = i < arg ? arg : i;
+			better = p0 < p1 || (p0 == p1 && q0 < q1);
+		}
+		if (better) {
+			best = a;
+			arg = j;
+		}
+	}
+	return arg;
+}
+
+// cluster i's partner this round (kNone: none): its nearest neighbour when
+// that is mutual; in a forced round (kept for inputs on which the
+// neighbour rounds stall) the places 2k and 2k+1
+LRH_DN inline uint32_t plocPartner(const uint32_t *nn, uint32_t nc, uint32_t i, bool forced) {
+	if (forced) {
+		const uint32_t j = i ^ 1u;
+		return j < nc ? j : kNone;
+	}
+	const uint32_t j = nn[i];
+	return (j != i && nn[j] == i) ? j : kNone;
+}
+
+LRH_DN inline uint32_t plocSize(uint32_t c, const uint32_t *count) { return (c & kLeafBit) ? 1u : count[c]; }
+
 }  // namespace lrhlbvh
 
 #endif

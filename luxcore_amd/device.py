@@ -47,7 +47,7 @@ class LrhBuildOpts(ctypes.Structure):
                 ("travCost", ctypes.c_float), ("builder", ctypes.c_uint), ("splitBudget", ctypes.c_float)]
 
 
-BUILD_HOST_SAH, BUILD_DEVICE_LBVH = 0, 1
+BUILD_HOST_SAH, BUILD_DEVICE_LBVH, BUILD_DEVICE_PLOC = 0, 1, 2
 
 
 class LrhSceneInfo(ctypes.Structure):
@@ -534,7 +534,7 @@ class Device:
     # -- scenes --
     def scene_bvh(self, meshes, layout=0, max_leaf=4, threads=0, trav_cost=1.0, builder=BUILD_HOST_SAH,
                   split_budget=0.0):
-        """lrh_scene_create_bvh; builder=BUILD_DEVICE_LBVH builds the tree on the GPU; split_budget:
+        """lrh_scene_create_bvh; builder=BUILD_DEVICE_LBVH or BUILD_DEVICE_PLOC builds the tree on the GPU; split_budget:
         extra references of early split clipping per triangle (0 = default, negative = off)."""
         arr, keep = _mesh_structs(meshes)
         opts = _build_opts(layout, max_leaf, threads, trav_cost, builder, split_budget)

@@ -6,14 +6,17 @@
  * [digit][tile] table, a scatter in index order), the same error word.  The
  * device build must produce these records byte for byte.
  *
- *   lbvh_emu <in.bin> <outdir> <maxLeaf> <layout> [mode]
+ *   lbvh_emu <in.bin> <outdir> <maxLeaf> <layout> [mode] [method]
  * in.bin: u32 nMeshes, then per mesh u32 nVerts, u32 nTris, float[3 nVerts],
  * u32[3 nTris].  mode: "normal" (default); "unsorted": the sort is skipped
  * (codes left in triangle order), as a sort that returned garbage would
- * leave them; "reversed": the sorted codes reversed.  Writes nodes.bin,
- * tris.bin and info.txt ("err <word> nodes <count> depth <d>"); with
- * codes out of order the run stops after the step that reports them, as the
- * device build does, and still terminates: every search is bounded.
+ * leave them; "reversed": the sorted codes reversed.  method: "lbvh"
+ * (default, the radix tree) or "ploc" (clustering rounds, each round's
+ * steps over all clusters before the next step, as the kernels run them).
+ * Writes nodes.bin, tris.bin and info.txt ("err <word> nodes <count> depth
+ * <d> rounds <r>"); with codes out of order the run stops after the step
+ * that reports them, as the device build does, and still terminates: every
+ * search is bounded.
  */
 #include <algorithm>
 #include <cstdint>
@@ -45,17 +48,104 @@ void exclusiveScan(std::vector<uint32_t> &v) {
 	}
 }
 
+// the clustering's tree in the radix tree's form, as the device build's
+// rounds and conversion make it; returns the rounds, 0 when a round's counts
+// are inconsistent
+uint32_t plocTree(const std::vector<GBox> &boxes, std::vector<uint32_t> &vals, std::vector<GBox> &leafBox,
+		std::vector<uint32_t> &left2, std::vector<uint32_t> &right2, std::vector<uint32_t> &first2,
+		std::vector<uint32_t> &last2, std::vector<GBox> &nodeBox2) {
+	const uint32_t n = (uint32_t)vals.size(), ni = n - 1;
+	std::vector<uint32_t> cid(n), cidB(n), nn(n), lead(n + 1), keep(n + 1);
+	std::vector<GBox> cbox(n), cboxB(n), nodeBox(ni);
+	std::vector<uint32_t> left(ni), right(ni);
+	for (uint32_t i = 0; i < n; ++i) {
+		leafBox[i] = boxes[vals[i]];
+		cid[i] = kLeafBit | i;
+		cbox[i] = leafBox[i];
+	}
+	std::vector<uint32_t> roundStart{0};
+	uint32_t nc = n, nodeBase = 0;
+	bool forced = false;
+	while (nc > 1) {
+		if (roundStart.size() > 4096) return 0;
+		if (!forced)
+			for (uint32_t i = 0; i < nc; ++i) nn[i] = plocNearest(cbox.data(), nc, i, kPlocRadius);
+		for (uint32_t i = 0; i < nc; ++i) {
+			const uint32_t j = plocPartner(nn.data(), nc, i, forced);
+			lead[i] = (j != kNone && i < j) ? 1u : 0u;
+			keep[i] = (j != kNone && i > j) ? 0u : 1u;
+		}
+		lead[nc] = keep[nc] = 0;
+		std::vector<uint32_t> leadOff(lead.begin(), lead.begin() + nc + 1), keepOff(keep.begin(), keep.begin() + nc + 1);
+		exclusiveScan(leadOff);
+		exclusiveScan(keepOff);
+		for (uint32_t i = 0; i < nc; ++i) {
+			const uint32_t j = plocPartner(nn.data(), nc, i, forced);
+			if (j != kNone && i > j) continue;
+			const uint32_t o = keepOff[i];
+			if (j != kNone) {
+				const uint32_t node = nodeBase + leadOff[i];
+				const GBox u = unionBox(cbox[i], cbox[j]);
+				left[node] = cid[i];
+				right[node] = cid[j];
+				nodeBox[node] = u;
+				cidB[o] = node;
+				cboxB[o] = u;
+			} else {
+				cidB[o] = cid[i];
+				cboxB[o] = cbox[i];
+			}
+		}
+		const uint32_t merged = leadOff[nc], kept = keepOff[nc];
+		if (merged == 0 || merged > nc / 2 || kept != nc - merged || nodeBase + merged > ni) return 0;
+		nodeBase += merged;
+		roundStart.push_back(nodeBase);
+		forced = !forced && (uint64_t)merged * 64u < nc;
+		nc = kept;
+		cid.swap(cidB);
+		cbox.swap(cboxB);
+	}
+	if (nodeBase != ni) return 0;
+	std::vector<uint32_t> count(ni), first(ni), leafFirst(n);
+	for (uint32_t x = 0; x < ni; ++x) count[x] = plocSize(left[x], count.data()) + plocSize(right[x], count.data());
+	first[ni - 1] = 0;
+	for (uint32_t x = ni; x-- > 0;) {
+		const uint32_t f = first[x], l = left[x], r = right[x], fr = f + plocSize(l, count.data());
+		if (l & kLeafBit) leafFirst[l & ~kLeafBit] = f; else first[l] = f;
+		if (r & kLeafBit) leafFirst[r & ~kLeafBit] = fr; else first[r] = fr;
+	}
+	std::vector<uint32_t> vals2(n);
+	std::vector<GBox> leafBox2(n);
+	for (uint32_t i = 0; i < n; ++i) {
+		vals2[leafFirst[i]] = vals[i];
+		leafBox2[leafFirst[i]] = leafBox[i];
+	}
+	auto remap = [&](uint32_t c) { return (c & kLeafBit) ? (kLeafBit | leafFirst[c & ~kLeafBit]) : n - 2 - c; };
+	for (uint32_t i = 0; i < ni; ++i) {
+		const uint32_t y = n - 2 - i;
+		left2[y] = remap(left[i]);
+		right2[y] = remap(right[i]);
+		first2[y] = first[i];
+		last2[y] = first[i] + count[i] - 1;
+		nodeBox2[y] = nodeBox[i];
+	}
+	vals.swap(vals2);
+	leafBox.swap(leafBox2);
+	return (uint32_t)roundStart.size() - 1;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
 	if (argc < 5) {
-		std::cerr << "usage: lbvh_emu in.bin outdir maxLeaf layout [normal|unsorted|reversed]\n";
+		std::cerr << "usage: lbvh_emu in.bin outdir maxLeaf layout [normal|unsorted|reversed] [lbvh|ploc]\n";
 		return 2;
 	}
 	const std::string outDir = argv[2];
 	const uint32_t maxLeaf = (uint32_t)std::stoul(argv[3]);
 	const uint32_t layout = (uint32_t)std::stoul(argv[4]);
 	const std::string mode = argc > 5 ? argv[5] : "normal";
+	const bool ploc = argc > 6 && std::string(argv[6]) == "ploc";
 	std::ifstream in(argv[1], std::ios::binary);
 	auto rd = [&](void *p, size_t b) {
 		in.read(reinterpret_cast<char *>(p), (std::streamsize)b);
@@ -78,7 +168,7 @@ int main(int argc, char **argv) {
 		vbase.push_back(vbase.back() + nv);
 	}
 	const uint32_t n = tbase.back();
-	uint32_t err = 0, depth = 0;
+	uint32_t err = 0, depth = 0, rounds = 0;
 	const size_t nodeSz = layout == 0 ? sizeof(lrh_node4q) : sizeof(lrh_node4f);
 	std::vector<uint8_t> nodes;
 	std::vector<lrh_tri> recs;
@@ -153,7 +243,12 @@ int main(int argc, char **argv) {
 		const uint32_t ni = n - 1;
 		std::vector<uint32_t> left(ni), right(ni), first(ni), last(ni), iparent(ni, kNone), lparent(n, kNone),
 				flags(ni, 0);
-		for (uint32_t i = 0; i < ni; ++i) {
+		std::vector<GBox> nodeBox(ni);
+		if (ploc && !err) {
+			rounds = plocTree(boxes, vals, leafBox, left, right, first, last, nodeBox);
+			if (rounds == 0) err |= kErrExpand;
+		}
+		for (uint32_t i = 0; i < ni && !ploc; ++i) {
 			uint32_t lc, rc, lo, hi;
 			if (!radixNode(keys.data(), n, i, lc, rc, lo, hi)) {
 				err |= kErrRange;
@@ -168,9 +263,9 @@ int main(int argc, char **argv) {
 		}
 		if (!err) {
 			// 4. boxes bottom-up, the second arrival at a node computes it
-			std::vector<GBox> nodeBox(ni);
-			for (uint32_t j = 0; j < n; ++j) leafBox[j] = boxes[vals[j]];
-			for (uint32_t j = 0; j < n; ++j) {
+			if (!ploc)
+				for (uint32_t j = 0; j < n; ++j) leafBox[j] = boxes[vals[j]];
+			for (uint32_t j = 0; j < n && !ploc; ++j) {
 				uint32_t p = lparent[j];
 				for (int steps = 0; p < n - 1; ++steps) {
 					if (steps >= 128) {
@@ -231,7 +326,9 @@ int main(int argc, char **argv) {
 	put(outDir + "/nodes.bin", nodes);
 	put(outDir + "/tris.bin", recs);
 	std::ofstream info(outDir + "/info.txt");
-	info << "err " << err << " nodes " << nodeCount << " depth " << (depth ? depth - 1 : 0) << "\n";
-	std::cout << "err " << err << " nodes " << nodeCount << " depth " << (depth ? depth - 1 : 0) << std::endl;
+	info << "err " << err << " nodes " << nodeCount << " depth " << (depth ? depth - 1 : 0) << " rounds " << rounds
+		 << "\n";
+	std::cout << "err " << err << " nodes " << nodeCount << " depth " << (depth ? depth - 1 : 0) << " rounds "
+			  << rounds << std::endl;
 	return 0;
 }

@@ -1,7 +1,12 @@
-"""Device-built trees (LRH_BUILD_DEVICE_LBVH): the same 4-wide records as the
-host builder, a well-formed tree over every triangle, deterministic, and
-traces equal to the oracle to the parity bar (closest hit does not depend on
-the tree outside exact ties and the edge band)."""
+"""Device-built trees (LRH_BUILD_DEVICE_LBVH, LRH_BUILD_DEVICE_PLOC): the
+same 4-wide records as the host builder, a well-formed tree over every
+triangle, deterministic, byte for byte the sequential CPU emulation of the
+same kernels (tests/native/lbvh_emu.cpp), and traces equal to the oracle to
+the parity bar (closest hit does not depend on the tree outside exact ties
+and the edge band)."""
+import os
+import subprocess
+
 import numpy as np
 import pytest
 
@@ -13,6 +18,9 @@ from tests.parity import assert_parity, compare
 from tests.test_abi_cpu import NODE_F, NODE_Q, TRI, check_tree
 
 pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BUILDERS = {"lbvh": D.BUILD_DEVICE_LBVH, "ploc": D.BUILD_DEVICE_PLOC}
 
 
 def _case(name, **kw):
@@ -38,11 +46,12 @@ def _records_sorted(tris):
     return tris[order]
 
 
+@pytest.mark.parametrize("builder", list(BUILDERS))
 @pytest.mark.parametrize("layout,max_leaf", [(0, 4), (0, 1), (0, 8), (1, 4)])
-def test_device_build_tree_and_records(gpu_device, layout, max_leaf):
-    _case("devbuild_tree", layout=layout, max_leaf=max_leaf)
+def test_device_build_tree_and_records(gpu_device, layout, max_leaf, builder):
+    _case("devbuild_tree", layout=layout, max_leaf=max_leaf, builder=builder)
     meshes = scenes.plain_scene(seed=1, n_random=20000)
-    dsc = gpu_device.scene_bvh(meshes, layout=layout, max_leaf=max_leaf, builder=D.BUILD_DEVICE_LBVH)
+    dsc = gpu_device.scene_bvh(meshes, layout=layout, max_leaf=max_leaf, builder=BUILDERS[builder])
     info = dsc.info()
     nb, pb = dsc.download()
     nodes = nb.view(NODE_Q if layout == 0 else NODE_F)
@@ -60,20 +69,47 @@ def test_device_build_tree_and_records(gpu_device, layout, max_leaf):
         return lo[s:s + c].min(0), hi[s:s + c].max(0)
     check_tree(nodes, boxes_of_child, 0, nodes.shape[0], ntri, layout)
     # deterministic
-    dsc2 = gpu_device.scene_bvh(meshes, layout=layout, max_leaf=max_leaf, builder=D.BUILD_DEVICE_LBVH)
+    dsc2 = gpu_device.scene_bvh(meshes, layout=layout, max_leaf=max_leaf, builder=BUILDERS[builder])
     nb2, pb2 = dsc2.download()
     assert nb.tobytes() == nb2.tobytes() and pb.tobytes() == pb2.tobytes()
     dsc.close()
     dsc2.close()
 
 
+@pytest.fixture(scope="module")
+def emu(tmp_path_factory):
+    exe = tmp_path_factory.mktemp("emu") / "lbvh_emu"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-o", str(exe),
+                    os.path.join(ROOT, "tests", "native", "lbvh_emu.cpp")], check=True, timeout=180)
+    return exe
+
+
+@pytest.mark.parametrize("builder", list(BUILDERS))
+@pytest.mark.parametrize("layout,max_leaf", [(0, 4), (1, 2)])
+def test_device_build_equals_cpu_emulation(gpu_device, emu, tmp_path, layout, max_leaf, builder):
+    """The kernels' tree is the sequential emulation's, node and record bytes
+    alike: the per-element steps are shared and every step is deterministic."""
+    from tests.test_lbvh_cpu import run_emu
+    _case("devbuild_emu", layout=layout, max_leaf=max_leaf, builder=builder)
+    meshes = scenes.plain_scene(seed=1, n_random=20000)
+    info, nodes, tris = run_emu(emu, tmp_path, meshes, max_leaf, layout, method=builder)
+    assert info["err"] == 0
+    dsc = gpu_device.scene_bvh(meshes, layout=layout, max_leaf=max_leaf, builder=BUILDERS[builder])
+    nb, pb = dsc.download()
+    assert dsc.info()["nodeCount"] == info["nodes"]
+    assert nb.tobytes() == nodes.tobytes()
+    assert pb.tobytes() == tris.tobytes()
+    dsc.close()
+
+
+@pytest.mark.parametrize("builder", list(BUILDERS))
 @pytest.mark.parametrize("layout", [0, 1])
-def test_device_build_trace_parity(gpu_device, layout):
-    _case("devbuild_parity", layout=layout)
+def test_device_build_trace_parity(gpu_device, layout, builder):
+    _case("devbuild_parity", layout=layout, builder=builder)
     meshes = scenes.plain_scene(seed=1, n_random=20000)
     rays = scenes.plain_rays(1 << 18, seed=21)
     rays["flags"][::11] = 1
-    dsc = gpu_device.scene_bvh(meshes, layout=layout, builder=D.BUILD_DEVICE_LBVH)
+    dsc = gpu_device.scene_bvh(meshes, layout=layout, builder=BUILDERS[builder])
     init = np.zeros(rays.shape[0], HIT_DTYPE)
     init["t"] = 3.25
     got = gpu_device.trace_numpy(dsc, rays, hits_init=init)
@@ -88,33 +124,38 @@ def test_device_build_trace_parity(gpu_device, layout):
     dsc.close()
 
 
-def test_device_build_small_and_ragged(gpu_device):
-    _case("devbuild_small")
+@pytest.mark.parametrize("builder", list(BUILDERS))
+def test_device_build_small_and_ragged(gpu_device, builder):
+    _case("devbuild_small", builder=builder)
     tri = scenes.Mesh(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), np.array([[0, 1, 2]], np.uint32))
     empty = scenes.Mesh(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32))
     box = scenes.box_mesh()
     rng = np.random.default_rng(5)
-    for meshes in ([tri], [empty, tri], [box], [empty, box, empty, tri], [scenes.random_triangles(9, (0, 0, 0), (10, 10, 10), rng)]):
+    # 300 copies of one triangle: every box equal (the clustering's stalled rounds)
+    same = scenes.Mesh(np.tile(tri.verts, (300, 1)), np.arange(900, dtype=np.uint32).reshape(300, 3))
+    for meshes in ([tri], [empty, tri], [box], [empty, box, empty, tri],
+                   [scenes.random_triangles(9, (0, 0, 0), (10, 10, 10), rng)], [same, box]):
         for max_leaf in (1, 4, 8):
-            dsc = gpu_device.scene_bvh(meshes, max_leaf=max_leaf, builder=D.BUILD_DEVICE_LBVH)
+            dsc = gpu_device.scene_bvh(meshes, max_leaf=max_leaf, builder=BUILDERS[builder])
             rays = scenes.random_rays(4096, (-1, -1, -1), (11, 11, 11), rng)
             got = gpu_device.trace_numpy(dsc, rays)
             ref = oracle.bruteforce(meshes, rays)
             assert_parity(compare(ref, got))
             dsc.close()
-    dsc = gpu_device.scene_bvh([empty], builder=D.BUILD_DEVICE_LBVH)
+    dsc = gpu_device.scene_bvh([empty], builder=BUILDERS[builder])
     got = gpu_device.trace_numpy(dsc, scenes.plain_rays(256, seed=3))
     assert np.all(got["meshIndex"] == NULL_INDEX)
     dsc.close()
 
 
-def test_device_build_heightfield_equals_host_tree(gpu_device):
+@pytest.mark.parametrize("builder", list(BUILDERS))
+def test_device_build_heightfield_equals_host_tree(gpu_device, builder):
     """Full 26M-triangle scene: the device-built tree's closest hits equal the
     host-built tree's (same rays) outside exact ties and the edge band."""
-    _case("devbuild_heightfield")
+    _case("devbuild_heightfield", builder=builder)
     meshes = scenes.heightfield_scene()
     hsc = gpu_device.scene_bvh(meshes)
-    dsc = gpu_device.scene_bvh(meshes, builder=D.BUILD_DEVICE_LBVH)
+    dsc = gpu_device.scene_bvh(meshes, builder=BUILDERS[builder])
     print("host build", hsc.info()["buildSeconds"], "device build", dsc.info()["buildSeconds"],
           "upload", dsc.info()["uploadSeconds"], flush=True)
     prim = scenes.pinhole_rays(960, 540)
