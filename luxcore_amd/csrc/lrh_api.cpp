@@ -834,6 +834,7 @@ void deviceBuild(lrh_scene *s, const lrh_mesh *meshes, unsigned nMeshes, unsigne
 	in.maxLeaf = s->params.maxLeaf;
 	in.layout = s->layout;
 	in.method = builder == LRH_BUILD_DEVICE_PLOC ? 1u : 0u;
+	in.travCost = s->params.travCost;
 	LrhGpuAlloc a{gbAlloc, gbRelease, dev};
 	LrhGpuBuildOut out;
 	try {

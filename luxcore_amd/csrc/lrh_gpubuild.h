@@ -29,6 +29,8 @@ struct LrhGpuBuildIn {
 	uint32_t maxLeaf;            // 1..8
 	uint32_t layout;             // 0 quantized 64 B nodes, 1 fp32 128 B nodes
 	uint32_t method = 0;         // binary tree over the sorted codes: 0 radix tree (LBVH), 1 clustering (PLOC)
+	float travCost = 1.f;        // cost of a binary node per box area, relative to one triangle test:
+	                             // a node of at most maxLeaf triangles is one leaf when that costs no more
 };
 
 struct LrhGpuBuildOut {

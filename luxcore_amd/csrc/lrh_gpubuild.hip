@@ -8,7 +8,9 @@
  *      scan of the [digit][tile] table, a scatter that ranks equal digits
  *      in index order with wave ballots), then checked to be in order;
  *   3. binary radix tree over the sorted codes;
- *   4. boxes bottom-up: the second child to finish computes its parent;
+ *   4. boxes bottom-up: the second child to finish computes its parent, and
+ *      its surface-area cost, which decides whether a node of at most
+ *      maxLeaf triangles becomes one leaf;
  *   5. breadth-first collapse into 4-wide nodes, child slots from an
  *      exclusive scan, so the node order is deterministic;
  *   6. triangle records in leaf order, bit for bit the host records.
@@ -232,10 +234,11 @@ __device__ GBox childBox(uint32_t c, const GBox *leafBox, const GBox *nodeBox) {
 	return b;
 }
 
-// 4. boxes bottom-up; also the leaf boxes in sorted order
+// 4. boxes and surface-area costs bottom-up; also the leaf boxes in sorted order
 __global__ void k_boxesUp(const GBox *boxes, const uint32_t *vals, uint32_t n, const uint32_t *left,
-		const uint32_t *right, const uint32_t *iparent, const uint32_t *lparent, uint32_t *flags, GBox *leafBox,
-		GBox *nodeBox, uint32_t *err) {
+		const uint32_t *right, const uint32_t *first, const uint32_t *last, const uint32_t *iparent,
+		const uint32_t *lparent, uint32_t maxLeaf, float ci, uint32_t *flags, GBox *leafBox, GBox *nodeBox,
+		float *cost, uint32_t *leafOK, uint32_t *err) {
 	const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
 	if (j >= n) return;
 	leafBox[j] = boxes[vals[j]];
@@ -251,12 +254,14 @@ __global__ void k_boxesUp(const GBox *boxes, const uint32_t *vals, uint32_t n, c
 		__threadfence();
 		if (atomicAdd(&flags[p], 1u) == 0u) return;  // the sibling finishes this node
 		__threadfence();
-		const GBox a = childBox(left[p], leafBox, nodeBox), b = childBox(right[p], leafBox, nodeBox);
-		GBox u;
-		for (int k = 0; k < 3; ++k) {
-			u.lo[k] = minT(a.lo[k], b.lo[k]);
-			u.hi[k] = maxT(a.hi[k], b.hi[k]);
-		}
+		const uint32_t l = left[p], r = right[p];
+		const GBox a = childBox(l, leafBox, nodeBox), b = childBox(r, leafBox, nodeBox);
+		const GBox u = unionBox(a, b);
+		const float cl = (l & kLeafBit) ? areaOf(a) : loadF(cost + l);
+		const float cr = (r & kLeafBit) ? areaOf(b) : loadF(cost + r);
+		uint32_t lf;
+		cost[p] = nodeCost(areaOf(u), cl, cr, last[p] - first[p] + 1u, maxLeaf, ci, lf);
+		leafOK[p] = lf;
 		nodeBox[p] = u;
 		p = iparent[p];
 	}
@@ -315,10 +320,21 @@ __global__ void k_plocApply(const uint32_t *nn, uint32_t nc, uint32_t forced, co
 	}
 }
 
-// leaves per subtree, the nodes of one round (their children are older)
-__global__ void k_plocCount(uint32_t b, uint32_t e, const uint32_t *left, const uint32_t *right, uint32_t *count) {
+// leaves per subtree and surface-area costs, the nodes of one round (their
+// children are older)
+__global__ void k_plocCount(uint32_t b, uint32_t e, const uint32_t *left, const uint32_t *right,
+		const GBox *leafBox, const GBox *nodeBox, uint32_t maxLeaf, float ci, uint32_t *count, float *cost,
+		uint32_t *leafOK) {
 	const uint32_t x = b + blockIdx.x * blockDim.x + threadIdx.x;
-	if (x < e) count[x] = plocSize(left[x], count) + plocSize(right[x], count);
+	if (x >= e) return;
+	const uint32_t l = left[x], r = right[x];
+	const uint32_t c = plocSize(l, count) + plocSize(r, count);
+	count[x] = c;
+	const float cl = (l & kLeafBit) ? areaOf(leafBox[l & ~kLeafBit]) : cost[l];
+	const float cr = (r & kLeafBit) ? areaOf(leafBox[r & ~kLeafBit]) : cost[r];
+	uint32_t lf;
+	cost[x] = nodeCost(areaOf(nodeBox[x]), cl, cr, c, maxLeaf, ci, lf);
+	leafOK[x] = lf;
 }
 
 // first leaf place of each subtree, top down (the nodes of one round; their
@@ -336,9 +352,9 @@ __global__ void k_plocFirst(uint32_t b, uint32_t e, const uint32_t *left, const 
 // the clustering's tree in the radix tree's form: internal node x becomes
 // n-2-x (root 0), leaves their places; leaf values and boxes in place order
 __global__ void k_plocFinish(uint32_t n, const uint32_t *left, const uint32_t *right, const uint32_t *count,
-		const uint32_t *first, const GBox *nodeBox, const uint32_t *leafFirst, const uint32_t *vals,
-		const GBox *leafBoxIn, uint32_t *left2, uint32_t *right2, uint32_t *first2, uint32_t *last2, GBox *nodeBox2,
-		uint32_t *vals2, GBox *leafBox2) {
+		const uint32_t *first, const GBox *nodeBox, const uint32_t *leafOK, const uint32_t *leafFirst,
+		const uint32_t *vals, const GBox *leafBoxIn, uint32_t *left2, uint32_t *right2, uint32_t *first2,
+		uint32_t *last2, GBox *nodeBox2, uint32_t *leafOK2, uint32_t *vals2, GBox *leafBox2) {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
 	const uint32_t p = leafFirst[i];
@@ -352,12 +368,13 @@ __global__ void k_plocFinish(uint32_t n, const uint32_t *left, const uint32_t *r
 	first2[y] = first[i];
 	last2[y] = first[i] + count[i] - 1;
 	nodeBox2[y] = nodeBox[i];
+	leafOK2[y] = leafOK[i];
 }
 
 // 5a. children of each frontier node and how many become inner nodes
 __global__ void k_expand(const uint32_t *frontier, uint32_t nf, uint32_t nInner, const uint32_t *left,
-		const uint32_t *right, const uint32_t *first, const uint32_t *last, const GBox *nodeBox, uint32_t maxLeaf,
-		uint32_t *kids, uint32_t *innerCount, uint32_t *err) {
+		const uint32_t *right, const uint32_t *leafOK, const GBox *nodeBox, uint32_t *kids, uint32_t *innerCount,
+		uint32_t *err) {
 	const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
 	if (e >= nf) return;
 	const uint32_t b = frontier[e];
@@ -368,19 +385,19 @@ __global__ void k_expand(const uint32_t *frontier, uint32_t nf, uint32_t nInner,
 		return;
 	}
 	uint32_t ch[4];
-	innerCount[e] = expandNode(b, left, right, first, last, nodeBox, maxLeaf, ch);
+	innerCount[e] = expandNode(b, left, right, leafOK, nodeBox, ch);
 	for (int k = 0; k < 4; ++k) kids[4 * e + k] = ch[k];
 }
 
 // 5b. the frontier's wide nodes; inner children get slots after the level
 __global__ void k_emit(uint32_t nf, const uint32_t *kids, const uint32_t *innerOff, const uint32_t *first,
-		const uint32_t *last, const GBox *leafBox, const GBox *nodeBox, uint32_t maxLeaf, uint32_t levelBase,
+		const uint32_t *last, const uint32_t *leafOK, const GBox *leafBox, const GBox *nodeBox, uint32_t levelBase,
 		uint32_t layout, void *nodes, uint32_t *next) {
 	const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
 	if (e >= nf) return;
 	uint32_t ch[4];
 	for (int k = 0; k < 4; ++k) ch[k] = kids[4 * e + k];
-	emitNode(ch, innerOff[e], first, last, leafBox, nodeBox, maxLeaf, levelBase + nf, layout, nodes, levelBase + e,
+	emitNode(ch, innerOff[e], first, last, leafOK, leafBox, nodeBox, levelBase + nf, layout, nodes, levelBase + e,
 			next);
 }
 
@@ -480,8 +497,9 @@ std::string errorText(uint32_t e) {
 // number of rounds is bounded by the size alone.  (Natural rounds merge a
 // third of the clusters at first and a tenth or less near the top, where
 // boxes overlap; forcing there would cost tree quality.)
-void plocTree(Scratch &S, const GBox *boxes, uint32_t n, hipStream_t st, uint32_t *&vals, GBox *&leafBox,
-		uint32_t *left, uint32_t *right, uint32_t *first, uint32_t *last, GBox *nodeBox, uint32_t *roundsOut) {
+void plocTree(Scratch &S, const GBox *boxes, uint32_t n, uint32_t maxLeaf, float ci, hipStream_t st, uint32_t *&vals,
+		GBox *&leafBox, uint32_t *left, uint32_t *right, uint32_t *first, uint32_t *last, GBox *nodeBox,
+		uint32_t *leafOK, uint32_t *roundsOut) {
 	const uint32_t ni = n - 1;
 	uint32_t *cidA = S.get<uint32_t>(n), *cidB = S.get<uint32_t>(n), *nn = S.get<uint32_t>(n);
 	GBox *cboxA = S.get<GBox>(n), *cboxB = S.get<GBox>(n);
@@ -525,10 +543,13 @@ void plocTree(Scratch &S, const GBox *boxes, uint32_t n, hipStream_t st, uint32_
 	if (nodeBase != ni) throw std::runtime_error("device build: clustering left nodes unmade");
 	*roundsOut = (uint32_t)roundStart.size() - 1;
 	// leaves per subtree, oldest round first; first places, newest first
-	uint32_t *count = S.get<uint32_t>(ni), *firstP = S.get<uint32_t>(ni), *leafFirst = S.get<uint32_t>(n);
+	uint32_t *count = S.get<uint32_t>(ni), *firstP = S.get<uint32_t>(ni), *leafFirst = S.get<uint32_t>(n),
+			 *leafP = S.get<uint32_t>(ni);
+	float *cost = S.get<float>(ni);
 	for (size_t r = 0; r + 1 < roundStart.size(); ++r) {
 		const uint32_t b = roundStart[r], e = roundStart[r + 1];
-		hipLaunchKernelGGL(k_plocCount, dim3(grid(e - b, 256)), dim3(256), 0, st, b, e, left, right, count);
+		hipLaunchKernelGGL(k_plocCount, dim3(grid(e - b, 256)), dim3(256), 0, st, b, e, left, right, leafBox, nodeBox,
+				maxLeaf, ci, count, cost, leafP);
 	}
 	hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, firstP + (ni - 1), 1u, 0u);  // the root, node n-2
 	for (size_t r = roundStart.size() - 1; r-- > 0;) {
@@ -547,7 +568,7 @@ void plocTree(Scratch &S, const GBox *boxes, uint32_t n, hipStream_t st, uint32_
 	uint32_t *vals2 = S.get<uint32_t>(n);
 	GBox *leafBox2 = S.get<GBox>(n);
 	hipLaunchKernelGGL(k_plocFinish, dim3(grid(n, 256)), dim3(256), 0, st, n, leftP, rightP, count, firstP, boxP,
-			leafFirst, vals, leafBox, left, right, first, last, nodeBox, vals2, leafBox2);
+			leafP, leafFirst, vals, leafBox, left, right, first, last, nodeBox, leafOK, vals2, leafBox2);
 	GB_CHECK(hipGetLastError());
 	vals = vals2;
 	leafBox = leafBox2;
@@ -560,6 +581,8 @@ void lrhGpuBuildBVH(const LrhGpuBuildIn &in, const LrhGpuAlloc &a, hipStream_t s
 	if (n == 0) throw std::runtime_error("device build: empty scene");
 	if (n > 0x7fffffffu) throw std::runtime_error("device build: at most 2^31-1 triangles");
 	if (in.maxLeaf < 1 || in.maxLeaf > LRH_LEAF_MAX) throw std::runtime_error("device build: maxLeaf must be 1..8");
+	if (!(in.travCost >= 0.f) || !std::isfinite(in.travCost))
+		throw std::runtime_error("device build: travCost must be finite and >= 0");
 	const size_t nodeSz = in.layout == 0 ? sizeof(lrh_node4q) : sizeof(lrh_node4f);
 	Scratch S(a);
 	hipEvent_t e0, e1;
@@ -620,10 +643,13 @@ void lrhGpuBuildBVH(const LrhGpuBuildIn &in, const LrhGpuAlloc &a, hipStream_t s
 			uint32_t *left = S.get<uint32_t>(ni), *right = S.get<uint32_t>(ni), *first = S.get<uint32_t>(ni),
 					 *last = S.get<uint32_t>(ni);
 			GBox *nodeBox = S.get<GBox>(ni);
+			uint32_t *leafOK = S.get<uint32_t>(ni);
 			if (in.method == 1) {
-				plocTree(S, boxes, n, st, vals, leafBox, left, right, first, last, nodeBox, &out->plocRounds);
+				plocTree(S, boxes, n, in.maxLeaf, in.travCost, st, vals, leafBox, left, right, first, last, nodeBox,
+						leafOK, &out->plocRounds);
 			} else {
 				uint32_t *iparent = S.get<uint32_t>(ni), *lparent = S.get<uint32_t>(n), *flags = S.get<uint32_t>(ni);
+				float *cost = S.get<float>(ni);
 				hipLaunchKernelGGL(k_fill, dim3(grid(ni, 256)), dim3(256), 0, st, flags, ni, 0u);
 				hipLaunchKernelGGL(k_fill, dim3(grid(ni, 256)), dim3(256), 0, st, iparent, ni, kNone);  // root: none
 				hipLaunchKernelGGL(k_fill, dim3(grid(n, 256)), dim3(256), 0, st, lparent, n, kNone);
@@ -631,8 +657,8 @@ void lrhGpuBuildBVH(const LrhGpuBuildIn &in, const LrhGpuAlloc &a, hipStream_t s
 						last, iparent, lparent, err);
 				GB_CHECK(hipGetLastError());
 				if (const uint32_t e = readError(err, st)) throw std::runtime_error(errorText(e));
-				hipLaunchKernelGGL(k_boxesUp, dim3(grid(n, 256)), dim3(256), 0, st, boxes, vals, n, left, right,
-						iparent, lparent, flags, leafBox, nodeBox, err);
+				hipLaunchKernelGGL(k_boxesUp, dim3(grid(n, 256)), dim3(256), 0, st, boxes, vals, n, left, right, first,
+						last, iparent, lparent, in.maxLeaf, in.travCost, flags, leafBox, nodeBox, cost, leafOK, err);
 				GB_CHECK(hipGetLastError());
 				if (const uint32_t e = readError(err, st)) throw std::runtime_error(errorText(e));
 			}
@@ -646,13 +672,13 @@ void lrhGpuBuildBVH(const LrhGpuBuildIn &in, const LrhGpuAlloc &a, hipStream_t s
 			while (nf) {
 				if ((size_t)levelBase + nf > capNodes) throw std::runtime_error("device build: node capacity");
 				if (depth > 128) throw std::runtime_error("device build: collapse deeper than any valid tree");
-				hipLaunchKernelGGL(k_expand, dim3(grid(nf, 256)), dim3(256), 0, st, fA, nf, ni, left, right, first,
-						last, nodeBox, in.maxLeaf, kids, cnt, err);
+				hipLaunchKernelGGL(k_expand, dim3(grid(nf, 256)), dim3(256), 0, st, fA, nf, ni, left, right, leafOK,
+						nodeBox, kids, cnt, err);
 				hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, cnt + nf, 1u, 0u);
 				GB_CHECK(hipGetLastError());
 				exclusiveScan(S, cnt, off, nf + 1, st);
-				hipLaunchKernelGGL(k_emit, dim3(grid(nf, 256)), dim3(256), 0, st, nf, kids, off, first, last, leafBox,
-						nodeBox, in.maxLeaf, levelBase, in.layout, nodes, fB);
+				hipLaunchKernelGGL(k_emit, dim3(grid(nf, 256)), dim3(256), 0, st, nf, kids, off, first, last, leafOK,
+						leafBox, nodeBox, levelBase, in.layout, nodes, fB);
 				GB_CHECK(hipGetLastError());
 				uint32_t nextN = 0;
 				GB_CHECK(hipMemcpyAsync(&nextN, off + nf, 4, hipMemcpyDeviceToHost, st));

@@ -101,12 +101,15 @@ LRH_DN inline uint64_t spread21(uint32_t v) {
 }
 
 // 2. Morton code of the box centre within the centre bounds (ordered keys cb:
-// min x y z, max x y z)
+// min x y z, max x y z), every axis scaled by the largest extent: a cube of
+// cells, so that a flat scene (a terrain) is not ordered by height as
+// finely as by position
 LRH_DN inline uint64_t mortonKey(const GBox &b, const uint32_t cb[6]) {
 	uint32_t q[3];
+	float ext = 0.f;
+	for (int k = 0; k < 3; ++k) ext = maxT(ext, lrhdev::unkey(cb[3 + k]) - lrhdev::unkey(cb[k]));
 	for (int k = 0; k < 3; ++k) {
-		const float lo = lrhdev::unkey(cb[k]), hi = lrhdev::unkey(cb[3 + k]);
-		const float ext = hi - lo;
+		const float lo = lrhdev::unkey(cb[k]);
 		const float u = ext > 0.f ? ((b.lo[k] + b.hi[k]) - lo) / ext : 0.f;
 		float s = u * 2097152.f;
 		s = s < 0.f ? 0.f : (s > 2097151.f ? 2097151.f : s);
@@ -184,18 +187,33 @@ LRH_DN inline float areaOf(const GBox &b) {
 	return 2.f * (dx * dy + dy * dz + dz * dx);
 }
 
+// surface-area cost of internal node x of the binary tree (N triangles, box
+// area A, children costs cl, cr; a one-triangle leaf costs its box area):
+// one leaf of its N triangles when N <= maxLeaf and that costs no more than
+// a node of cost ci per area plus its children's costs
+LRH_DN inline float nodeCost(float A, float cl, float cr, uint32_t N, uint32_t maxLeaf, float ci, uint32_t &leaf) {
+	const float split = ci * A + (cl + cr);
+	const float asLeaf = (float)N * A;
+	leaf = (N <= maxLeaf && asLeaf <= split) ? 1u : 0u;
+	return leaf ? asLeaf : split;
+}
+
+// a child that becomes a leaf of the wide tree: a triangle, or a node the
+// cost chose as a leaf
+LRH_DN inline bool leafChild(uint32_t c, const uint32_t *leafOK) { return (c & kLeafBit) || leafOK[c]; }
+
 // 4a. the children of internal node b (Morton order): its two, then the
-// largest-area child above maxLeaf triangles split until there are four;
+// largest-area child that is not a leaf split until there are four;
 // returns how many of them become inner nodes
-LRH_DN inline uint32_t expandNode(uint32_t b, const uint32_t *left, const uint32_t *right, const uint32_t *first,
-		const uint32_t *last, const GBox *nodeBox, uint32_t maxLeaf, uint32_t kids[4]) {
+LRH_DN inline uint32_t expandNode(uint32_t b, const uint32_t *left, const uint32_t *right, const uint32_t *leafOK,
+		const GBox *nodeBox, uint32_t kids[4]) {
 	uint32_t ch[4] = {left[b], right[b], kNone, kNone};
 	int nch = 2;
 	while (nch < 4) {
 		int pick = -1;
 		float best = -1.f;
 		for (int k = 0; k < nch; ++k) {
-			if ((ch[k] & kLeafBit) || countOf(ch[k], first, last) <= maxLeaf) continue;
+			if (leafChild(ch[k], leafOK)) continue;
 			const float a = areaOf(nodeBox[ch[k]]);
 			if (a > best) {
 				best = a;
@@ -212,7 +230,7 @@ LRH_DN inline uint32_t expandNode(uint32_t b, const uint32_t *left, const uint32
 	uint32_t inner = 0;
 	for (int k = 0; k < 4; ++k) {
 		kids[k] = ch[k];
-		if (ch[k] != kNone && countOf(ch[k], first, last) > maxLeaf) ++inner;
+		if (ch[k] != kNone && !leafChild(ch[k], leafOK)) ++inner;
 	}
 	return inner;
 }
@@ -225,7 +243,7 @@ LRH_DN inline uint32_t leafRef(uint32_t start, uint32_t count) {
 // range, inner children the slots innerOff, innerOff+1, ... after the level
 // (their binary ids go to next[slot])
 LRH_DN inline void emitNode(const uint32_t kids[4], uint32_t innerOff, const uint32_t *first, const uint32_t *last,
-		const GBox *leafBox, const GBox *nodeBox, uint32_t maxLeaf, uint32_t childBase, uint32_t layout, void *nodes,
+		const uint32_t *leafOK, const GBox *leafBox, const GBox *nodeBox, uint32_t childBase, uint32_t layout, void *nodes,
 		uint32_t nodeIndex, uint32_t *next) {
 	GBox cb[4];
 	uint32_t ref[4] = {LRH_CHILD_EMPTY, LRH_CHILD_EMPTY, LRH_CHILD_EMPTY, LRH_CHILD_EMPTY};
@@ -235,10 +253,9 @@ LRH_DN inline void emitNode(const uint32_t kids[4], uint32_t innerOff, const uin
 		const uint32_t c = kids[k];
 		if (c == kNone) continue;
 		cb[n] = (c & kLeafBit) ? leafBox[c & ~kLeafBit] : nodeBox[c];
-		const uint32_t cnt = countOf(c, first, last);
-		if (cnt <= maxLeaf) {
+		if (leafChild(c, leafOK)) {
 			const uint32_t start = (c & kLeafBit) ? (c & ~kLeafBit) : first[c];
-			ref[n] = leafRef(start, cnt);
+			ref[n] = leafRef(start, countOf(c, first, last));
 		} else {
 			ref[n] = childBase + slot;
 			next[slot] = c;

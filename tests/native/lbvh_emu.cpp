@@ -6,13 +6,14 @@
  * [digit][tile] table, a scatter in index order), the same error word.  The
  * device build must produce these records byte for byte.
  *
- *   lbvh_emu <in.bin> <outdir> <maxLeaf> <layout> [mode] [method]
+ *   lbvh_emu <in.bin> <outdir> <maxLeaf> <layout> [mode] [method] [travCost]
  * in.bin: u32 nMeshes, then per mesh u32 nVerts, u32 nTris, float[3 nVerts],
  * u32[3 nTris].  mode: "normal" (default); "unsorted": the sort is skipped
  * (codes left in triangle order), as a sort that returned garbage would
  * leave them; "reversed": the sorted codes reversed.  method: "lbvh"
  * (default, the radix tree) or "ploc" (clustering rounds, each round's
  * steps over all clusters before the next step, as the kernels run them).
+ * travCost (default 1): the node cost of the leaf decisions.
  * Writes nodes.bin, tris.bin and info.txt ("err <word> nodes <count> depth
  * <d> rounds <r>"); with codes out of order the run stops after the step
  * that reports them, as the device build does, and still terminates: every
@@ -51,9 +52,10 @@ void exclusiveScan(std::vector<uint32_t> &v) {
 // the clustering's tree in the radix tree's form, as the device build's
 // rounds and conversion make it; returns the rounds, 0 when a round's counts
 // are inconsistent
-uint32_t plocTree(const std::vector<GBox> &boxes, std::vector<uint32_t> &vals, std::vector<GBox> &leafBox,
-		std::vector<uint32_t> &left2, std::vector<uint32_t> &right2, std::vector<uint32_t> &first2,
-		std::vector<uint32_t> &last2, std::vector<GBox> &nodeBox2) {
+uint32_t plocTree(const std::vector<GBox> &boxes, uint32_t maxLeaf, float ci, std::vector<uint32_t> &vals,
+		std::vector<GBox> &leafBox, std::vector<uint32_t> &left2, std::vector<uint32_t> &right2,
+		std::vector<uint32_t> &first2, std::vector<uint32_t> &last2, std::vector<GBox> &nodeBox2,
+		std::vector<uint32_t> &leafOK2) {
 	const uint32_t n = (uint32_t)vals.size(), ni = n - 1;
 	std::vector<uint32_t> cid(n), cidB(n), nn(n), lead(n + 1), keep(n + 1);
 	std::vector<GBox> cbox(n), cboxB(n), nodeBox(ni);
@@ -106,8 +108,15 @@ uint32_t plocTree(const std::vector<GBox> &boxes, std::vector<uint32_t> &vals, s
 		cbox.swap(cboxB);
 	}
 	if (nodeBase != ni) return 0;
-	std::vector<uint32_t> count(ni), first(ni), leafFirst(n);
-	for (uint32_t x = 0; x < ni; ++x) count[x] = plocSize(left[x], count.data()) + plocSize(right[x], count.data());
+	std::vector<uint32_t> count(ni), first(ni), leafFirst(n), leafOK(ni);
+	std::vector<float> cost(ni);
+	for (uint32_t x = 0; x < ni; ++x) {
+		const uint32_t l = left[x], r = right[x];
+		count[x] = plocSize(l, count.data()) + plocSize(r, count.data());
+		const float cl = (l & kLeafBit) ? areaOf(leafBox[l & ~kLeafBit]) : cost[l];
+		const float cr = (r & kLeafBit) ? areaOf(leafBox[r & ~kLeafBit]) : cost[r];
+		cost[x] = nodeCost(areaOf(nodeBox[x]), cl, cr, count[x], maxLeaf, ci, leafOK[x]);
+	}
 	first[ni - 1] = 0;
 	for (uint32_t x = ni; x-- > 0;) {
 		const uint32_t f = first[x], l = left[x], r = right[x], fr = f + plocSize(l, count.data());
@@ -128,6 +137,7 @@ uint32_t plocTree(const std::vector<GBox> &boxes, std::vector<uint32_t> &vals, s
 		first2[y] = first[i];
 		last2[y] = first[i] + count[i] - 1;
 		nodeBox2[y] = nodeBox[i];
+		leafOK2[y] = leafOK[i];
 	}
 	vals.swap(vals2);
 	leafBox.swap(leafBox2);
@@ -138,7 +148,7 @@ uint32_t plocTree(const std::vector<GBox> &boxes, std::vector<uint32_t> &vals, s
 
 int main(int argc, char **argv) {
 	if (argc < 5) {
-		std::cerr << "usage: lbvh_emu in.bin outdir maxLeaf layout [normal|unsorted|reversed] [lbvh|ploc]\n";
+		std::cerr << "usage: lbvh_emu in.bin outdir maxLeaf layout [normal|unsorted|reversed] [lbvh|ploc] [travCost]\n";
 		return 2;
 	}
 	const std::string outDir = argv[2];
@@ -146,6 +156,7 @@ int main(int argc, char **argv) {
 	const uint32_t layout = (uint32_t)std::stoul(argv[4]);
 	const std::string mode = argc > 5 ? argv[5] : "normal";
 	const bool ploc = argc > 6 && std::string(argv[6]) == "ploc";
+	const float ci = argc > 7 ? std::stof(argv[7]) : 1.f;
 	std::ifstream in(argv[1], std::ios::binary);
 	auto rd = [&](void *p, size_t b) {
 		in.read(reinterpret_cast<char *>(p), (std::streamsize)b);
@@ -244,8 +255,10 @@ int main(int argc, char **argv) {
 		std::vector<uint32_t> left(ni), right(ni), first(ni), last(ni), iparent(ni, kNone), lparent(n, kNone),
 				flags(ni, 0);
 		std::vector<GBox> nodeBox(ni);
+		std::vector<uint32_t> leafOK(ni, 0);
+		std::vector<float> cost(ni, 0.f);
 		if (ploc && !err) {
-			rounds = plocTree(boxes, vals, leafBox, left, right, first, last, nodeBox);
+			rounds = plocTree(boxes, maxLeaf, ci, vals, leafBox, left, right, first, last, nodeBox, leafOK);
 			if (rounds == 0) err |= kErrExpand;
 		}
 		for (uint32_t i = 0; i < ni && !ploc; ++i) {
@@ -273,14 +286,13 @@ int main(int argc, char **argv) {
 						break;
 					}
 					if (flags[p]++ == 0u) break;
-					const uint32_t cl = left[p], cr = right[p];
-					const GBox &a = (cl & kLeafBit) ? leafBox[cl & ~kLeafBit] : nodeBox[cl];
-					const GBox &b = (cr & kLeafBit) ? leafBox[cr & ~kLeafBit] : nodeBox[cr];
-					GBox u;
-					for (int k = 0; k < 3; ++k) {
-						u.lo[k] = minT(a.lo[k], b.lo[k]);
-						u.hi[k] = maxT(a.hi[k], b.hi[k]);
-					}
+					const uint32_t l = left[p], r = right[p];
+					const GBox &a = (l & kLeafBit) ? leafBox[l & ~kLeafBit] : nodeBox[l];
+					const GBox &b = (r & kLeafBit) ? leafBox[r & ~kLeafBit] : nodeBox[r];
+					const GBox u = unionBox(a, b);
+					const float cl = (l & kLeafBit) ? areaOf(a) : cost[l];
+					const float cr = (r & kLeafBit) ? areaOf(b) : cost[r];
+					cost[p] = nodeCost(areaOf(u), cl, cr, last[p] - first[p] + 1u, maxLeaf, ci, leafOK[p]);
 					nodeBox[p] = u;
 					p = iparent[p];
 				}
@@ -301,14 +313,14 @@ int main(int argc, char **argv) {
 						err |= kErrExpand;
 						continue;
 					}
-					cnt[e] = expandNode(fA[e], left.data(), right.data(), first.data(), last.data(), nodeBox.data(),
-							maxLeaf, &kids[4 * (size_t)e]);
+					cnt[e] = expandNode(fA[e], left.data(), right.data(), leafOK.data(), nodeBox.data(),
+							&kids[4 * (size_t)e]);
 				}
 				exclusiveScan(cnt);
 				fB.assign(cnt[nf], 0);
 				for (uint32_t e = 0; e < nf; ++e)
-					emitNode(&kids[4 * (size_t)e], cnt[e], first.data(), last.data(), leafBox.data(), nodeBox.data(),
-							maxLeaf, levelBase + nf, layout, nodes.data(), levelBase + e, fB.data());
+					emitNode(&kids[4 * (size_t)e], cnt[e], first.data(), last.data(), leafOK.data(), leafBox.data(),
+							nodeBox.data(), levelBase + nf, layout, nodes.data(), levelBase + e, fB.data());
 				levelBase += nf;
 				fA.swap(fB);
 				++depth;
