@@ -8,8 +8,8 @@
  *      scan of the [digit][tile] table, a scatter that ranks equal digits
  *      in index order with wave ballots), then checked to be in order;
  *   3. binary radix tree over the sorted codes;
- *   4. boxes bottom-up: the second child to finish computes its parent, and
- *      its surface-area cost, which decides whether a node of at most
+ *   4. boxes bottom-up, one launch per depth (depths by pointer jumping),
+ *      and surface-area costs, which decide whether a node of at most
  *      maxLeaf triangles becomes one leaf;
  *   5. breadth-first collapse into 4-wide nodes, child slots from an
  *      exclusive scan, so the node order is deterministic;
@@ -202,7 +202,7 @@ __global__ void k_checkSorted(const uint64_t *keys, uint32_t n, uint32_t *err) {
 
 // 3. binary radix tree
 __global__ void k_radixTree(const uint64_t *keys, uint32_t n, uint32_t *left, uint32_t *right, uint32_t *first,
-		uint32_t *last, uint32_t *iparent, uint32_t *lparent, uint32_t *err) {
+		uint32_t *last, uint32_t *iparent, uint32_t *err) {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i + 1 >= n) return;
 	uint32_t lc, rc, lo, hi;
@@ -214,57 +214,80 @@ __global__ void k_radixTree(const uint64_t *keys, uint32_t n, uint32_t *left, ui
 	right[i] = rc;
 	first[i] = lo;
 	last[i] = hi;
-	if (lc & kLeafBit) lparent[lc & ~kLeafBit] = i; else iparent[lc] = i;
-	if (rc & kLeafBit) lparent[rc & ~kLeafBit] = i; else iparent[rc] = i;
+	if (!(lc & kLeafBit)) iparent[lc] = i;
+	if (!(rc & kLeafBit)) iparent[rc] = i;
 }
 
-__device__ __forceinline__ float loadF(const float *p) {
-	// read another thread's result past this CU's L1
-	return __uint_as_float(__hip_atomic_load(reinterpret_cast<const uint32_t *>(p), __ATOMIC_RELAXED,
-			__HIP_MEMORY_SCOPE_AGENT));
+// 4. node depths by pointer jumping: after round r every node knows its
+// distance to the root if that is below 2^(r+1), and jump[x] is its ancestor
+// 2^(r+1) levels up, or the root
+__global__ void k_depthInit(const uint32_t *iparent, uint32_t ni, uint32_t *depth, uint32_t *jump) {
+	const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+	if (x >= ni) return;
+	const uint32_t p = iparent[x];
+	depth[x] = (x == 0 || p >= ni) ? 0u : 1u;
+	jump[x] = (x == 0 || p >= ni) ? 0u : p;
 }
 
-__device__ GBox childBox(uint32_t c, const GBox *leafBox, const GBox *nodeBox) {
-	const float *p = (c & kLeafBit) ? leafBox[c & ~kLeafBit].lo : nodeBox[c].lo;
-	GBox b;
-	for (int k = 0; k < 3; ++k) {
-		b.lo[k] = loadF(p + k);
-		b.hi[k] = loadF(p + 3 + k);
-	}
-	return b;
+__global__ void k_depthJump(uint32_t ni, const uint32_t *depth, const uint32_t *jump, uint32_t *depth2,
+		uint32_t *jump2) {
+	const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+	if (x >= ni) return;
+	const uint32_t j = jump[x];
+	depth2[x] = depth[x] + depth[j];
+	jump2[x] = jump[j];
 }
 
-// 4. boxes and surface-area costs bottom-up; also the leaf boxes in sorted order
-__global__ void k_boxesUp(const GBox *boxes, const uint32_t *vals, uint32_t n, const uint32_t *left,
-		const uint32_t *right, const uint32_t *first, const uint32_t *last, const uint32_t *iparent,
-		const uint32_t *lparent, uint32_t maxLeaf, float ci, uint32_t *flags, GBox *leafBox, GBox *nodeBox,
-		float *cost, uint32_t *leafOK, uint32_t *err) {
-	const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-	if (j >= n) return;
-	leafBox[j] = boxes[vals[j]];
-	uint32_t p = lparent[j];
-	// a binary radix tree over n keys is at most 96 levels deep (64 code bits
-	// + 32 position bits)
-	int steps = 0;
-	for (; p < n - 1; ++steps) {
-		if (steps >= 128) {
+// nodes per depth (depth of a node not yet at the root: an error); the
+// deepest level is histogram slot kMaxDepth
+constexpr uint32_t kMaxDepth = 128;
+__global__ void k_depthCount(uint32_t ni, const uint32_t *depth, const uint32_t *jump, uint32_t *hist,
+		uint32_t *err) {
+	__shared__ uint32_t h[kMaxDepth];
+	for (uint32_t k = threadIdx.x; k < kMaxDepth; k += blockDim.x) h[k] = 0;
+	__syncthreads();
+	const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+	if (x < ni) {
+		if (jump[x] != 0 || depth[x] >= kMaxDepth)
 			atomicOr(err, kErrClimb);
-			return;
-		}
-		__threadfence();
-		if (atomicAdd(&flags[p], 1u) == 0u) return;  // the sibling finishes this node
-		__threadfence();
-		const uint32_t l = left[p], r = right[p];
-		const GBox a = childBox(l, leafBox, nodeBox), b = childBox(r, leafBox, nodeBox);
-		const GBox u = unionBox(a, b);
-		const float cl = (l & kLeafBit) ? areaOf(a) : loadF(cost + l);
-		const float cr = (r & kLeafBit) ? areaOf(b) : loadF(cost + r);
-		uint32_t lf;
-		cost[p] = nodeCost(areaOf(u), cl, cr, last[p] - first[p] + 1u, maxLeaf, ci, lf);
-		leafOK[p] = lf;
-		nodeBox[p] = u;
-		p = iparent[p];
+		else
+			atomicAdd(&h[depth[x]], 1u);
 	}
+	__syncthreads();
+	for (uint32_t k = threadIdx.x; k < kMaxDepth; k += blockDim.x)
+		if (h[k]) atomicAdd(&hist[k], h[k]);
+}
+
+// sort keys (depths) and values (node ids) for the counting sort by depth
+__global__ void k_depthKeys(uint32_t ni, const uint32_t *depth, uint64_t *keys, uint32_t *ids) {
+	const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+	if (x >= ni) return;
+	keys[x] = depth[x];
+	ids[x] = x;
+}
+
+__global__ void k_leafBoxes(const GBox *boxes, const uint32_t *vals, uint32_t n, GBox *leafBox) {
+	const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j < n) leafBox[j] = boxes[vals[j]];
+}
+
+// boxes and surface-area costs of the nodes of one depth, deepest level
+// first: the children's are the previous launch's
+__global__ void k_boxLevel(const uint32_t *order, uint32_t cnt, const uint32_t *left, const uint32_t *right,
+		const uint32_t *first, const uint32_t *last, const GBox *leafBox, uint32_t maxLeaf, float ci, GBox *nodeBox,
+		float *cost, uint32_t *leafOK) {
+	const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= cnt) return;
+	const uint32_t x = order[e], l = left[x], r = right[x];
+	const GBox a = (l & kLeafBit) ? leafBox[l & ~kLeafBit] : nodeBox[l];
+	const GBox b = (r & kLeafBit) ? leafBox[r & ~kLeafBit] : nodeBox[r];
+	const GBox u = unionBox(a, b);
+	const float cl = (l & kLeafBit) ? areaOf(a) : cost[l];
+	const float cr = (r & kLeafBit) ? areaOf(b) : cost[r];
+	uint32_t lf;
+	cost[x] = nodeCost(areaOf(u), cl, cr, last[x] - first[x] + 1u, maxLeaf, ci, lf);
+	leafOK[x] = lf;
+	nodeBox[x] = u;
 }
 
 // 3'. clustering (PLOC): clusters start as the sorted leaves
@@ -482,9 +505,63 @@ std::string errorText(uint32_t e) {
 	std::string s = "device build: ";
 	if (e & kErrUnsorted) s += "sorted Morton codes out of order; ";
 	if (e & kErrRange) s += "radix tree node without a key range (codes not in order); ";
-	if (e & kErrClimb) s += "bottom-up box walk longer than any valid tree; ";
+	if (e & kErrClimb) s += "radix tree deeper than any valid tree (or not rooted); ";
 	if (e & kErrExpand) s += "collapse reached a non-internal node; ";
 	return s;
+}
+
+// boxes, costs and leaf decisions of the radix tree's nodes bottom-up with a
+// kernel boundary between levels instead of a fenced walk per leaf (a walk
+// publishes each box to the other XCDs' caches by an agent-scope fence,
+// which writes back and invalidates L2 on every step): depths by pointer
+// jumping, nodes sorted by depth, then one launch per level from the
+// deepest.
+void boxesByDepth(Scratch &S, const GBox *boxes, const uint32_t *vals, uint32_t n, const uint32_t *left,
+		const uint32_t *right, const uint32_t *first, const uint32_t *last, const uint32_t *iparent, uint32_t maxLeaf,
+		float ci, hipStream_t st, uint32_t *err, GBox *leafBox, GBox *nodeBox, uint32_t *leafOK) {
+	const uint32_t ni = n - 1;
+	uint32_t *dA = S.get<uint32_t>(ni), *dB = S.get<uint32_t>(ni), *jA = S.get<uint32_t>(ni),
+			 *jB = S.get<uint32_t>(ni), *order = S.get<uint32_t>(ni), *hist = S.get<uint32_t>(kMaxDepth);
+	uint64_t *dKeys = S.get<uint64_t>(ni), *dKeysOut = S.get<uint64_t>(ni);
+	float *cost = S.get<float>(ni);
+	hipLaunchKernelGGL(k_depthInit, dim3(grid(ni, 256)), dim3(256), 0, st, iparent, ni, dA, jA);
+	// depth < kMaxDepth = 2^7: seven doublings reach the root
+	for (int r = 0; r < 7; ++r) {
+		hipLaunchKernelGGL(k_depthJump, dim3(grid(ni, 256)), dim3(256), 0, st, ni, dA, jA, dB, jB);
+		std::swap(dA, dB);
+		std::swap(jA, jB);
+	}
+	hipLaunchKernelGGL(k_fill, dim3(1), dim3(kMaxDepth), 0, st, hist, kMaxDepth, 0u);
+	hipLaunchKernelGGL(k_depthCount, dim3(grid(ni, 256)), dim3(256), 0, st, ni, dA, jA, hist, err);
+	GB_CHECK(hipGetLastError());
+	uint32_t h[kMaxDepth];
+	GB_CHECK(hipMemcpyAsync(h, hist, sizeof(h), hipMemcpyDeviceToHost, st));
+	if (const uint32_t e = readError(err, st)) throw std::runtime_error(errorText(e));
+	uint32_t off[kMaxDepth + 1];
+	off[0] = 0;
+	for (uint32_t d = 0; d < kMaxDepth; ++d) off[d + 1] = off[d] + h[d];
+	if (off[kMaxDepth] != ni) throw std::runtime_error("device build: radix tree nodes not all below the root");
+	// nodes in depth order: one stable pass of the radix sort (depth < 2^8
+	// is one digit) with the node ids as values, so no counter is contended
+	const uint32_t tiles = grid(ni, kSortTile);
+	uint32_t *table = S.get<uint32_t>((size_t)256 * tiles), *ids = S.get<uint32_t>(ni);
+	hipLaunchKernelGGL(k_depthKeys, dim3(grid(ni, 256)), dim3(256), 0, st, ni, dA, dKeys, ids);
+	hipLaunchKernelGGL(k_sortCount, dim3(tiles), dim3(kSortBlock), 0, st, dKeys, ni, 0u, tiles, table);
+	GB_CHECK(hipGetLastError());
+	exclusiveScan(S, table, table, 256u * tiles, st);
+	hipLaunchKernelGGL(k_sortScatter, dim3(tiles), dim3(kSortBlock), 0, st, dKeys, ids, ni, 0u, tiles, table, dKeysOut,
+			order, err);
+	hipLaunchKernelGGL(k_leafBoxes, dim3(grid(n, 256)), dim3(256), 0, st, boxes, vals, n, leafBox);
+	GB_CHECK(hipGetLastError());
+	for (uint32_t d = kMaxDepth; d-- > 0;) {
+		const uint32_t cnt = h[d];
+		if (!cnt) continue;
+		hipLaunchKernelGGL(k_boxLevel, dim3(grid(cnt, 256)), dim3(256), 0, st, order + off[d], cnt, left, right, first,
+				last, leafBox, maxLeaf, ci, nodeBox, cost, leafOK);
+	}
+	GB_CHECK(hipGetLastError());
+	GB_CHECK(hipGetLastError());
+	if (const uint32_t e = readError(err, st)) throw std::runtime_error(errorText(e));
 }
 
 // the clustering's binary tree over the n >= 2 sorted leaves, written in the
@@ -648,19 +725,14 @@ void lrhGpuBuildBVH(const LrhGpuBuildIn &in, const LrhGpuAlloc &a, hipStream_t s
 				plocTree(S, boxes, n, in.maxLeaf, in.travCost, st, vals, leafBox, left, right, first, last, nodeBox,
 						leafOK, &out->plocRounds);
 			} else {
-				uint32_t *iparent = S.get<uint32_t>(ni), *lparent = S.get<uint32_t>(n), *flags = S.get<uint32_t>(ni);
-				float *cost = S.get<float>(ni);
-				hipLaunchKernelGGL(k_fill, dim3(grid(ni, 256)), dim3(256), 0, st, flags, ni, 0u);
+				uint32_t *iparent = S.get<uint32_t>(ni);
 				hipLaunchKernelGGL(k_fill, dim3(grid(ni, 256)), dim3(256), 0, st, iparent, ni, kNone);  // root: none
-				hipLaunchKernelGGL(k_fill, dim3(grid(n, 256)), dim3(256), 0, st, lparent, n, kNone);
 				hipLaunchKernelGGL(k_radixTree, dim3(grid(ni, 256)), dim3(256), 0, st, keys, n, left, right, first,
-						last, iparent, lparent, err);
+						last, iparent, err);
 				GB_CHECK(hipGetLastError());
 				if (const uint32_t e = readError(err, st)) throw std::runtime_error(errorText(e));
-				hipLaunchKernelGGL(k_boxesUp, dim3(grid(n, 256)), dim3(256), 0, st, boxes, vals, n, left, right, first,
-						last, iparent, lparent, in.maxLeaf, in.travCost, flags, leafBox, nodeBox, cost, leafOK, err);
-				GB_CHECK(hipGetLastError());
-				if (const uint32_t e = readError(err, st)) throw std::runtime_error(errorText(e));
+				boxesByDepth(S, boxes, vals, n, left, right, first, last, iparent, in.maxLeaf, in.travCost, st, err,
+						leafBox, nodeBox, leafOK);
 			}
 
 			// breadth-first collapse: level frontiers of binary node ids

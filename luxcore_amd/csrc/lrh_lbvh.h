@@ -36,7 +36,7 @@ constexpr uint32_t kNone = 0xffffffffu;
 enum : uint32_t {
 	kErrUnsorted = 1u,   // sorted codes out of order
 	kErrRange = 2u,      // a radix-tree node found no range (keys not in order around it)
-	kErrClimb = 4u,      // a bottom-up box walk longer than any valid tree
+	kErrClimb = 4u,      // a radix-tree node deeper than any valid tree (or not below the root)
 	kErrExpand = 8u,     // a frontier entry that is not an internal node
 };
 
