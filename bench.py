@@ -438,6 +438,62 @@ def run_refit(args, dev):
             "speedup": round(res["update"]["edit_ms_median"] / res["refit"]["edit_ms_median"], 1)}
 
 
+def run_build(args, dev):
+    """Device BVH builds of the headline scene (26.2M triangles): LBVH (radix
+    tree) and PLOC (clustering), each built --steps times (vertices and
+    triangles uploaded from host memory each time, timed apart), and the
+    closest-hit rate of the headline bounce rays over each tree and over the
+    host binned-SAH tree."""
+    meshes = scenes.heightfield_scene(cells=args.cells, n_spheres=64, seed=3)
+    ntris = sum(m.ntris for m in meshes)
+    t0 = time.perf_counter()
+    sah = dev.scene_bvh(meshes, layout=args.layout, max_leaf=args.max_leaf)
+    sah_s = time.perf_counter() - t0
+    prim = scenes.pinhole_rays(1920, 1080)
+    rays = scenes.bounce_rays(meshes, prim, dev.trace_numpy(sah, prim), seed=5)
+    n = rays.shape[0]
+    rb = dev.upload(rays)
+    hb = dev.alloc(n * HIT_DTYPE.itemsize)
+    trace_steps = 50
+
+    def trace_rate(sc):
+        wall, _ = _event_timed(dev, lambda: dev.trace(sc, rb, hb, n, 0), trace_steps, 5)
+        vis, tests, cnt = dev.trace_stats(sc, rb, hb, n, 0)
+        return round(n * trace_steps / wall / 1e6, 1), round(vis / cnt, 2), round(tests / cnt, 2)
+    res = {}
+    r, v, t = trace_rate(sah)
+    res["host_sah"] = {"build_s": round(sah_s, 3), "trace_mrays": r, "node_visits": v, "tri_tests": t,
+                       "nodes": sah.info()["nodeCount"]}
+    sah.close()
+    log(f"host SAH: {res['host_sah']}")
+    for name, b in (("lbvh", D.BUILD_DEVICE_LBVH), ("ploc", D.BUILD_DEVICE_PLOC)):
+        builds, uploads, sc = [], [], None
+        for _ in range(args.warmup // 5 + 1 + args.steps):
+            if sc is not None:
+                sc.close()
+            sc = dev.scene_bvh(meshes, layout=args.layout, max_leaf=args.max_leaf, builder=b)
+            builds.append(sc.info()["buildSeconds"])
+            uploads.append(sc.info()["uploadSeconds"])
+        builds, uploads = builds[-args.steps:], uploads[-args.steps:]
+        r, v, t = trace_rate(sc)
+        med = float(np.median(builds))
+        res[name] = {"build_s_median": round(med, 4), "build_s_min": round(min(builds), 4),
+                     "upload_s_median": round(float(np.median(uploads)), 4),
+                     "mtris_per_s": round(ntris / med / 1e6, 1), "trace_mrays": r, "node_visits": v,
+                     "tri_tests": t, "nodes": sc.info()["nodeCount"], "depth": sc.info()["maxDepth"]}
+        sc.close()
+        log(f"{name}: {res[name]}")
+    return {"metric": "device BVH build throughput (Mtris/s), LBVH", "value": res["lbvh"]["mtris_per_s"],
+            "unit": "Mtris/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup // 5 + 1,
+            "ms_per_step": round(res["lbvh"]["build_s_median"] * 1e3, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "heightfield 3240x3240 fBm + 64 L6 icospheres; build from vertices and "
+                                   "triangles resident in HBM (upload timed apart); trace: the headline bounce "
+                                   "rays", "triangles": ntris, "rays": n, "layout": args.layout,
+                       "max_leaf": args.max_leaf},
+            "builders": res}
+
+
 def run_configs(args, dev):
     """Closest-hit and any-hit rates of every workload the parity tests use,
     HIP-event timed, with the oracle on this host's cores beside each
@@ -624,14 +680,16 @@ def calibration_copy(args, dev):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--mode", choices=("headline", "batched", "served", "scene", "refit", "configs", "render"),
+    ap.add_argument("--mode", choices=("headline", "batched", "served", "scene", "refit", "configs", "render",
+                                       "build"),
                     default="headline",
                     help="headline: incoherent closest-hit on the 26.2M-tri scene (the default JSON line); "
                          "batched: 64 buffers over 8 resident scenes in one launch; served: continuous batching; "
                          "scene: a LuxCore scene file (--scene, default the bundled lightinstances scene); "
                          "refit: two-level edit latency, host rebuild (lrh_scene_update) vs GPU refit; "
                          "configs: closest/any-hit rates of the plain, heightfield and instanced workloads; "
-                         "render: path tracing of a scene file with lrh_shade (samples per second)")
+                         "render: path tracing of a scene file with lrh_shade (samples per second); "
+                         "build: device BVH builds (LBVH, PLOC) of the headline scene and their trace rates")
     ap.add_argument("--sets", default="plain,heightfield,instanced",
                     help="--mode configs: which workloads (comma list of plain, heightfield, instanced)")
     ap.add_argument("--scene", default=None, help="scene file for --mode scene (.scn/.cfg)")
@@ -672,7 +730,7 @@ def main():
     dev = D.Device(0)
     log(f"device {dev.name}, {dev.cu_count} CUs")
     modes = {"batched": run_batched, "scene": run_scene, "refit": run_refit, "configs": run_configs,
-             "render": run_render}
+             "render": run_render, "build": run_build}
     if args.mode == "served":
         from luxcore_amd import serve
         modes["served"] = lambda a, d: serve.bench(a, d, log)

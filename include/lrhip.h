@@ -171,6 +171,9 @@ int lrh_host_free(lrh_device *dev, void *hptr);
 int lrh_stream_create(lrh_device *dev, void **stream);
 int lrh_stream_destroy(lrh_device *dev, void *stream);
 int lrh_sync(lrh_device *dev, void *stream);
+/* Device memory the library holds on this device: scenes, buffers, trace
+ * workspaces, and the scratch of device BVH builds, which is kept (at most
+ * 32 GiB) for the next build of the same sizes and released with the device. */
 size_t lrh_used_memory(lrh_device *dev);
 int lrh_event_create(lrh_device *dev, lrh_event **ev);
 int lrh_event_destroy(lrh_event *ev);

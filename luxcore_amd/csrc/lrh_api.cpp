@@ -157,6 +157,49 @@ struct lrh_device {
 		(void)hipFree(p);
 		used -= bytes;
 	}
+	// blocks a device build released, kept (up to kBuildPoolMax, counted in
+	// `used`) for the next build asking for the same sizes: a build's scratch
+	// runs to gigabytes, and releasing it and mapping it again on every build
+	// came to several times the build once the runtime's own cache was full
+	std::mutex poolMu;
+	std::multimap<size_t, void *> buildPool;
+	size_t pooledBytes = 0;
+	static constexpr size_t kBuildPoolMax = size_t(32) << 30;
+	void *buildAlloc(size_t bytes) {
+		{
+			std::lock_guard<std::mutex> g(poolMu);
+			auto it = buildPool.find(bytes);
+			if (it != buildPool.end()) {
+				void *p = it->second;
+				buildPool.erase(it);
+				pooledBytes -= bytes;
+				return p;
+			}
+		}
+		return devAlloc(bytes);
+	}
+	void buildRelease(void *p, size_t bytes) {
+		if (!p) return;
+		std::lock_guard<std::mutex> g(poolMu);
+		if (bytes > kBuildPoolMax) {
+			devFree(p, bytes);
+			return;
+		}
+		while (pooledBytes + bytes > kBuildPoolMax && !buildPool.empty()) {  // largest blocks go first
+			auto it = std::prev(buildPool.end());
+			devFree(it->second, it->first);
+			pooledBytes -= it->first;
+			buildPool.erase(it);
+		}
+		buildPool.emplace(bytes, p);
+		pooledBytes += bytes;
+	}
+	void drainPool() {
+		std::lock_guard<std::mutex> g(poolMu);
+		for (auto &b : buildPool) devFree(b.second, b.first);
+		buildPool.clear();
+		pooledBytes = 0;
+	}
 };
 
 struct DevBuf {
@@ -624,6 +667,7 @@ int lrh_device_close(lrh_device *dev) {
 		dev->devFree(w.second.table, w.second.tableBytes);
 	}
 	dev->devFree(dev->dError, sizeof(uint32_t));
+	dev->drainPool();
 	for (auto &a : dev->allocs) (void)hipFree((void *)a.first);
 	if (dev->stream) (void)hipStreamDestroy(dev->stream);
 	delete dev;
@@ -779,8 +823,8 @@ int lrh_event_elapsed(lrh_event *a, lrh_event *b, float *ms) {
 
 namespace {
 
-void *gbAlloc(void *ctx, size_t bytes) { return static_cast<lrh_device *>(ctx)->devAlloc(bytes); }
-void gbRelease(void *ctx, void *p, size_t bytes) { static_cast<lrh_device *>(ctx)->devFree(p, bytes); }
+void *gbAlloc(void *ctx, size_t bytes) { return static_cast<lrh_device *>(ctx)->buildAlloc(bytes); }
+void gbRelease(void *ctx, void *p, size_t bytes) { static_cast<lrh_device *>(ctx)->buildRelease(p, bytes); }
 
 // single-level scene built on the device (LRH_BUILD_DEVICE_LBVH / _PLOC):
 // vertices and triangles are uploaded mesh by mesh, the tree and the records

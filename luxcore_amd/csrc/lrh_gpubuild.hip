@@ -461,19 +461,39 @@ __global__ void k_records(const float *verts, const uint32_t *tris, const uint32
 	invPerm[i] = j;
 }
 
+// the build's scratch arrays, carved from chunks of at least 256 MiB (an
+// array above that is a chunk of its own) and released together at the end:
+// the clustering's rounds ask for hundreds of small scan arrays, and a
+// device allocation and release each would cost more than the kernels
 struct Scratch {
+	static constexpr size_t kChunk = size_t(256) << 20;
 	const LrhGpuAlloc &a;
 	std::vector<std::pair<void *, size_t>> held;
-	explicit Scratch(const LrhGpuAlloc &al) : a(al) {}
+	char *cur = nullptr;
+	size_t left = 0;
+	uint32_t *pin = nullptr;  // 1 KiB of pinned host memory for the readbacks (word 0: error, 1-3: counts, 128+: depths)
+	explicit Scratch(const LrhGpuAlloc &al) : a(al) {
+		GB_CHECK(hipHostMalloc(reinterpret_cast<void **>(&pin), 1024, hipHostMallocDefault));
+	}
 	template <class T>
 	T *get(size_t count) {
-		const size_t bytes = std::max<size_t>(count * sizeof(T), 16);
-		void *p = a.alloc(a.ctx, bytes);
-		held.push_back({p, bytes});
-		return static_cast<T *>(p);
+		const size_t bytes = (std::max<size_t>(count * sizeof(T), 16) + 255) & ~size_t(255);
+		if (bytes > left) {
+			const size_t chunk = std::max(bytes, kChunk);
+			void *p = a.alloc(a.ctx, chunk);
+			held.push_back({p, chunk});
+			if (bytes == chunk) return static_cast<T *>(p);  // a chunk of its own; the current one stays
+			cur = static_cast<char *>(p);
+			left = chunk;
+		}
+		T *r = reinterpret_cast<T *>(cur);
+		cur += bytes;
+		left -= bytes;
+		return r;
 	}
 	~Scratch() {
 		for (auto &h : held) a.release(a.ctx, h.first, h.second);
+		(void)hipHostFree(pin);
 	}
 };
 
@@ -494,11 +514,12 @@ void exclusiveScan(Scratch &S, const uint32_t *in, uint32_t *out, uint32_t n, hi
 	}
 }
 
-uint32_t readError(const uint32_t *err, hipStream_t st) {
-	uint32_t e = 0;
-	GB_CHECK(hipMemcpyAsync(&e, err, 4, hipMemcpyDeviceToHost, st));
+// waits for the stream (and the readbacks queued before) and returns the
+// error word
+uint32_t readError(Scratch &S, const uint32_t *err, hipStream_t st) {
+	GB_CHECK(hipMemcpyAsync(S.pin, err, 4, hipMemcpyDeviceToHost, st));
 	GB_CHECK(hipStreamSynchronize(st));
-	return e;
+	return S.pin[0];
 }
 
 std::string errorText(uint32_t e) {
@@ -534,9 +555,10 @@ void boxesByDepth(Scratch &S, const GBox *boxes, const uint32_t *vals, uint32_t 
 	hipLaunchKernelGGL(k_fill, dim3(1), dim3(kMaxDepth), 0, st, hist, kMaxDepth, 0u);
 	hipLaunchKernelGGL(k_depthCount, dim3(grid(ni, 256)), dim3(256), 0, st, ni, dA, jA, hist, err);
 	GB_CHECK(hipGetLastError());
+	GB_CHECK(hipMemcpyAsync(S.pin + 128, hist, kMaxDepth * 4, hipMemcpyDeviceToHost, st));
+	if (const uint32_t e = readError(S, err, st)) throw std::runtime_error(errorText(e));
 	uint32_t h[kMaxDepth];
-	GB_CHECK(hipMemcpyAsync(h, hist, sizeof(h), hipMemcpyDeviceToHost, st));
-	if (const uint32_t e = readError(err, st)) throw std::runtime_error(errorText(e));
+	std::copy(S.pin + 128, S.pin + 128 + kMaxDepth, h);
 	uint32_t off[kMaxDepth + 1];
 	off[0] = 0;
 	for (uint32_t d = 0; d < kMaxDepth; ++d) off[d + 1] = off[d] + h[d];
@@ -561,7 +583,7 @@ void boxesByDepth(Scratch &S, const GBox *boxes, const uint32_t *vals, uint32_t 
 	}
 	GB_CHECK(hipGetLastError());
 	GB_CHECK(hipGetLastError());
-	if (const uint32_t e = readError(err, st)) throw std::runtime_error(errorText(e));
+	if (const uint32_t e = readError(S, err, st)) throw std::runtime_error(errorText(e));
 }
 
 // the clustering's binary tree over the n >= 2 sorted leaves, written in the
@@ -582,7 +604,6 @@ void plocTree(Scratch &S, const GBox *boxes, uint32_t n, uint32_t maxLeaf, float
 	GBox *cboxA = S.get<GBox>(n), *cboxB = S.get<GBox>(n);
 	uint32_t *lead = S.get<uint32_t>((size_t)n + 1), *keep = S.get<uint32_t>((size_t)n + 1),
 			 *leadOff = S.get<uint32_t>((size_t)n + 1), *keepOff = S.get<uint32_t>((size_t)n + 1);
-	uint32_t back[2] = {0, 0};
 	hipLaunchKernelGGL(k_plocInit, dim3(grid(n, 256)), dim3(256), 0, st, boxes, vals, n, cidA, cboxA, leafBox);
 	GB_CHECK(hipGetLastError());
 	std::vector<uint32_t> roundStart{0};
@@ -604,10 +625,10 @@ void plocTree(Scratch &S, const GBox *boxes, uint32_t n, uint32_t maxLeaf, float
 		hipLaunchKernelGGL(k_plocApply, dim3(grid(nc, 256)), dim3(256), 0, st, nn, nc, forced ? 1u : 0u, cidA, cboxA,
 				leadOff, keepOff, nodeBase, cidB, cboxB, left, right, nodeBox);
 		GB_CHECK(hipGetLastError());
-		GB_CHECK(hipMemcpyAsync(back, leadOff + nc, 4, hipMemcpyDeviceToHost, st));
-		GB_CHECK(hipMemcpyAsync(back + 1, keepOff + nc, 4, hipMemcpyDeviceToHost, st));
+		GB_CHECK(hipMemcpyAsync(S.pin + 1, leadOff + nc, 4, hipMemcpyDeviceToHost, st));
+		GB_CHECK(hipMemcpyAsync(S.pin + 2, keepOff + nc, 4, hipMemcpyDeviceToHost, st));
 		GB_CHECK(hipStreamSynchronize(st));
-		const uint32_t merged = back[0], kept = back[1];
+		const uint32_t merged = S.pin[1], kept = S.pin[2];
 		if (merged == 0 || merged > nc / 2 || kept != nc - merged || nodeBase + merged > ni)
 			throw std::runtime_error("device build: clustering round without a consistent merge count");
 		nodeBase += merged;
@@ -714,7 +735,7 @@ void lrhGpuBuildBVH(const LrhGpuBuildIn &in, const LrhGpuAlloc &a, hipStream_t s
 			}
 			hipLaunchKernelGGL(k_checkSorted, dim3(grid(n, 256)), dim3(256), 0, st, keys, n, err);
 			GB_CHECK(hipGetLastError());
-			if (const uint32_t e = readError(err, st)) throw std::runtime_error(errorText(e));
+			if (const uint32_t e = readError(S, err, st)) throw std::runtime_error(errorText(e));
 
 			const uint32_t ni = n - 1;
 			uint32_t *left = S.get<uint32_t>(ni), *right = S.get<uint32_t>(ni), *first = S.get<uint32_t>(ni),
@@ -730,7 +751,7 @@ void lrhGpuBuildBVH(const LrhGpuBuildIn &in, const LrhGpuAlloc &a, hipStream_t s
 				hipLaunchKernelGGL(k_radixTree, dim3(grid(ni, 256)), dim3(256), 0, st, keys, n, left, right, first,
 						last, iparent, err);
 				GB_CHECK(hipGetLastError());
-				if (const uint32_t e = readError(err, st)) throw std::runtime_error(errorText(e));
+				if (const uint32_t e = readError(S, err, st)) throw std::runtime_error(errorText(e));
 				boxesByDepth(S, boxes, vals, n, left, right, first, last, iparent, in.maxLeaf, in.travCost, st, err,
 						leafBox, nodeBox, leafOK);
 			}
@@ -752,9 +773,9 @@ void lrhGpuBuildBVH(const LrhGpuBuildIn &in, const LrhGpuAlloc &a, hipStream_t s
 				hipLaunchKernelGGL(k_emit, dim3(grid(nf, 256)), dim3(256), 0, st, nf, kids, off, first, last, leafOK,
 						leafBox, nodeBox, levelBase, in.layout, nodes, fB);
 				GB_CHECK(hipGetLastError());
-				uint32_t nextN = 0;
-				GB_CHECK(hipMemcpyAsync(&nextN, off + nf, 4, hipMemcpyDeviceToHost, st));
-				if (const uint32_t e = readError(err, st)) throw std::runtime_error(errorText(e));
+				GB_CHECK(hipMemcpyAsync(S.pin + 3, off + nf, 4, hipMemcpyDeviceToHost, st));
+				if (const uint32_t e = readError(S, err, st)) throw std::runtime_error(errorText(e));
+				const uint32_t nextN = S.pin[3];
 				levelBase += nf;
 				nf = nextN;
 				std::swap(fA, fB);
