@@ -54,29 +54,47 @@ __global__ void k_initBounds(uint32_t *cb) {
 	}
 }
 
-// 1. leaf boxes and centre bounds
-__global__ void k_prims(const float *verts, const uint32_t *tris, const uint32_t *tbase, const uint32_t *vbase,
-		uint32_t nMeshes, uint32_t n, GBox *boxes, uint32_t *cb) {
-	__shared__ uint32_t smin[3], smax[3];
-	if (threadIdx.x < 3) {
-		smin[threadIdx.x] = 0xffffffffu;
-		smax[threadIdx.x] = 0u;
-	}
-	__syncthreads();
-	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n) {
+// 1. leaf boxes and centre bounds: a grid-stride loop, the bounds reduced
+// in registers, across the wave by lane exchanges and across the block in
+// LDS, then six global atomics per block
+constexpr uint32_t kPrimBlocks = 4096;
+__device__ __forceinline__ uint32_t minU(uint32_t a, uint32_t b) { return a < b ? a : b; }
+__device__ __forceinline__ uint32_t maxU(uint32_t a, uint32_t b) { return a > b ? a : b; }
+
+__global__ __launch_bounds__(256) void k_prims(const float *verts, const uint32_t *tris, const uint32_t *tbase,
+		const uint32_t *vbase, uint32_t nMeshes, uint32_t n, GBox *boxes, uint32_t *cb) {
+	uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
+	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
 		const GBox b = primBox(verts, tris, tbase, vbase, nMeshes, i);
 		boxes[i] = b;
 		for (int k = 0; k < 3; ++k) {
 			const uint32_t c = lrhdev::fkey(b.lo[k] + b.hi[k]);
-			atomicMin(&smin[k], c);
-			atomicMax(&smax[k], c);
+			mn[k] = minU(mn[k], c);
+			mx[k] = maxU(mx[k], c);
 		}
 	}
+	for (int k = 0; k < 3; ++k)
+		for (int o = 32; o > 0; o >>= 1) {
+			mn[k] = minU(mn[k], (uint32_t)__shfl_xor((int)mn[k], o));
+			mx[k] = maxU(mx[k], (uint32_t)__shfl_xor((int)mx[k], o));
+		}
+	__shared__ uint32_t smin[4][3], smax[4][3];
+	const uint32_t wave = threadIdx.x >> 6;
+	if ((threadIdx.x & 63u) == 0)
+		for (int k = 0; k < 3; ++k) {
+			smin[wave][k] = mn[k];
+			smax[wave][k] = mx[k];
+		}
 	__syncthreads();
 	if (threadIdx.x < 3) {
-		atomicMin(&cb[threadIdx.x], smin[threadIdx.x]);
-		atomicMax(&cb[3 + threadIdx.x], smax[threadIdx.x]);
+		const uint32_t k = threadIdx.x;
+		uint32_t a = smin[0][k], b = smax[0][k];
+		for (uint32_t w = 1; w < blockDim.x / 64; ++w) {
+			a = minU(a, smin[w][k]);
+			b = maxU(b, smax[w][k]);
+		}
+		atomicMin(&cb[k], a);
+		atomicMax(&cb[3 + k], b);
 	}
 }
 
@@ -238,24 +256,20 @@ __global__ void k_depthJump(uint32_t ni, const uint32_t *depth, const uint32_t *
 	jump2[x] = jump[j];
 }
 
-// nodes per depth (depth of a node not yet at the root: an error); the
-// deepest level is histogram slot kMaxDepth
+// a node not yet at the root after the doublings, or deeper than any valid
+// tree: an error
 constexpr uint32_t kMaxDepth = 128;
-__global__ void k_depthCount(uint32_t ni, const uint32_t *depth, const uint32_t *jump, uint32_t *hist,
-		uint32_t *err) {
-	__shared__ uint32_t h[kMaxDepth];
-	for (uint32_t k = threadIdx.x; k < kMaxDepth; k += blockDim.x) h[k] = 0;
-	__syncthreads();
+__global__ void k_depthCheck(uint32_t ni, const uint32_t *depth, const uint32_t *jump, uint32_t *err) {
 	const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
-	if (x < ni) {
-		if (jump[x] != 0 || depth[x] >= kMaxDepth)
-			atomicOr(err, kErrClimb);
-		else
-			atomicAdd(&h[depth[x]], 1u);
-	}
-	__syncthreads();
-	for (uint32_t k = threadIdx.x; k < kMaxDepth; k += blockDim.x)
-		if (h[k]) atomicAdd(&hist[k], h[k]);
+	if (x < ni && (jump[x] != 0 || depth[x] >= kMaxDepth)) atomicOr(err, kErrClimb);
+}
+
+// the first place of each depth (digit) in the sorted order: the scanned
+// digit table's entry of the first tile; out[kMaxDepth] counts the nodes
+// of depth below kMaxDepth
+__global__ void k_digitStarts(const uint32_t *table, uint32_t tiles, uint32_t *out) {
+	const uint32_t d = threadIdx.x;
+	if (d <= kMaxDepth) out[d] = table[(size_t)d * tiles];
 }
 
 // sort keys (depths) and values (node ids) for the counting sort by depth
@@ -471,9 +485,9 @@ struct Scratch {
 	std::vector<std::pair<void *, size_t>> held;
 	char *cur = nullptr;
 	size_t left = 0;
-	uint32_t *pin = nullptr;  // 1 KiB of pinned host memory for the readbacks (word 0: error, 1-3: counts, 128+: depths)
+	uint32_t *pin = nullptr;  // 2 KiB of pinned host memory for the readbacks (word 0: error, 1-3: counts, 128+: depths)
 	explicit Scratch(const LrhGpuAlloc &al) : a(al) {
-		GB_CHECK(hipHostMalloc(reinterpret_cast<void **>(&pin), 1024, hipHostMallocDefault));
+		GB_CHECK(hipHostMalloc(reinterpret_cast<void **>(&pin), 2048, hipHostMallocDefault));
 	}
 	template <class T>
 	T *get(size_t count) {
@@ -542,7 +556,7 @@ void boxesByDepth(Scratch &S, const GBox *boxes, const uint32_t *vals, uint32_t 
 		float ci, hipStream_t st, uint32_t *err, GBox *leafBox, GBox *nodeBox, uint32_t *leafOK) {
 	const uint32_t ni = n - 1;
 	uint32_t *dA = S.get<uint32_t>(ni), *dB = S.get<uint32_t>(ni), *jA = S.get<uint32_t>(ni),
-			 *jB = S.get<uint32_t>(ni), *order = S.get<uint32_t>(ni), *hist = S.get<uint32_t>(kMaxDepth);
+			 *jB = S.get<uint32_t>(ni), *order = S.get<uint32_t>(ni), *starts = S.get<uint32_t>(kMaxDepth + 1);
 	uint64_t *dKeys = S.get<uint64_t>(ni), *dKeysOut = S.get<uint64_t>(ni);
 	float *cost = S.get<float>(ni);
 	hipLaunchKernelGGL(k_depthInit, dim3(grid(ni, 256)), dim3(256), 0, st, iparent, ni, dA, jA);
@@ -552,19 +566,12 @@ void boxesByDepth(Scratch &S, const GBox *boxes, const uint32_t *vals, uint32_t 
 		std::swap(dA, dB);
 		std::swap(jA, jB);
 	}
-	hipLaunchKernelGGL(k_fill, dim3(1), dim3(kMaxDepth), 0, st, hist, kMaxDepth, 0u);
-	hipLaunchKernelGGL(k_depthCount, dim3(grid(ni, 256)), dim3(256), 0, st, ni, dA, jA, hist, err);
+	hipLaunchKernelGGL(k_depthCheck, dim3(grid(ni, 256)), dim3(256), 0, st, ni, dA, jA, err);
 	GB_CHECK(hipGetLastError());
-	GB_CHECK(hipMemcpyAsync(S.pin + 128, hist, kMaxDepth * 4, hipMemcpyDeviceToHost, st));
 	if (const uint32_t e = readError(S, err, st)) throw std::runtime_error(errorText(e));
-	uint32_t h[kMaxDepth];
-	std::copy(S.pin + 128, S.pin + 128 + kMaxDepth, h);
-	uint32_t off[kMaxDepth + 1];
-	off[0] = 0;
-	for (uint32_t d = 0; d < kMaxDepth; ++d) off[d + 1] = off[d] + h[d];
-	if (off[kMaxDepth] != ni) throw std::runtime_error("device build: radix tree nodes not all below the root");
 	// nodes in depth order: one stable pass of the radix sort (depth < 2^8
-	// is one digit) with the node ids as values, so no counter is contended
+	// is one digit) with the node ids as values, so no counter is contended;
+	// the scanned digit table gives where each depth starts
 	const uint32_t tiles = grid(ni, kSortTile);
 	uint32_t *table = S.get<uint32_t>((size_t)256 * tiles), *ids = S.get<uint32_t>(ni);
 	hipLaunchKernelGGL(k_depthKeys, dim3(grid(ni, 256)), dim3(256), 0, st, ni, dA, dKeys, ids);
@@ -573,6 +580,18 @@ void boxesByDepth(Scratch &S, const GBox *boxes, const uint32_t *vals, uint32_t 
 	exclusiveScan(S, table, table, 256u * tiles, st);
 	hipLaunchKernelGGL(k_sortScatter, dim3(tiles), dim3(kSortBlock), 0, st, dKeys, ids, ni, 0u, tiles, table, dKeysOut,
 			order, err);
+	hipLaunchKernelGGL(k_digitStarts, dim3(1), dim3(256), 0, st, table, tiles, starts);
+	GB_CHECK(hipGetLastError());
+	GB_CHECK(hipMemcpyAsync(S.pin + 128, starts, (kMaxDepth + 1) * 4, hipMemcpyDeviceToHost, st));
+	if (const uint32_t e = readError(S, err, st)) throw std::runtime_error(errorText(e));
+	uint32_t off[kMaxDepth + 1], h[kMaxDepth];
+	std::copy(S.pin + 128, S.pin + 128 + kMaxDepth + 1, off);
+	for (uint32_t d = 0; d < kMaxDepth; ++d) {
+		if (off[d + 1] < off[d]) throw std::runtime_error("device build: depth table out of order");
+		h[d] = off[d + 1] - off[d];
+	}
+	if (off[0] != 0 || off[kMaxDepth] != ni)
+		throw std::runtime_error("device build: radix tree nodes not all below the root");
 	hipLaunchKernelGGL(k_leafBoxes, dim3(grid(n, 256)), dim3(256), 0, st, boxes, vals, n, leafBox);
 	GB_CHECK(hipGetLastError());
 	for (uint32_t d = kMaxDepth; d-- > 0;) {
@@ -581,7 +600,6 @@ void boxesByDepth(Scratch &S, const GBox *boxes, const uint32_t *vals, uint32_t 
 		hipLaunchKernelGGL(k_boxLevel, dim3(grid(cnt, 256)), dim3(256), 0, st, order + off[d], cnt, left, right, first,
 				last, leafBox, maxLeaf, ci, nodeBox, cost, leafOK);
 	}
-	GB_CHECK(hipGetLastError());
 	GB_CHECK(hipGetLastError());
 	if (const uint32_t e = readError(S, err, st)) throw std::runtime_error(errorText(e));
 }
@@ -696,7 +714,7 @@ void lrhGpuBuildBVH(const LrhGpuBuildIn &in, const LrhGpuAlloc &a, hipStream_t s
 	GBox *boxes = S.get<GBox>(n);
 	uint32_t *cb = S.get<uint32_t>(6);
 	hipLaunchKernelGGL(k_initBounds, dim3(1), dim3(64), 0, st, cb);
-	hipLaunchKernelGGL(k_prims, dim3(grid(n, 256)), dim3(256), 0, st, in.verts, in.tris, tbase, vbase, in.nMeshes, n,
+	hipLaunchKernelGGL(k_prims, dim3(std::min<unsigned>(grid(n, 256), kPrimBlocks)), dim3(256), 0, st, in.verts, in.tris, tbase, vbase, in.nMeshes, n,
 			boxes, cb);
 	GB_CHECK(hipGetLastError());
 

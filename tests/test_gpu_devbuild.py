@@ -85,13 +85,18 @@ def emu(tmp_path_factory):
 
 
 @pytest.mark.parametrize("builder", list(BUILDERS))
-@pytest.mark.parametrize("layout,max_leaf", [(0, 4), (1, 2)])
-def test_device_build_equals_cpu_emulation(gpu_device, emu, tmp_path, layout, max_leaf, builder):
+@pytest.mark.parametrize("scene,layout,max_leaf", [("plain", 0, 4), ("plain", 1, 2), ("heightfield", 0, 4)])
+def test_device_build_equals_cpu_emulation(gpu_device, emu, tmp_path, scene, layout, max_leaf, builder):
     """The kernels' tree is the sequential emulation's, node and record bytes
-    alike: the per-element steps are shared and every step is deterministic."""
+    alike: the per-element steps are shared and every step is deterministic
+    (the heightfield's centre keys span the whole unsigned range, so a bound
+    reduced at less than 32 bits would show)."""
     from tests.test_lbvh_cpu import run_emu
-    _case("devbuild_emu", layout=layout, max_leaf=max_leaf, builder=builder)
-    meshes = scenes.plain_scene(seed=1, n_random=20000)
+    _case("devbuild_emu", scene=scene, layout=layout, max_leaf=max_leaf, builder=builder)
+    if scene == "plain":
+        meshes = scenes.plain_scene(seed=1, n_random=20000)
+    else:
+        meshes = scenes.heightfield_scene(cells=256, n_spheres=4, seed=3)
     info, nodes, tris = run_emu(emu, tmp_path, meshes, max_leaf, layout, method=builder)
     assert info["err"] == 0
     dsc = gpu_device.scene_bvh(meshes, layout=layout, max_leaf=max_leaf, builder=BUILDERS[builder])
