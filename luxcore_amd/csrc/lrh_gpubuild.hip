@@ -315,9 +315,24 @@ __global__ void k_plocInit(const GBox *boxes, const uint32_t *vals, uint32_t n, 
 	cbox[i] = b;
 }
 
-__global__ void k_plocNearest(const GBox *cbox, uint32_t nc, uint32_t *nn) {
-	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < nc) nn[i] = plocNearest(cbox, nc, i, kPlocRadius);
+// nearest neighbours of a tile of 256 clusters, their boxes and the
+// kPlocRadius on either side staged in LDS (each box is read by 2r+1 threads)
+constexpr uint32_t kPlocTile = 256;
+__global__ __launch_bounds__(kPlocTile) void k_plocNearest(const GBox *cbox, uint32_t nc, uint32_t *nn) {
+	constexpr uint32_t kSpan = kPlocTile + 2 * kPlocRadius;
+	__shared__ float tile[kSpan * 6];
+	const uint32_t t0 = blockIdx.x * kPlocTile;
+	const int64_t base = (int64_t)t0 - kPlocRadius;  // cluster of tile entry 0
+	const float *src = reinterpret_cast<const float *>(cbox);
+	for (uint32_t k = threadIdx.x; k < kSpan * 6; k += kPlocTile) {
+		const int64_t j = base + k / 6;
+		tile[k] = (j >= 0 && j < (int64_t)nc) ? src[j * 6 + k % 6] : 0.f;
+	}
+	__syncthreads();
+	const uint32_t i = t0 + threadIdx.x;
+	if (i >= nc) return;
+	const GBox *lds = reinterpret_cast<const GBox *>(tile);
+	nn[i] = plocNearestAt([&](uint32_t j) -> const GBox & { return lds[(int64_t)j - base]; }, nc, i, kPlocRadius);
 }
 
 // lead: the lower place of a merging pair (a new node); keep: the place
@@ -634,7 +649,7 @@ void plocTree(Scratch &S, const GBox *boxes, uint32_t n, uint32_t maxLeaf, float
 	while (nc > 1) {
 		if (roundStart.size() > kMaxRounds) throw std::runtime_error("device build: clustering did not converge");
 		if (!forced)
-			hipLaunchKernelGGL(k_plocNearest, dim3(grid(nc, 256)), dim3(256), 0, st, cboxA, nc, nn);
+			hipLaunchKernelGGL(k_plocNearest, dim3(grid(nc, kPlocTile)), dim3(kPlocTile), 0, st, cboxA, nc, nn);
 		hipLaunchKernelGGL(k_plocFlags, dim3(grid((size_t)nc + 1, 256)), dim3(256), 0, st, nn, nc, forced ? 1u : 0u,
 				lead, keep);
 		GB_CHECK(hipGetLastError());

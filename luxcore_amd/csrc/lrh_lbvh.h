@@ -317,14 +317,18 @@ LRH_DN inline float mergedArea(const GBox &a, const GBox &b) {
 // nearest neighbour of cluster i among places [i - r, i + r]: the smallest
 // merged area, equal areas by the pair's (lower, upper) place, so the pair
 // with the smallest such key of all is mutual and every round merges
-LRH_DN inline uint32_t plocNearest(const GBox *c, uint32_t nc, uint32_t i, uint32_t r) {
+// (box(j) returns cluster j's box: the array itself, or a tile of it staged
+// in LDS; the arithmetic is the same either way)
+template <class BoxAt>
+LRH_DN inline uint32_t plocNearestAt(const BoxAt &box, uint32_t nc, uint32_t i, uint32_t r) {
 	const uint32_t lo = i > r ? i - r : 0u;
 	const uint32_t hi = (nc - 1u - i > r) ? i + r : nc - 1u;
 	float best = INFINITY;
 	uint32_t arg = i;
+	const GBox ci = box(i);
 	for (uint32_t j = lo; j <= hi; ++j) {
 		if (j == i) continue;
-		const float a = mergedArea(c[i], c[j]);
+		const float a = mergedArea(ci, box(j));
 		bool better = arg == i || a < best;
 		if (arg != i && a == best) {
 			const uint32_t p0 = i < j ? i : j, q0 = i < j ? j : i;
@@ -337,6 +341,10 @@ LRH_DN inline uint32_t plocNearest(const GBox *c, uint32_t nc, uint32_t i, uint3
 		}
 	}
 	return arg;
+}
+
+LRH_DN inline uint32_t plocNearest(const GBox *c, uint32_t nc, uint32_t i, uint32_t r) {
+	return plocNearestAt([c](uint32_t j) -> const GBox & { return c[j]; }, nc, i, r);
 }
 
 // cluster i's partner this round (kNone: none): its nearest neighbour when
