@@ -699,7 +699,10 @@ def main():
                          "taken over them; 20 per set in the other modes)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cells", type=int, default=3240)
-    ap.add_argument("--layout", type=int, default=0)
+    ap.add_argument("--layout", type=int, default=None, choices=(0, 1, 2),
+                    help="node layout: 0 4-wide 64B quantized, 1 4-wide 128B fp32, 2 8-wide 128B quantized "
+                         "(default: 2 for the headline over the host SAH tree, else 0; the device builders and "
+                         "the batched, two-level and served paths take 0 or 1)")
     ap.add_argument("--max-leaf", type=int, default=4)
     ap.add_argument("--builder", choices=("sah", "lbvh", "ploc"), default="sah",
                     help="headline tree: host binned SAH (default), or a device build: LBVH (radix tree) or "
@@ -713,8 +716,9 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="headline only: skip the two-level, batched and served workloads of the JSON line")
     ap.add_argument("--extra-steps", type=int, default=20, help="timed traces per extra workload")
-    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_heightfield_bounce.json"),
-                    help="rocprofv3 FETCH_SIZE/WRITE_SIZE measurement of this workload (bytes per ray)")
+    ap.add_argument("--traffic-file", default=None,
+                    help="rocprofv3 FETCH_SIZE/WRITE_SIZE measurement of this workload (bytes per ray); default: "
+                         "the profiles/ file of the node layout")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="after the timed region, copy this many MiB device-to-device (a kernel of known "
                          "traffic for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -727,6 +731,8 @@ def main():
 
     if args.steps is None:
         args.steps = 1000 if args.mode == "headline" else 20
+    if args.layout is None:
+        args.layout = 2 if args.mode == "headline" and args.builder == "sah" else 0
     dev = D.Device(0)
     log(f"device {dev.name}, {dev.cu_count} CUs")
     modes = {"batched": run_batched, "scene": run_scene, "refit": run_refit, "configs": run_configs,
@@ -748,7 +754,9 @@ def main():
 
     # counting pass: node and triangle-record fetches per ray of this traversal
     nodes, prims, counted = dev.trace_stats(scene, rb, hb, n, 0, use_opts)
-    node_bytes = 64 if args.layout == 0 else 128
+    # bytes a node visit reads: 64-byte 4-wide record, 128-byte fp32 record,
+    # the 96 bytes of a 128-byte 8-wide record that hold its data
+    node_bytes = {0: 64, 1: 128, 2: 96}[args.layout]
     b_ray = RAY_BYTES + HIT_BYTES + node_bytes * nodes / counted + 48 * prims / counted
     log(f"per ray: {nodes / counted:.2f} node visits, {prims / counted:.2f} triangle tests, "
         f"B_ray = {b_ray:.1f} bytes")
@@ -788,10 +796,16 @@ def main():
         log(f"cpu baseline {cpu['value']:.3f} Mrays/s on {cpu['cores']} threads")
 
     traffic, traffic_source = None, None
-    if (args.traffic_file and os.path.exists(args.traffic_file) and args.cells == 3240 and args.layout == 0
-            and args.builder == "sah"):
+    if args.traffic_file is None:
+        args.traffic_file = os.path.join(ROOT, "profiles", {0: "traffic_heightfield_bounce.json",
+                                                            2: "traffic_heightfield_bounce_wide8.json"}
+                                         .get(args.layout, "none.json"))
+    tr = None
+    if args.traffic_file and os.path.exists(args.traffic_file):
         with open(args.traffic_file) as f:
             tr = json.load(f)
+    if (tr is not None and args.cells == 3240 and tr.get("layout", 0) == args.layout and args.builder == "sah"
+            and args.max_leaf == 4):
         traffic = round(tr["traffic_bytes_per_ray"] * n)
         traffic_source = (f"not measured in this run: {os.path.relpath(args.traffic_file, ROOT)} "
                           f"(rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel on this workload, scaled by a copy "
@@ -816,8 +830,8 @@ def main():
                                "bounce per hit (incoherent closest-hit)",
                    "triangles": ntris, "rays_per_step": int(n),
                    "builder": {"sah": "host binned SAH", "lbvh": "device LBVH", "ploc": "device PLOC"}[args.builder],
-                   "build_s": round(info["buildSeconds"], 3), "node_layout": "bvh4 64B quantized"
-                   if args.layout == 0 else "bvh4 128B fp32", "max_leaf": args.max_leaf,
+                   "build_s": round(info["buildSeconds"], 3), "node_layout": {0: "bvh4 64B quantized", 1: "bvh4 128B fp32",
+                                                    2: "bvh8 128B quantized"}[args.layout], "max_leaf": args.max_leaf,
                    "parallelism": "single GPU"},
         "latency_ms": {"p50": round(float(np.percentile(lat_p, 50)), 4),
                        "p99": round(float(np.percentile(lat_p, 99)), 4), "samples": int(lat_p.size),

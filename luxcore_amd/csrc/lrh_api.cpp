@@ -289,7 +289,7 @@ void checkMesh(const lrh_mesh &m, unsigned i) {
 lrh::BuildParams toParams(const lrh_build_opts *o) {
 	lrh::BuildParams p;
 	if (o) {
-		if (o->layout > 1) throw Error(LRH_EINVAL, "layout must be 0 or 1");
+		if (o->layout > 2) throw Error(LRH_EINVAL, "layout must be 0, 1 or 2");
 		p.layout = o->layout;
 		if (o->maxLeafSize) p.maxLeaf = o->maxLeafSize;
 		if (o->nThreads) p.nThreads = o->nThreads;
@@ -318,15 +318,24 @@ void parallelRange(size_t n, unsigned nt, F fn) {
 struct HostTree {
 	std::vector<lrh_node4q> q;
 	std::vector<lrh_node4f> f;
-	size_t count(unsigned layout) const { return layout ? f.size() : q.size(); }
+	std::vector<lrh_node8q> q8;
+	size_t count(unsigned layout) const { return layout == 2 ? q8.size() : layout ? f.size() : q.size(); }
 	void append(const lrh::WideBVH &w, unsigned layout) {
-		if (layout) f.insert(f.end(), w.fnodes.begin(), w.fnodes.end());
+		if (layout == 2) q8.insert(q8.end(), w.q8nodes.begin(), w.q8nodes.end());
+		else if (layout) f.insert(f.end(), w.fnodes.begin(), w.fnodes.end());
 		else q.insert(q.end(), w.qnodes.begin(), w.qnodes.end());
 	}
-	const void *data(unsigned layout) const { return layout ? (const void *)f.data() : (const void *)q.data(); }
-	size_t bytes(unsigned layout) const { return layout ? f.size() * sizeof(lrh_node4f) : q.size() * sizeof(lrh_node4q); }
+	const void *data(unsigned layout) const {
+		return layout == 2 ? (const void *)q8.data() : layout ? (const void *)f.data() : (const void *)q.data();
+	}
+	size_t bytes(unsigned layout) const {
+		return layout == 2 ? q8.size() * sizeof(lrh_node8q)
+				: layout ? f.size() * sizeof(lrh_node4f) : q.size() * sizeof(lrh_node4q);
+	}
 	void reserve(size_t n, unsigned layout) {
-		if (layout) f.resize(f.size() + n); else q.resize(q.size() + n);
+		if (layout == 2) q8.resize(q8.size() + n);
+		else if (layout) f.resize(f.size() + n);
+		else q.resize(q.size() + n);
 	}
 };
 
@@ -913,6 +922,7 @@ int lrh_scene_create_bvh(lrh_device *dev, const lrh_mesh *meshes, unsigned int n
 					"builder must be LRH_BUILD_HOST_SAH, LRH_BUILD_DEVICE_LBVH or LRH_BUILD_DEVICE_PLOC");
 		if (builder != LRH_BUILD_HOST_SAH) {
 			if (!dev) throw Error(LRH_EINVAL, "the device builder needs a device");
+			if (s->layout == 2) throw Error(LRH_EINVAL, "the device builders write layouts 0 and 1");
 			deviceBuild(s.get(), meshes, nMeshes, builder);
 			return;
 		}
@@ -963,6 +973,7 @@ int lrh_scene_create_mbvh(lrh_device *dev, const lrh_mesh *meshes, unsigned int 
 		s->dev = dev;
 		s->params = toParams(opts);
 		s->layout = s->params.layout;
+		if (s->layout == 2) throw Error(LRH_EINVAL, "two-level scenes use layouts 0 and 1");
 		s->twoLevel = true;
 		const double t0 = now();
 		HostTree ht;
@@ -1270,7 +1281,8 @@ LaunchShape chooseShape(lrh_device *dev, unsigned layout, bool two, bool any, bo
 	const unsigned persistent = (unsigned)perCU * (unsigned)dev->cuCount;
 	const unsigned need = (unsigned)(((uint64_t)count + L.block - 1) / L.block);
 	L.blocks = std::max(1u, std::min(persistent, need));
-	const unsigned worst = 3u * (maxDepth + 2u) + 4u;
+	// a visit pushes at most width - 1 entries
+	const unsigned worst = (layout == 2 ? 7u : 3u) * (maxDepth + 2u) + 4u;
 	L.spillDepth = worst > L.lds ? worst - L.lds : 1u;
 	return L;
 }
@@ -1401,6 +1413,7 @@ BatchTable makeTable(lrh_device *dev, const lrh_batch_item *items, unsigned n) {
 		if (((uintptr_t)it.rays & 15u) || ((uintptr_t)it.hits & 3u))
 			throw Error(LRH_EINVAL, "batch item " + std::to_string(i) + ": misaligned buffer");
 		if (it.query > 1) throw Error(LRH_EINVAL, "batch item " + std::to_string(i) + ": bad query");
+		if (s->layout == 2) throw Error(LRH_EINVAL, "8-wide nodes (layout 2) are traced by lrh_trace only");
 		if (j == 0) T.layout = s->layout;
 		else if (s->layout != T.layout) throw Error(LRH_EINVAL, "all scenes of a batch must share the node layout");
 		T.two = T.two || s->twoLevel;

@@ -622,10 +622,11 @@ void splitLargeTriangles(const std::vector<Box> &boxes, const TriangleSource &tr
 // ---------------------------------------------------------------------------
 namespace {
 
+constexpr int kMaxWidth = 8;
 struct WNode {
 	uint32_t n;
-	uint32_t child[4];
-	Box cbox[4];
+	uint32_t child[kMaxWidth];
+	Box cbox[kMaxWidth];
 };
 
 class WideBuilder {
@@ -762,10 +763,11 @@ private:
 		uint32_t md = maxDepth.load();
 		while (depth > md && !maxDepth.compare_exchange_weak(md, depth)) {}
 		const bool median = depth > 40;
-		Range ch[4];
+		const int width = P.layout == 2 ? 8 : 4;
+		Range ch[kMaxWidth];
 		int nch = 1;
 		ch[0] = Range{b, e, box, false};
-		while (nch < 4) {
+		while (nch < width) {
 			int pick = -1;
 			float best = -1.f;
 			for (int i = 0; i < nch; ++i) {
@@ -870,6 +872,47 @@ void quantize(const WNode &w, lrh_node4q *q) {
 	for (uint32_t c = 0; c < 4; ++c) q->child[c] = c < w.n ? w.child[c] : LRH_CHILD_EMPTY;
 }
 
+// the 8-wide node: the same outward rounding as quantize, eight slots
+void quantize8(const WNode &w, lrh_node8q *q) {
+	std::memset(q, 0, sizeof(*q));
+	Box pb = Box::Empty();
+	for (uint32_t c = 0; c < w.n; ++c) pb.grow(w.cbox[c]);
+	uint8_t *qlo[3] = {q->qlo_x, q->qlo_y, q->qlo_z};
+	uint8_t *qhi[3] = {q->qhi_x, q->qhi_y, q->qhi_z};
+	uint8_t *ex[3] = {&q->ex, &q->ey, &q->ez};
+	for (int k = 0; k < 3; ++k) {
+		const float o = pb.lo[k];
+		q->org[k] = o;
+		const float ext = pb.hi[k] - o;
+		int e2 = 0;
+		if (ext > 0.f)
+			frexpf(ext / 255.f, &e2);
+		else
+			e2 = -126;
+		int e = e2 + 127;
+		if (e < 1) e = 1;
+		while (e < 254 && decodeQ(o, 255, expScale((uint32_t)e)) < pb.hi[k]) ++e;
+		const float s = expScale((uint32_t)e);
+		*ex[k] = (uint8_t)e;
+		for (uint32_t c = 0; c < 8; ++c) {
+			if (c >= w.n) { qlo[k][c] = 255; qhi[k][c] = 0; continue; }
+			const float lo = w.cbox[c].lo[k], hi = w.cbox[c].hi[k];
+			long ql = (long)floorf((lo - o) / s);
+			if (ql < 0) ql = 0;
+			if (ql > 255) ql = 255;
+			while (ql > 0 && decodeQ(o, (uint32_t)ql, s) > lo) --ql;
+			long qh = (long)ceilf((hi - o) / s);
+			if (qh < 0) qh = 0;
+			if (qh > 255) qh = 255;
+			while (qh < 255 && decodeQ(o, (uint32_t)qh, s) < hi) ++qh;
+			qlo[k][c] = (uint8_t)ql;
+			qhi[k][c] = (uint8_t)qh;
+		}
+	}
+	q->nchild = (uint8_t)w.n;
+	for (uint32_t c = 0; c < 8; ++c) q->child[c] = c < w.n ? w.child[c] : LRH_CHILD_EMPTY;
+}
+
 void toFloatNode(const WNode &w, lrh_node4f *f) {
 	std::memset(f, 0, sizeof(*f));
 	for (uint32_t c = 0; c < 4; ++c) {
@@ -946,9 +989,13 @@ void buildWideBVH(const std::vector<Box> &boxes, const BuildParams &p, uint32_t 
 		for (uint32_t c = 0; c < wb.nodes[i].n; ++c) wb.nodes[i].child[c] = reloc(wb.nodes[i].child[c]);
 	out->qnodes.clear();
 	out->fnodes.clear();
+	out->q8nodes.clear();
 	if (p.layout == 0) {
 		out->qnodes.resize(nNodes);
 		parallelFor(nNodes, nt, [&](size_t i) { quantize(wb.nodes[oldOf[i]], &out->qnodes[i]); });
+	} else if (p.layout == 2) {
+		out->q8nodes.resize(nNodes);
+		parallelFor(nNodes, nt, [&](size_t i) { quantize8(wb.nodes[oldOf[i]], &out->q8nodes[i]); });
 	} else {
 		out->fnodes.resize(nNodes);
 		parallelFor(nNodes, nt, [&](size_t i) { toFloatNode(wb.nodes[oldOf[i]], &out->fnodes[i]); });

@@ -59,7 +59,7 @@ struct BuildParams {
 	unsigned maxLeaf = 4;
 	unsigned nThreads = 0;
 	float travCost = 1.0f;
-	unsigned layout = 0;  // 0 quantized 64 B, 1 fp32 128 B
+	unsigned layout = 0;  // 0 quantized 64 B, 1 fp32 128 B, 2 quantized 8-wide 128 B
 	float splitBudget = 0.f;  // extra triangle references of early split clipping, per triangle; 0 = none
 };
 
@@ -81,10 +81,11 @@ void splitLargeTriangles(const std::vector<Box> &boxes, const TriangleSource &tr
 struct WideBVH {
 	std::vector<lrh_node4q> qnodes;
 	std::vector<lrh_node4f> fnodes;
+	std::vector<lrh_node8q> q8nodes;  // layout 2: 8-wide
 	std::vector<uint32_t> order;  // primitive order of the leaf ranges
 	uint32_t root = LRH_CHILD_EMPTY;
 	uint32_t maxDepth = 0;
-	size_t nodeCount() const { return qnodes.size() + fnodes.size(); }
+	size_t nodeCount() const { return qnodes.size() + fnodes.size() + q8nodes.size(); }
 };
 
 // Builds a 4-wide tree over primitive boxes. Node indices start at nodeBase,

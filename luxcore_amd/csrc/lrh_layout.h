@@ -54,6 +54,22 @@ typedef struct __attribute__((aligned(64))) {
 	uint32_t pad[2];     /* 56 */
 } lrh_node4q;
 
+/* 128-byte 8-wide node with 8-bit child bounds, one L2 line: the same
+ * quantization as lrh_node4q for eight children (single-level scenes). */
+typedef struct __attribute__((aligned(128))) {
+	float org[3];        /*  0 */
+	uint8_t ex, ey, ez;  /* 12 */
+	uint8_t nchild;      /* 15 */
+	uint8_t qlo_x[8];    /* 16 */
+	uint8_t qhi_x[8];    /* 24 */
+	uint8_t qlo_y[8];    /* 32 */
+	uint8_t qhi_y[8];    /* 40 */
+	uint8_t qlo_z[8];    /* 48 */
+	uint8_t qhi_z[8];    /* 56 */
+	uint32_t child[8];   /* 64 */
+	uint32_t pad[8];     /* 96 */
+} lrh_node8q;
+
 /* 128-byte 4-wide node with fp32 child bounds (autotuner alternative). */
 typedef struct __attribute__((aligned(64))) {
 	float lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4]; /* 0..95 */

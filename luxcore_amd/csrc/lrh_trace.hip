@@ -148,6 +148,35 @@ DEVI void testNodeQ(const lrh_node4q *__restrict__ nodes, uint32_t idx, const Lo
 	}
 }
 
+// 8-wide quantized node: six 16-byte loads (origin and scales, the three
+// axes' bytes, eight child references)
+DEVI void testNode8Q(const lrh_node8q *__restrict__ nodes, uint32_t idx, const LocalRay &r, float mint, float best,
+		float t[8], uint32_t c[8]) {
+	const uint4 *p = reinterpret_cast<const uint4 *>(nodes + idx);
+	const uint4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3], q4 = p[4], q5 = p[5];
+	const float sx = asf((q0.w & 0xffu) << 23), sy = asf(((q0.w >> 8) & 0xffu) << 23),
+			sz = asf(((q0.w >> 16) & 0xffu) << 23);
+	const float Ax = sx * r.ix, Ay = sy * r.iy, Az = sz * r.iz;
+	const float Bx = fmaf(asf(q0.x), r.ix, r.nx), By = fmaf(asf(q0.y), r.iy, r.ny), Bz = fmaf(asf(q0.z), r.iz, r.nz);
+	const bool px = r.ix >= 0.f, py = r.iy >= 0.f, pz = r.iz >= 0.f;
+	const uint32_t nX[2] = {px ? q1.x : q1.z, px ? q1.y : q1.w}, fX[2] = {px ? q1.z : q1.x, px ? q1.w : q1.y};
+	const uint32_t nY[2] = {py ? q2.x : q2.z, py ? q2.y : q2.w}, fY[2] = {py ? q2.z : q2.x, py ? q2.w : q2.y};
+	const uint32_t nZ[2] = {pz ? q3.x : q3.z, pz ? q3.y : q3.w}, fZ[2] = {pz ? q3.z : q3.x, pz ? q3.w : q3.y};
+	c[0] = q4.x; c[1] = q4.y; c[2] = q4.z; c[3] = q4.w;
+	c[4] = q5.x; c[5] = q5.y; c[6] = q5.z; c[7] = q5.w;
+#pragma unroll
+	for (int k = 0; k < 8; ++k) {
+		const int w = k >> 2, b = k & 3;
+		const float tnx = fmaf(byteF(nX[w], b), Ax, Bx), tfx = fmaf(byteF(fX[w], b), Ax, Bx);
+		const float tny = fmaf(byteF(nY[w], b), Ay, By), tfy = fmaf(byteF(fY[w], b), Ay, By);
+		const float tnz = fmaf(byteF(nZ[w], b), Az, Bz), tfz = fmaf(byteF(fZ[w], b), Az, Bz);
+		const float tn = widenLo(fmaxf(fmaxf(tnx, tny), tnz));
+		const float tf = widenHi(fminf(fminf(tfx, tfy), tfz));
+		const float a = fmaxf(tn, mint), bb = fminf(tf, best);
+		t[k] = (a <= bb) ? a : INFINITY;
+	}
+}
+
 DEVI void testNodeF(const lrh_node4f *__restrict__ nodes, uint32_t idx, const LocalRay &r, float mint, float best,
 		float t[4], uint32_t c[4]) {
 	const float4 *p = reinterpret_cast<const float4 *>(nodes + idx);
@@ -411,12 +440,42 @@ __device__ __forceinline__ void traceBody(const LrhTraceParams &P) {
 					done = true;
 					break;
 				}
-				float t4[4];
-				uint32_t c4[4];
 				if (COUNT) {
 					++cNode;
 					tested = true;
 				}
+				if (LAYOUT == 2) {
+					float t8[8];
+					uint32_t c8[8];
+					testNode8Q(reinterpret_cast<const lrh_node8q *>(v.nodes), cur, r, mint, best, t8, c8);
+					// 19-comparator sorting network for eight entries
+					cswap(t8[0], c8[0], t8[2], c8[2]); cswap(t8[1], c8[1], t8[3], c8[3]);
+					cswap(t8[4], c8[4], t8[6], c8[6]); cswap(t8[5], c8[5], t8[7], c8[7]);
+					cswap(t8[0], c8[0], t8[4], c8[4]); cswap(t8[1], c8[1], t8[5], c8[5]);
+					cswap(t8[2], c8[2], t8[6], c8[6]); cswap(t8[3], c8[3], t8[7], c8[7]);
+					cswap(t8[0], c8[0], t8[1], c8[1]); cswap(t8[2], c8[2], t8[3], c8[3]);
+					cswap(t8[4], c8[4], t8[5], c8[5]); cswap(t8[6], c8[6], t8[7], c8[7]);
+					cswap(t8[2], c8[2], t8[4], c8[4]); cswap(t8[3], c8[3], t8[5], c8[5]);
+					cswap(t8[1], c8[1], t8[4], c8[4]); cswap(t8[3], c8[3], t8[6], c8[6]);
+					cswap(t8[1], c8[1], t8[2], c8[2]); cswap(t8[3], c8[3], t8[4], c8[4]);
+					cswap(t8[5], c8[5], t8[6], c8[6]);
+					bool ok = true;
+					// sorted: the lanes that push entry k also push k-1..1; a
+					// position no lane of the wave reaches is skipped as a whole
+#pragma unroll
+					for (int k = 7; k >= 1; --k)
+						if (__ballot(t8[k] != INFINITY))
+							if (t8[k] != INFINITY) ok = ok && push(st, c8[k], t8[k], P.error);
+					if (t8[0] != INFINITY) {
+						cur = c8[0];
+						curT = t8[0];
+					} else
+						cur = LRH_CHILD_EMPTY;
+					if (!ok) done = true;
+					break;
+				}
+				float t4[4];
+				uint32_t c4[4];
 				if (LAYOUT == 0)
 					testNodeQ(reinterpret_cast<const lrh_node4q *>(v.nodes), cur, r, mint, best, t4, c4);
 				else
@@ -609,6 +668,10 @@ typedef void (*KernelFn)(LrhTraceParams);
 	                         : (const void *)&lrh_trace2_kernel<L, A, C, B>)                  \
 	     : (const void *)&lrh_trace_kernel<L, A, C, B>)
 static const void *kernelPtr(int layout, bool two, bool any, bool count, bool batch) {
+	// 8-wide nodes: single-level, not batched
+	static const void *wide8[2][2] = {{LRH_K(2, false, false, false, false), LRH_K(2, false, false, true, false)},
+			{LRH_K(2, false, true, false, false), LRH_K(2, false, true, true, false)}};
+	if (layout == 2) return (two || batch) ? nullptr : wide8[any ? 1 : 0][count ? 1 : 0];
 	static const void *plain[2][2][2][2] = {
 		{{{LRH_K(0, false, false, false, false), LRH_K(0, false, false, true, false)},
 		  {LRH_K(0, false, true, false, false), LRH_K(0, false, true, true, false)}},
@@ -632,12 +695,15 @@ hipError_t lrhLaunchReset(uint32_t *p, uint32_t n, hipStream_t s) {
 
 hipError_t lrhOccupancy(int layout, bool two, bool any, bool batch, int block, size_t lds,
 		int *blocksPerCU) {
-	return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kernelPtr(layout, two, any, false, batch),
-			block, lds);
+	const void *k = kernelPtr(layout, two, any, false, batch);
+	if (!k) return hipErrorInvalidDeviceFunction;
+	return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, k, block, lds);
 }
 
 hipError_t lrhLaunchTrace(int layout, bool two, bool any, bool count, bool batch, dim3 grid, dim3 block,
 		size_t lds, hipStream_t s, const LrhTraceParams &P) {
 	void *args[] = {(void *)&P};
-	return hipLaunchKernel(kernelPtr(layout, two, any, count, batch), grid, block, args, lds, s);
+	const void *k = kernelPtr(layout, two, any, count, batch);
+	if (!k) return hipErrorInvalidDeviceFunction;
+	return hipLaunchKernel(k, grid, block, args, lds, s);
 }

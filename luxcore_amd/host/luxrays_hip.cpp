@@ -232,7 +232,8 @@ HIPAccelerator::HIPAccelerator(const Context *c, AcceleratorType t) : ctx(c), ty
 	const std::string layout = cfg.Get("hip.bvh.layout", "QUANTIZED");
 	if (layout == "QUANTIZED") buildOpts.layout = 0;
 	else if (layout == "FP32") buildOpts.layout = 1;
-	else throw std::runtime_error("hip.bvh.layout must be QUANTIZED or FP32: " + layout);
+	else if (layout == "WIDE8") buildOpts.layout = 2;  // single-level (BVH) scenes only
+	else throw std::runtime_error("hip.bvh.layout must be QUANTIZED, FP32 or WIDE8: " + layout);
 	const int leaf = cfg.GetInt("hip.bvh.maxleafsize", 4);
 	if (leaf < 1 || leaf > 8) throw std::runtime_error("hip.bvh.maxleafsize must be 1..8");
 	buildOpts.maxLeafSize = (unsigned)leaf;

@@ -24,7 +24,8 @@
  *   accelerator.bvh.treetype, .costsamples, .isectcost, .travcost, .emptybonus          bvhaccel.cpp:49-70
  *   context.verbose                   true                                                   context.cpp:49
  * and the device's own:
- *   hip.bvh.layout                    QUANTIZED (64-byte nodes, default) | FP32 (128-byte nodes)
+ *   hip.bvh.layout                    QUANTIZED (64-byte nodes, default) | FP32 (128-byte nodes) |
+ *                                     WIDE8 (128-byte 8-wide nodes, BVH accelerator only)
  *   hip.bvh.maxleafsize               1..8 (default 4)
  *   hip.bvh.travcost                  SAH traversal cost per triangle test (default 1.0)
  *   hip.bvh.splitbudget               early split clipping, extra references per triangle (default 4; < 0 off)

@@ -17,12 +17,20 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 NODE_Q = np.dtype([("org", "<f4", 3), ("e", "u1", 3), ("nchild", "u1"), ("qlo_x", "u1", 4), ("qhi_x", "u1", 4),
                    ("qlo_y", "u1", 4), ("qhi_y", "u1", 4), ("qlo_z", "u1", 4), ("qhi_z", "u1", 4),
                    ("child", "<u4", 4), ("pad", "<u4", 2)])
+NODE_Q8 = np.dtype([("org", "<f4", 3), ("e", "u1", 3), ("nchild", "u1"), ("qlo_x", "u1", 8), ("qhi_x", "u1", 8),
+                    ("qlo_y", "u1", 8), ("qhi_y", "u1", 8), ("qlo_z", "u1", 8), ("qhi_z", "u1", 8),
+                    ("child", "<u4", 8), ("pad", "<u4", 8)])
 NODE_F = np.dtype([("lo_x", "<f4", 4), ("hi_x", "<f4", 4), ("lo_y", "<f4", 4), ("hi_y", "<f4", 4),
                    ("lo_z", "<f4", 4), ("hi_z", "<f4", 4), ("child", "<u4", 4), ("pad", "<u4", 4)])
 TRI = np.dtype([("p0", "<f4", 3), ("e1", "<f4", 3), ("e2", "<f4", 3), ("tri", "<u4"), ("mesh", "<u4"),
                 ("pad", "<u4")])
 INST = np.dtype([("root", "<u4"), ("kind", "<u4"), ("index", "<u4"), ("object", "<u4")])
-assert NODE_Q.itemsize == 64 and NODE_F.itemsize == 128 and TRI.itemsize == 48
+assert NODE_Q.itemsize == 64 and NODE_F.itemsize == 128 and NODE_Q8.itemsize == 128 and TRI.itemsize == 48
+
+
+def node_dtype(layout):
+    """Node record of layout 0 (4-wide quantized), 1 (4-wide fp32), 2 (8-wide quantized)."""
+    return {0: NODE_Q, 1: NODE_F, 2: NODE_Q8}[layout]
 
 
 def header_functions():
@@ -65,7 +73,7 @@ def decode_q(nodes):
 def check_tree(nodes, boxes_of_child, root, n_nodes, n_prims, layout):
     """Walk the tree: every primitive reached once, every child box contains
     the epsilon-expanded boxes of what it holds."""
-    if layout == 0:
+    if layout != 1:
         dec = decode_q(nodes)
     else:
         dec = {k: nodes[k] for k in ("lo_x", "hi_x", "lo_y", "hi_y", "lo_z", "hi_z")}
@@ -74,7 +82,7 @@ def check_tree(nodes, boxes_of_child, root, n_nodes, n_prims, layout):
     while stack:
         i = stack.pop()
         assert 0 <= i < n_nodes
-        for c in range(4):
+        for c in range(nodes["child"].shape[1]):
             ref = int(nodes["child"][i][c])
             if ref == 0xFFFFFFFF:
                 continue
@@ -91,14 +99,14 @@ def check_tree(nodes, boxes_of_child, root, n_nodes, n_prims, layout):
     assert np.all(count == 1)
 
 
-@pytest.mark.parametrize("layout,max_leaf", [(0, 4), (0, 1), (0, 8), (1, 4), (1, 2)])
+@pytest.mark.parametrize("layout,max_leaf", [(0, 4), (0, 1), (0, 8), (1, 4), (1, 2), (2, 4), (2, 1), (2, 8)])
 def test_host_builder_records(layout, max_leaf):
     # one record per triangle (early split clipping off), boxes of whole triangles
     meshes = scenes.plain_scene(seed=1, n_random=2000)
     sc = D.host_scene_bvh(meshes, layout=layout, max_leaf=max_leaf, threads=4, split_budget=-1.0)
     info = sc.info()
     nb, pb = sc.download()
-    nodes = nb.view(NODE_Q if layout == 0 else NODE_F)
+    nodes = nb.view(node_dtype(layout))
     tris = pb.view(TRI)
     assert nodes.shape[0] == info["nodeCount"] and tris.shape[0] == sum(m.ntris for m in meshes)
     # records carry p0 and the fp32 edges of the original triangle
@@ -136,13 +144,13 @@ def _grid(k):
 
 def leaf_boxes_of_records(nodes, root, n_prims, layout):
     """(lo, hi) of the leaf child slot that holds each record."""
-    dec = decode_q(nodes) if layout == 0 else {k: nodes[k] for k in ("lo_x", "hi_x", "lo_y", "hi_y", "lo_z", "hi_z")}
+    dec = decode_q(nodes) if layout != 1 else {k: nodes[k] for k in ("lo_x", "hi_x", "lo_y", "hi_y", "lo_z", "hi_z")}
     lo = np.full((n_prims, 3), np.nan, np.float32)
     hi = np.full((n_prims, 3), np.nan, np.float32)
     stack = [root]
     while stack:
         i = stack.pop()
-        for c in range(4):
+        for c in range(nodes["child"].shape[1]):
             ref = int(nodes["child"][i][c])
             if ref == 0xFFFFFFFF:
                 continue
@@ -155,7 +163,7 @@ def leaf_boxes_of_records(nodes, root, n_prims, layout):
     return lo, hi
 
 
-@pytest.mark.parametrize("layout,budget", [(0, 0.0), (0, 4.0), (1, 1.0)])
+@pytest.mark.parametrize("layout,budget", [(0, 0.0), (0, 4.0), (1, 1.0), (2, 4.0)])
 def test_split_triangles_cover_their_triangles(layout, budget):
     """Early split clipping: triangles whose box is far above the median are
     cut into references with tighter boxes.  Every triangle keeps at least
@@ -167,7 +175,7 @@ def test_split_triangles_cover_their_triangles(layout, budget):
     n_tris = sum(m.ntris for m in meshes)
     sc = D.host_scene_bvh(meshes, layout=layout, threads=4, split_budget=budget)
     nb, pb = sc.download()
-    nodes = nb.view(NODE_Q if layout == 0 else NODE_F)
+    nodes = nb.view(node_dtype(layout))
     tris = pb.view(TRI)
     assert n_tris < tris.shape[0] <= n_tris * (1 + (budget or 4.0)) + 64
     lo, hi = leaf_boxes_of_records(nodes, 0, tris.shape[0], layout)
@@ -260,8 +268,9 @@ def test_object_boxes_equal_oracle_leaf_boxes():
 
 def test_documented_limits_fail_cleanly():
     """Size limits are checked before anything touches a device: 2^31-1 rays
-    per trace, query 0/1, maxLeaf 1..8 (0: default), layouts 0/1; updates and refits
-    need a two-level device scene with the same objects."""
+    per trace, query 0/1, maxLeaf 1..8 (0: default), layouts 0..2 (8-wide
+    layout 2 single-level only); updates and refits need a two-level device
+    scene with the same objects."""
     L = D.lib()
     host = D.host_scene_bvh(scenes.plain_scene(n_random=10))
     fake = ctypes.c_void_p(1 << 20)
@@ -272,10 +281,12 @@ def test_documented_limits_fail_cleanly():
     assert b"host-only" in L.lrh_last_error()
     assert L.lrh_scene_refit(host.h, None, 0) == -1
     assert L.lrh_scene_update(host.h, None, 0) == -1
-    for bad in (dict(max_leaf=9), dict(layout=2)):  # 0 selects the default
+    for bad in (dict(max_leaf=9), dict(layout=3)):  # 0 selects the default
         with pytest.raises(D.LrhError):
             D.host_scene_bvh(scenes.plain_scene(n_random=10), **bad)
     sc = scenes.instanced_scene(n_inst=6, motion_frac=0.5, freq=4, extent=5.0, seed=3)
+    with pytest.raises(D.LrhError, match="two-level"):
+        D.host_scene_mbvh(sc, layout=2)
     two = D.host_scene_mbvh(sc)
     with pytest.raises(D.LrhError):
         two.object_boxes(5)  # wrong object count

@@ -30,7 +30,7 @@ def plain():
     return meshes, rays, ref
 
 
-@pytest.mark.parametrize("layout,max_leaf", [(0, 4), (0, 1), (0, 8), (1, 4)])
+@pytest.mark.parametrize("layout,max_leaf", [(0, 4), (0, 1), (0, 8), (1, 4), (2, 4), (2, 1), (2, 8)])
 def test_plain_closest_parity(gpu_device, plain, layout, max_leaf):
     meshes, rays, ref = plain
     _case("plain_closest", layout=layout, max_leaf=max_leaf, rays=rays.shape[0])

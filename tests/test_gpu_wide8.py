@@ -1,0 +1,120 @@
+"""8-wide nodes (layout 2: one 128-byte L2 line per node, eight children
+with 8-bit bounds, sorted by entry distance with a 19-comparator network):
+the same hits as the oracle to the parity bar, the same hits for every
+launch shape, fewer node visits than the 4-wide tree, the no-alloc entry
+point equal to lrh_trace, and the paths that do not take 8-wide nodes
+(batches, device builds) refusing them."""
+import numpy as np
+import pytest
+
+import oracle
+from luxcore_amd import device as D
+from luxcore_amd import scenes
+from luxcore_amd.raytypes import HIT_DTYPE, NULL_INDEX
+from tests.parity import assert_parity, compare, tri_tie_checker
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(name, **kw):
+    print(f"[case] {name} {kw}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def hf():
+    meshes = scenes.heightfield_scene(cells=256, n_spheres=8, seed=3)
+    prim = scenes.pinhole_rays(320, 180)
+    ob = oracle.OracleBVH(meshes, builder="SAH")
+    ref_p = ob.intersect(prim)
+    bounce = scenes.bounce_rays(meshes, prim, ref_p)
+    return meshes, prim, ob, ref_p, bounce
+
+
+def test_wide8_heightfield_against_oracle(gpu_device, hf):
+    meshes, prim, ob, ref_p, bounce = hf
+    _case("wide8_heightfield")
+    sc = gpu_device.scene_bvh(meshes, layout=2)
+    assert sc.info()["layout"] == 2
+    for name, r, ref in (("primary", prim, ref_p), ("bounce", bounce, ob.intersect(bounce))):
+        got = gpu_device.trace_numpy(sc, r)
+        st = compare(ref, got, tri_tie_checker(oracle, meshes, r, ref, got))
+        print(name, st, flush=True)
+        assert_parity(st)
+    shadow = scenes.shadow_rays(meshes, prim, ref_p)
+    ref_s = ob.intersect(shadow)
+    got_s = gpu_device.trace_numpy(sc, shadow, query=1)
+    assert np.array_equal(ref_s["meshIndex"] == NULL_INDEX, got_s["meshIndex"] == NULL_INDEX)
+    sc.close()
+
+
+def test_wide8_launch_shapes_and_masks(gpu_device, hf):
+    """Every LDS ring depth and launch shape gives the default's hits bit for
+    bit (8-wide visits push up to seven entries: deep traversals spill);
+    masked rays keep their hit records."""
+    meshes, prim, ob, ref_p, bounce = hf
+    _case("wide8_shapes")
+    sc = gpu_device.scene_bvh(meshes, layout=2)
+    rays = np.concatenate([bounce, scenes.plain_rays(20000, seed=9)])
+    rays["flags"][::17] = 1
+    init = np.zeros(rays.shape[0], HIT_DTYPE)
+    init["t"] = -3.0
+    init["meshIndex"] = 0xABCDEF
+    base = gpu_device.trace_numpy(sc, rays, hits_init=init)
+    masked = (rays["flags"] & 1) != 0
+    assert np.all(base["t"][masked] == -3.0) and np.all(base["meshIndex"][masked] == 0xABCDEF)
+    for block, lds, refill in ((256, 4, 8), (256, 16, 8), (128, 32, 16), (64, 64, 4), (256, 4, 1)):
+        o = D.trace_opts(block, lds, 0, refill)
+        got = gpu_device.trace_numpy(sc, rays, opts=o, hits_init=init)
+        assert np.array_equal(got, base), (block, lds, refill)
+    sc.close()
+
+
+def test_wide8_fewer_visits_same_hits(gpu_device, hf):
+    meshes, prim, ob, ref_p, bounce = hf
+    _case("wide8_stats")
+    s4 = gpu_device.scene_bvh(meshes, layout=0)
+    s8 = gpu_device.scene_bvh(meshes, layout=2)
+    n = bounce.shape[0]
+    rb = gpu_device.upload(bounce)
+    hb = gpu_device.alloc(n * HIT_DTYPE.itemsize)
+    v4, p4, c4 = gpu_device.trace_stats(s4, rb, hb, n)
+    h4 = gpu_device.download(hb, HIT_DTYPE, n)
+    v8, p8, c8 = gpu_device.trace_stats(s8, rb, hb, n)
+    h8 = gpu_device.download(hb, HIT_DTYPE, n)
+    print("visits per ray 4-wide", v4 / c4, "8-wide", v8 / c8, "tests", p4 / c4, p8 / c8, flush=True)
+    assert c4 == c8 == n
+    assert v8 < 0.8 * v4
+    st = compare(h4, h8)
+    print(st, flush=True)
+    assert_parity(st)
+    assert s8.info()["nodeCount"] < s4.info()["nodeCount"]
+
+
+def test_wide8_noalloc_equals_trace(gpu_device):
+    _case("wide8_noalloc")
+    meshes = scenes.plain_scene(seed=1, n_random=20000)
+    sc = gpu_device.scene_bvh(meshes, layout=2)
+    for n in (1 << 17, (1 << 17) - 999):  # second call: fewer rays
+        rays = scenes.plain_rays(n, seed=40 + n)
+        rb = gpu_device.upload(rays)
+        ha = gpu_device.upload(np.zeros(n, HIT_DTYPE))
+        hbuf = gpu_device.upload(np.zeros(n, HIT_DTYPE))
+        D.check(D.lib().lrh_trace(sc.h, rb.ptr, ha.ptr, n, 0, None))
+        D.check(D.lib().lrh_trace_noalloc(sc.h, rb.ptr, hbuf.ptr, n, 0, None))
+        gpu_device.sync()
+        assert gpu_device.download(ha, HIT_DTYPE, n).tobytes() == gpu_device.download(hbuf, HIT_DTYPE, n).tobytes()
+    sc.close()
+
+
+def test_wide8_refused_where_not_supported(gpu_device):
+    _case("wide8_refused")
+    meshes = scenes.plain_scene(seed=1, n_random=100)
+    with pytest.raises(D.LrhError):
+        gpu_device.scene_bvh(meshes, layout=2, builder=D.BUILD_DEVICE_LBVH)
+    sc = gpu_device.scene_bvh(meshes, layout=2)
+    rays = scenes.plain_rays(1024, seed=5)
+    rb = gpu_device.upload(rays)
+    hb = gpu_device.alloc(1024 * HIT_DTYPE.itemsize)
+    with pytest.raises(D.LrhError, match="8-wide"):
+        gpu_device.trace_batch([(sc, rb, hb, 1024, 0)])
+    sc.close()
