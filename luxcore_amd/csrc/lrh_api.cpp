@@ -922,7 +922,6 @@ int lrh_scene_create_bvh(lrh_device *dev, const lrh_mesh *meshes, unsigned int n
 					"builder must be LRH_BUILD_HOST_SAH, LRH_BUILD_DEVICE_LBVH or LRH_BUILD_DEVICE_PLOC");
 		if (builder != LRH_BUILD_HOST_SAH) {
 			if (!dev) throw Error(LRH_EINVAL, "the device builder needs a device");
-			if (s->layout == 2) throw Error(LRH_EINVAL, "the device builders write layouts 0 and 1");
 			deviceBuild(s.get(), meshes, nMeshes, builder);
 			return;
 		}

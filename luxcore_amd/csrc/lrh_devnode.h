@@ -71,8 +71,11 @@ LRH_DN inline float expScale(uint32_t e) { return fromBits(e << 23); }
 LRH_DN inline float decodeQ(float org, uint32_t q, float s) { return org + (float)q * s; }
 
 // outward-rounded 8-bit child bounds, the rule of the host builder's quantize
-LRH_DN inline void quantizeNode(const GBox *cb, int n, const uint32_t child[4], lrh_node4q *out) {
-	lrh_node4q q;
+// W-wide quantized node (lrh_node4q, lrh_node8q): the host builder's
+// outward rounding
+template <int W, class Node>
+LRH_DN inline void quantizeNodeW(const GBox *cb, int n, const uint32_t *child, Node *out) {
+	Node q;
 	GBox pb;
 	for (int k = 0; k < 3; ++k) {
 		pb.lo[k] = INFINITY;
@@ -97,7 +100,7 @@ LRH_DN inline void quantizeNode(const GBox *cb, int n, const uint32_t child[4], 
 		while (e < 254 && decodeQ(o, 255, expScale((uint32_t)e)) < pb.hi[k]) ++e;
 		const float s = expScale((uint32_t)e);
 		*ex[k] = (uint8_t)e;
-		for (int c = 0; c < 4; ++c) {
+		for (int c = 0; c < W; ++c) {
 			if (c >= n) {
 				qlo[k][c] = 255;  // empty slot: inverted box
 				qhi[k][c] = 0;
@@ -117,9 +120,17 @@ LRH_DN inline void quantizeNode(const GBox *cb, int n, const uint32_t child[4], 
 		}
 	}
 	q.nchild = (uint8_t)n;
-	for (int c = 0; c < 4; ++c) q.child[c] = child[c];
-	q.pad[0] = q.pad[1] = 0;
+	for (int c = 0; c < W; ++c) q.child[c] = child[c];
+	for (auto &x : q.pad) x = 0;
 	*out = q;
+}
+
+LRH_DN inline void quantizeNode(const GBox *cb, int n, const uint32_t child[4], lrh_node4q *out) {
+	quantizeNodeW<4>(cb, n, child, out);
+}
+
+LRH_DN inline void quantizeNode8(const GBox *cb, int n, const uint32_t child[8], lrh_node8q *out) {
+	quantizeNodeW<8>(cb, n, child, out);
 }
 
 LRH_DN inline void floatNode(const GBox *cb, int n, const uint32_t child[4], lrh_node4f *out) {

@@ -425,20 +425,20 @@ __global__ void k_plocFinish(uint32_t n, const uint32_t *left, const uint32_t *r
 
 // 5a. children of each frontier node and how many become inner nodes
 __global__ void k_expand(const uint32_t *frontier, uint32_t nf, uint32_t nInner, const uint32_t *left,
-		const uint32_t *right, const uint32_t *leafOK, const GBox *nodeBox, uint32_t *kids, uint32_t *innerCount,
-		uint32_t *err) {
+		const uint32_t *right, const uint32_t *leafOK, const GBox *nodeBox, int width, uint32_t *kids,
+		uint32_t *innerCount, uint32_t *err) {
 	const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
 	if (e >= nf) return;
 	const uint32_t b = frontier[e];
 	if (b >= nInner) {  // not an internal node
 		atomicOr(err, kErrExpand);
-		for (int k = 0; k < 4; ++k) kids[4 * e + k] = kNone;
+		for (int k = 0; k < kMaxWide; ++k) kids[(size_t)kMaxWide * e + k] = kNone;
 		innerCount[e] = 0;
 		return;
 	}
-	uint32_t ch[4];
-	innerCount[e] = expandNode(b, left, right, leafOK, nodeBox, ch);
-	for (int k = 0; k < 4; ++k) kids[4 * e + k] = ch[k];
+	uint32_t ch[kMaxWide];
+	innerCount[e] = expandNode(b, left, right, leafOK, nodeBox, width, ch);
+	for (int k = 0; k < kMaxWide; ++k) kids[(size_t)kMaxWide * e + k] = ch[k];
 }
 
 // 5b. the frontier's wide nodes; inner children get slots after the level
@@ -447,8 +447,8 @@ __global__ void k_emit(uint32_t nf, const uint32_t *kids, const uint32_t *innerO
 		uint32_t layout, void *nodes, uint32_t *next) {
 	const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
 	if (e >= nf) return;
-	uint32_t ch[4];
-	for (int k = 0; k < 4; ++k) ch[k] = kids[4 * e + k];
+	uint32_t ch[kMaxWide];
+	for (int k = 0; k < kMaxWide; ++k) ch[k] = kids[(size_t)kMaxWide * e + k];
 	emitNode(ch, innerOff[e], first, last, leafOK, leafBox, nodeBox, levelBase + nf, layout, nodes, levelBase + e,
 			next);
 }
@@ -466,9 +466,12 @@ __global__ void k_smallRoot(const GBox *leafBox, uint32_t n, uint32_t layout, vo
 			b.lo[k] = minT(b.lo[k], leafBox[i].lo[k]);
 			b.hi[k] = maxT(b.hi[k], leafBox[i].hi[k]);
 		}
-	const uint32_t ref[4] = {leafRef(0, n), LRH_CHILD_EMPTY, LRH_CHILD_EMPTY, LRH_CHILD_EMPTY};
+	uint32_t ref[kMaxWide];
+	for (int k = 0; k < kMaxWide; ++k) ref[k] = k ? LRH_CHILD_EMPTY : leafRef(0, n);
 	if (layout == 0)
 		lrhdev::quantizeNode(&b, 1, ref, reinterpret_cast<lrh_node4q *>(nodes));
+	else if (layout == 2)
+		lrhdev::quantizeNode8(&b, 1, ref, reinterpret_cast<lrh_node8q *>(nodes));
 	else
 		lrhdev::floatNode(&b, 1, ref, reinterpret_cast<lrh_node4f *>(nodes));
 }
@@ -714,7 +717,9 @@ void lrhGpuBuildBVH(const LrhGpuBuildIn &in, const LrhGpuAlloc &a, hipStream_t s
 	if (in.maxLeaf < 1 || in.maxLeaf > LRH_LEAF_MAX) throw std::runtime_error("device build: maxLeaf must be 1..8");
 	if (!(in.travCost >= 0.f) || !std::isfinite(in.travCost))
 		throw std::runtime_error("device build: travCost must be finite and >= 0");
-	const size_t nodeSz = in.layout == 0 ? sizeof(lrh_node4q) : sizeof(lrh_node4f);
+	if (in.layout > 2) throw std::runtime_error("device build: layout must be 0, 1 or 2");
+	const size_t nodeSz = in.layout == 0 ? sizeof(lrh_node4q) : in.layout == 2 ? sizeof(lrh_node8q) : sizeof(lrh_node4f);
+	const int width = in.layout == 2 ? 8 : 4;
 	Scratch S(a);
 	hipEvent_t e0, e1;
 	GB_CHECK(hipEventCreate(&e0));
@@ -791,7 +796,7 @@ void lrhGpuBuildBVH(const LrhGpuBuildIn &in, const LrhGpuAlloc &a, hipStream_t s
 
 			// breadth-first collapse: level frontiers of binary node ids
 			uint32_t *fA = S.get<uint32_t>(ni), *fB = S.get<uint32_t>(ni);
-			uint32_t *kids = S.get<uint32_t>(4 * (size_t)ni), *cnt = S.get<uint32_t>(ni + 1),
+			uint32_t *kids = S.get<uint32_t>((size_t)kMaxWide * ni), *cnt = S.get<uint32_t>(ni + 1),
 					 *off = S.get<uint32_t>(ni + 1);
 			hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, fA, 1u, 0u);  // the root
 			uint32_t nf = 1, levelBase = 0;
@@ -799,7 +804,7 @@ void lrhGpuBuildBVH(const LrhGpuBuildIn &in, const LrhGpuAlloc &a, hipStream_t s
 				if ((size_t)levelBase + nf > capNodes) throw std::runtime_error("device build: node capacity");
 				if (depth > 128) throw std::runtime_error("device build: collapse deeper than any valid tree");
 				hipLaunchKernelGGL(k_expand, dim3(grid(nf, 256)), dim3(256), 0, st, fA, nf, ni, left, right, leafOK,
-						nodeBox, kids, cnt, err);
+						nodeBox, width, kids, cnt, err);
 				hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, cnt + nf, 1u, 0u);
 				GB_CHECK(hipGetLastError());
 				exclusiveScan(S, cnt, off, nf + 1, st);

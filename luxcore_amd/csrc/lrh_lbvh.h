@@ -203,13 +203,14 @@ LRH_DN inline float nodeCost(float A, float cl, float cr, uint32_t N, uint32_t m
 LRH_DN inline bool leafChild(uint32_t c, const uint32_t *leafOK) { return (c & kLeafBit) || leafOK[c]; }
 
 // 4a. the children of internal node b (Morton order): its two, then the
-// largest-area child that is not a leaf split until there are four;
-// returns how many of them become inner nodes
+// largest-area child that is not a leaf split until there are `width` (4 or
+// 8); returns how many of them become inner nodes
+constexpr int kMaxWide = 8;
 LRH_DN inline uint32_t expandNode(uint32_t b, const uint32_t *left, const uint32_t *right, const uint32_t *leafOK,
-		const GBox *nodeBox, uint32_t kids[4]) {
-	uint32_t ch[4] = {left[b], right[b], kNone, kNone};
+		const GBox *nodeBox, int width, uint32_t kids[kMaxWide]) {
+	uint32_t ch[kMaxWide] = {left[b], right[b], kNone, kNone, kNone, kNone, kNone, kNone};
 	int nch = 2;
-	while (nch < 4) {
+	while (nch < width) {
 		int pick = -1;
 		float best = -1.f;
 		for (int k = 0; k < nch; ++k) {
@@ -228,7 +229,7 @@ LRH_DN inline uint32_t expandNode(uint32_t b, const uint32_t *left, const uint32
 		++nch;
 	}
 	uint32_t inner = 0;
-	for (int k = 0; k < 4; ++k) {
+	for (int k = 0; k < kMaxWide; ++k) {
 		kids[k] = ch[k];
 		if (ch[k] != kNone && !leafChild(ch[k], leafOK)) ++inner;
 	}
@@ -242,14 +243,15 @@ LRH_DN inline uint32_t leafRef(uint32_t start, uint32_t count) {
 // 4b. the wide node of a frontier entry: leaf children reference their key
 // range, inner children the slots innerOff, innerOff+1, ... after the level
 // (their binary ids go to next[slot])
-LRH_DN inline void emitNode(const uint32_t kids[4], uint32_t innerOff, const uint32_t *first, const uint32_t *last,
-		const uint32_t *leafOK, const GBox *leafBox, const GBox *nodeBox, uint32_t childBase, uint32_t layout, void *nodes,
-		uint32_t nodeIndex, uint32_t *next) {
-	GBox cb[4];
-	uint32_t ref[4] = {LRH_CHILD_EMPTY, LRH_CHILD_EMPTY, LRH_CHILD_EMPTY, LRH_CHILD_EMPTY};
+LRH_DN inline void emitNode(const uint32_t kids[kMaxWide], uint32_t innerOff, const uint32_t *first,
+		const uint32_t *last, const uint32_t *leafOK, const GBox *leafBox, const GBox *nodeBox, uint32_t childBase,
+		uint32_t layout, void *nodes, uint32_t nodeIndex, uint32_t *next) {
+	GBox cb[kMaxWide];
+	uint32_t ref[kMaxWide];
+	for (int k = 0; k < kMaxWide; ++k) ref[k] = LRH_CHILD_EMPTY;
 	int n = 0;
 	uint32_t slot = innerOff;
-	for (int k = 0; k < 4; ++k) {
+	for (int k = 0; k < kMaxWide; ++k) {
 		const uint32_t c = kids[k];
 		if (c == kNone) continue;
 		cb[n] = (c & kLeafBit) ? leafBox[c & ~kLeafBit] : nodeBox[c];
@@ -265,6 +267,8 @@ LRH_DN inline void emitNode(const uint32_t kids[4], uint32_t innerOff, const uin
 	}
 	if (layout == 0)
 		lrhdev::quantizeNode(cb, n, ref, reinterpret_cast<lrh_node4q *>(nodes) + nodeIndex);
+	else if (layout == 2)
+		lrhdev::quantizeNode8(cb, n, ref, reinterpret_cast<lrh_node8q *>(nodes) + nodeIndex);
 	else
 		lrhdev::floatNode(cb, n, ref, reinterpret_cast<lrh_node4f *>(nodes) + nodeIndex);
 }

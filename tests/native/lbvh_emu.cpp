@@ -180,7 +180,8 @@ int main(int argc, char **argv) {
 	}
 	const uint32_t n = tbase.back();
 	uint32_t err = 0, depth = 0, rounds = 0;
-	const size_t nodeSz = layout == 0 ? sizeof(lrh_node4q) : sizeof(lrh_node4f);
+	const size_t nodeSz = layout == 0 ? sizeof(lrh_node4q) : layout == 2 ? sizeof(lrh_node8q) : sizeof(lrh_node4f);
+	const int width = layout == 2 ? 8 : 4;
 	std::vector<uint8_t> nodes;
 	std::vector<lrh_tri> recs;
 	std::vector<uint32_t> vals(n);
@@ -212,10 +213,13 @@ int main(int argc, char **argv) {
 				b.lo[k] = minT(b.lo[k], leafBox[i].lo[k]);
 				b.hi[k] = maxT(b.hi[k], leafBox[i].hi[k]);
 			}
-		const uint32_t ref[4] = {leafRef(0, n), LRH_CHILD_EMPTY, LRH_CHILD_EMPTY, LRH_CHILD_EMPTY};
+		uint32_t ref[kMaxWide];
+		for (int k = 0; k < kMaxWide; ++k) ref[k] = k ? LRH_CHILD_EMPTY : leafRef(0, n);
 		nodes.assign(nodeSz, 0);
 		if (layout == 0)
 			lrhdev::quantizeNode(&b, 1, ref, reinterpret_cast<lrh_node4q *>(nodes.data()));
+		else if (layout == 2)
+			lrhdev::quantizeNode8(&b, 1, ref, reinterpret_cast<lrh_node8q *>(nodes.data()));
 		else
 			lrhdev::floatNode(&b, 1, ref, reinterpret_cast<lrh_node4f *>(nodes.data()));
 		nodeCount = 1;
@@ -307,19 +311,19 @@ int main(int argc, char **argv) {
 					err |= kErrExpand;
 					break;
 				}
-				std::vector<uint32_t> kids(4 * (size_t)nf), cnt(nf + 1, 0);
+				std::vector<uint32_t> kids((size_t)kMaxWide * nf), cnt(nf + 1, 0);
 				for (uint32_t e = 0; e < nf; ++e) {
 					if (fA[e] >= ni) {
 						err |= kErrExpand;
 						continue;
 					}
 					cnt[e] = expandNode(fA[e], left.data(), right.data(), leafOK.data(), nodeBox.data(),
-							&kids[4 * (size_t)e]);
+							width, &kids[(size_t)kMaxWide * e]);
 				}
 				exclusiveScan(cnt);
 				fB.assign(cnt[nf], 0);
 				for (uint32_t e = 0; e < nf; ++e)
-					emitNode(&kids[4 * (size_t)e], cnt[e], first.data(), last.data(), leafOK.data(), leafBox.data(),
+					emitNode(&kids[(size_t)kMaxWide * e], cnt[e], first.data(), last.data(), leafOK.data(), leafBox.data(),
 							nodeBox.data(), levelBase + nf, layout, nodes.data(), levelBase + e, fB.data());
 				levelBase += nf;
 				fA.swap(fB);

@@ -15,7 +15,7 @@ from luxcore_amd import device as D
 from luxcore_amd import scenes
 from luxcore_amd.raytypes import HIT_DTYPE, NULL_INDEX, machine_epsilon
 from tests.parity import assert_parity, compare
-from tests.test_abi_cpu import NODE_F, NODE_Q, TRI, check_tree
+from tests.test_abi_cpu import TRI, check_tree, node_dtype
 
 pytestmark = pytest.mark.gpu
 
@@ -47,14 +47,14 @@ def _records_sorted(tris):
 
 
 @pytest.mark.parametrize("builder", list(BUILDERS))
-@pytest.mark.parametrize("layout,max_leaf", [(0, 4), (0, 1), (0, 8), (1, 4)])
+@pytest.mark.parametrize("layout,max_leaf", [(0, 4), (0, 1), (0, 8), (1, 4), (2, 4)])
 def test_device_build_tree_and_records(gpu_device, layout, max_leaf, builder):
     _case("devbuild_tree", layout=layout, max_leaf=max_leaf, builder=builder)
     meshes = scenes.plain_scene(seed=1, n_random=20000)
     dsc = gpu_device.scene_bvh(meshes, layout=layout, max_leaf=max_leaf, builder=BUILDERS[builder])
     info = dsc.info()
     nb, pb = dsc.download()
-    nodes = nb.view(NODE_Q if layout == 0 else NODE_F)
+    nodes = nb.view(node_dtype(layout))
     tris = pb.view(TRI)
     ntri = sum(m.ntris for m in meshes)
     assert tris.shape[0] == ntri and nodes.shape[0] == info["nodeCount"]
@@ -85,7 +85,8 @@ def emu(tmp_path_factory):
 
 
 @pytest.mark.parametrize("builder", list(BUILDERS))
-@pytest.mark.parametrize("scene,layout,max_leaf", [("plain", 0, 4), ("plain", 1, 2), ("heightfield", 0, 4)])
+@pytest.mark.parametrize("scene,layout,max_leaf", [("plain", 0, 4), ("plain", 1, 2), ("heightfield", 0, 4),
+                                                   ("heightfield", 2, 4)])
 def test_device_build_equals_cpu_emulation(gpu_device, emu, tmp_path, scene, layout, max_leaf, builder):
     """The kernels' tree is the sequential emulation's, node and record bytes
     alike: the per-element steps are shared and every step is deterministic
@@ -108,7 +109,7 @@ def test_device_build_equals_cpu_emulation(gpu_device, emu, tmp_path, scene, lay
 
 
 @pytest.mark.parametrize("builder", list(BUILDERS))
-@pytest.mark.parametrize("layout", [0, 1])
+@pytest.mark.parametrize("layout", [0, 1, 2])
 def test_device_build_trace_parity(gpu_device, layout, builder):
     _case("devbuild_parity", layout=layout, builder=builder)
     meshes = scenes.plain_scene(seed=1, n_random=20000)

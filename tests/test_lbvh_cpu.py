@@ -19,7 +19,7 @@ import pytest
 from luxcore_amd import device as D
 from luxcore_amd import scenes
 from luxcore_amd.raytypes import machine_epsilon
-from tests.test_abi_cpu import NODE_F, NODE_Q, TRI, check_tree, decode_q
+from tests.test_abi_cpu import NODE_F, TRI, check_tree, decode_q, node_dtype
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ERR_UNSORTED, ERR_RANGE = 1, 2
@@ -55,7 +55,7 @@ def run_emu(emu, tmp_path, meshes, max_leaf, layout, mode="normal", method="lbvh
     assert p.returncode == 0, p.stderr
     f = (out / "info.txt").read_text().split()
     info = {f[i]: int(f[i + 1]) for i in range(0, len(f), 2)}
-    nodes = np.fromfile(out / "nodes.bin", NODE_Q if layout == 0 else NODE_F)
+    nodes = np.fromfile(out / "nodes.bin", node_dtype(layout))
     tris = np.fromfile(out / "tris.bin", TRI)
     return info, nodes, tris
 
@@ -99,7 +99,7 @@ def sah_cost(nodes, layout, c_node=1.0, c_tri=1.0):
 
 
 @pytest.mark.parametrize("method", ["lbvh", "ploc"])
-@pytest.mark.parametrize("max_leaf,layout", [(4, 0), (1, 0), (8, 1)])
+@pytest.mark.parametrize("max_leaf,layout", [(4, 0), (1, 0), (8, 1), (4, 2), (1, 2)])
 def test_emulated_device_build_is_a_valid_tree(emu, tmp_path, max_leaf, layout, method):
     meshes = scenes.plain_scene(seed=1, n_random=2000)
     info, nodes, tris = run_emu(emu, tmp_path, meshes, max_leaf, layout, method=method)
