@@ -701,8 +701,8 @@ def main():
     ap.add_argument("--cells", type=int, default=3240)
     ap.add_argument("--layout", type=int, default=None, choices=(0, 1, 2),
                     help="node layout: 0 4-wide 64B quantized, 1 4-wide 128B fp32, 2 8-wide 128B quantized "
-                         "(default: 2 for the headline over the host SAH tree, else 0; the device builders and "
-                         "the batched, two-level and served paths take 0 or 1)")
+                         "(default: 2 for the headline over the host SAH tree, else 0; the batched, two-level "
+                         "and served paths take 0 or 1)")
     ap.add_argument("--max-leaf", type=int, default=4)
     ap.add_argument("--builder", choices=("sah", "lbvh", "ploc"), default="sah",
                     help="headline tree: host binned SAH (default), or a device build: LBVH (radix tree) or "
