@@ -193,22 +193,23 @@ def extra_workloads(args, dev):
     t0 = time.time()
     meshes = scenes.batch_scenes()
     bufs = scenes.batch_buffers()
-    wl = B.BatchedWorkload(dev, meshes, bufs)
+    wl = B.BatchedWorkload(dev, meshes, bufs, layout=2)
     items = wl.items()
     total = wl.total_rays()
-    key = B.signature(dev, [m[0].ntris for m in meshes], [c for _, c, _ in bufs], "batched-v2")
+    key = B.signature(dev, [m[0].ntris for m in meshes], [c for _, c, _ in bufs], "batched-v3-layout2")
     opts, _ = B.autotune(dev, items, key=key)
     cnt = {"node_visits": 0, "prim_tests": 0, "rays": 0}
     for (s, r, h, cn, q) in items:
         c = dev.trace_counters(s, r, h, cn, q, opts)
         for k in cnt:
             cnt[k] += c[k]
-    b_ray = RAY_BYTES + HIT_BYTES + (64 * cnt["node_visits"] + 48 * cnt["prim_tests"]) / max(cnt["rays"], 1)
+    # 8-wide nodes: 96 bytes read per visit
+    b_ray = RAY_BYTES + HIT_BYTES + (96 * cnt["node_visits"] + 48 * cnt["prim_tests"]) / max(cnt["rays"], 1)
     bt = dev.batch(items, opts)
     bt.capture()
     wall, lat = _event_timed(dev, bt.replay, steps, 2)
     bl = {"workload": "8 scenes 10k..5M tris (soups + sphere grids), 64 buffers log-uniform 4Ki..4Mi rays, "
-                      "mixed closest/any", "rays": total,
+                      "mixed closest/any; 8-wide nodes", "rays": total,
           "shape": {"block": opts.blockSize, "lds": opts.ldsStack, "refill": opts.refillThreshold}}
     bl.update(_rate_line(total, wall, lat, steps, b_ray))
     bl["per_ray"] = {"node_visits": round(cnt["node_visits"] / max(cnt["rays"], 1), 3),
@@ -261,12 +262,12 @@ def run_batched(args, dev):
     t0 = time.time()
     meshes = scenes.batch_scenes()
     bufs = scenes.batch_buffers()
-    wl = B.BatchedWorkload(dev, meshes, bufs)
+    wl = B.BatchedWorkload(dev, meshes, bufs, layout=args.layout)
     total = wl.total_rays()
     log(f"batched workload: {sum(m[0].ntris for m in meshes)} tris in 8 scenes, {len(bufs)} buffers, "
         f"{total} rays ({time.time() - t0:.1f}s)")
     items = wl.items()
-    key = B.signature(dev, [m[0].ntris for m in meshes], [c for _, c, _ in bufs], "batched-v1")
+    key = B.signature(dev, [m[0].ntris for m in meshes], [c for _, c, _ in bufs], f"batched-v3-layout{args.layout}")
     opts, table = B.autotune(dev, items, key=key, log=log)
     log(f"autotuned shape: block {opts.blockSize} lds {opts.ldsStack} refill {opts.refillThreshold}")
     # plain: one lrh_trace per buffer, same launch shape
@@ -701,8 +702,8 @@ def main():
     ap.add_argument("--cells", type=int, default=3240)
     ap.add_argument("--layout", type=int, default=None, choices=(0, 1, 2),
                     help="node layout: 0 4-wide 64B quantized, 1 4-wide 128B fp32, 2 8-wide 128B quantized "
-                         "(default: 2 for the headline over the host SAH tree, else 0; the batched, two-level "
-                         "and served paths take 0 or 1)")
+                         "(default: 2 for the headline over the host SAH tree and the batched and served modes, "
+                         "else 0; two-level scenes take 0 or 1)")
     ap.add_argument("--max-leaf", type=int, default=4)
     ap.add_argument("--builder", choices=("sah", "lbvh", "ploc"), default="sah",
                     help="headline tree: host binned SAH (default), or a device build: LBVH (radix tree) or "
@@ -732,7 +733,8 @@ def main():
     if args.steps is None:
         args.steps = 1000 if args.mode == "headline" else 20
     if args.layout is None:
-        args.layout = 2 if args.mode == "headline" and args.builder == "sah" else 0
+        args.layout = 2 if (args.mode == "headline" and args.builder == "sah") or args.mode in ("batched", "served") \
+            else 0
     dev = D.Device(0)
     log(f"device {dev.name}, {dev.cu_count} CUs")
     modes = {"batched": run_batched, "scene": run_scene, "refit": run_refit, "configs": run_configs,

@@ -1412,7 +1412,6 @@ BatchTable makeTable(lrh_device *dev, const lrh_batch_item *items, unsigned n) {
 		if (((uintptr_t)it.rays & 15u) || ((uintptr_t)it.hits & 3u))
 			throw Error(LRH_EINVAL, "batch item " + std::to_string(i) + ": misaligned buffer");
 		if (it.query > 1) throw Error(LRH_EINVAL, "batch item " + std::to_string(i) + ": bad query");
-		if (s->layout == 2) throw Error(LRH_EINVAL, "8-wide nodes (layout 2) are traced by lrh_trace only");
 		if (j == 0) T.layout = s->layout;
 		else if (s->layout != T.layout) throw Error(LRH_EINVAL, "all scenes of a batch must share the node layout");
 		T.two = T.two || s->twoLevel;

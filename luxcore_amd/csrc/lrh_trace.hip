@@ -648,6 +648,12 @@ __global__ __launch_bounds__(256) void lrh_trace2_kernel(LrhTraceParams P) {
 	traceBody<LAYOUT, true, ANY, COUNT, BATCH>(P);
 }
 
+// batched traces over 8-wide nodes: at 7 waves per SIMD the per-lane item
+// view and the eight-entry sort spill; built for 6
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void lrh_trace8b_kernel(LrhTraceParams P) {
+	traceBody<2, false, false, false, true>(P);
+}
+
 template <bool ANY, bool COUNT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void lrh_trace2q_kernel(LrhTraceParams P) {
 	traceBody<0, true, ANY, COUNT, false>(P);
@@ -668,10 +674,14 @@ typedef void (*KernelFn)(LrhTraceParams);
 	                         : (const void *)&lrh_trace2_kernel<L, A, C, B>)                  \
 	     : (const void *)&lrh_trace_kernel<L, A, C, B>)
 static const void *kernelPtr(int layout, bool two, bool any, bool count, bool batch) {
-	// 8-wide nodes: single-level, not batched
+	// 8-wide nodes: single-level scenes (plain and batched)
 	static const void *wide8[2][2] = {{LRH_K(2, false, false, false, false), LRH_K(2, false, false, true, false)},
 			{LRH_K(2, false, true, false, false), LRH_K(2, false, true, true, false)}};
-	if (layout == 2) return (two || batch) ? nullptr : wide8[any ? 1 : 0][count ? 1 : 0];
+	static const void *wide8Batched = (const void *)&lrh_trace8b_kernel;
+	if (layout == 2) {
+		if (two) return nullptr;
+		return batch ? wide8Batched : wide8[any ? 1 : 0][count ? 1 : 0];
+	}
 	static const void *plain[2][2][2][2] = {
 		{{{LRH_K(0, false, false, false, false), LRH_K(0, false, false, true, false)},
 		  {LRH_K(0, false, true, false, false), LRH_K(0, false, true, true, false)}},

@@ -200,7 +200,7 @@ def bench(args, dev, log):
     Mrays/s per rate, next to a single plain batch of the same kind of rays."""
     from . import scenes as S
     meshes = S.batch_scenes()
-    scenes = [dev.scene_bvh(m) for m in meshes]
+    scenes = [dev.scene_bvh(m, layout=args.layout) for m in meshes]
     opts = D.trace_opts(256, 8, 0, 12)
     w = Server(dev, scenes, opts=opts)
     w.run(poisson_specs(1000.0, 0.004, len(scenes), seed=99))

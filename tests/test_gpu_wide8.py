@@ -2,9 +2,8 @@
 with 8-bit bounds, sorted by entry distance with a 19-comparator network):
 the same hits as the oracle to the parity bar, the same hits for every
 launch shape, fewer node visits than the 4-wide tree, the no-alloc entry
-point equal to lrh_trace, and the batched path, which does not take 8-wide
-nodes, refusing them (device builds of 8-wide trees are tested with the
-device builds)."""
+point equal to lrh_trace, and batches over 8-wide scenes equal to plain
+traces (device builds of 8-wide trees are tested with the device builds)."""
 import numpy as np
 import pytest
 
@@ -107,13 +106,30 @@ def test_wide8_noalloc_equals_trace(gpu_device):
     sc.close()
 
 
-def test_wide8_refused_where_not_supported(gpu_device):
-    _case("wide8_refused")
-    meshes = scenes.plain_scene(seed=1, n_random=100)
-    sc = gpu_device.scene_bvh(meshes, layout=2)
-    rays = scenes.plain_rays(1024, seed=5)
-    rb = gpu_device.upload(rays)
-    hb = gpu_device.alloc(1024 * HIT_DTYPE.itemsize)
-    with pytest.raises(D.LrhError, match="8-wide"):
-        gpu_device.trace_batch([(sc, rb, hb, 1024, 0)])
-    sc.close()
+def test_wide8_batched_equals_plain_traces(gpu_device):
+    """A batch over 8-wide scenes (its own kernel, built for 6 waves) gives
+    each buffer the hits of lrh_trace on it, closest and any hit; a batch
+    mixing node layouts is refused."""
+    _case("wide8_batched")
+    meshes = [scenes.plain_scene(seed=1, n_random=3000), scenes.heightfield_scene(cells=128, n_spheres=2, seed=3)]
+    scs = [gpu_device.scene_bvh(m, layout=2) for m in meshes]
+    items, refs = [], []
+    for i, (si, n, q) in enumerate(((0, 30000, 0), (1, 20000, 1), (1, 45000, 0), (0, 7000, 1))):
+        rays = scenes.plain_rays(n, seed=70 + i)
+        rb = gpu_device.upload(rays)
+        hb = gpu_device.alloc(n * HIT_DTYPE.itemsize)
+        items.append((scs[si], rb, hb, n, q))
+        refs.append(gpu_device.trace_numpy(scs[si], rays, query=q))
+    gpu_device.trace_batch(items)
+    gpu_device.sync()
+    for (sc, rb, hb, n, q), ref in zip(items, refs):
+        got = gpu_device.download(hb, HIT_DTYPE, n)
+        if q == 0:
+            assert got.tobytes() == ref.tobytes()
+        else:
+            assert np.array_equal(got["meshIndex"] == NULL_INDEX, ref["meshIndex"] == NULL_INDEX)
+    s4 = gpu_device.scene_bvh(meshes[0], layout=0)
+    with pytest.raises(D.LrhError, match="layout"):
+        gpu_device.trace_batch([items[0], (s4,) + items[2][1:]])
+    for s in scs + [s4]:
+        s.close()
