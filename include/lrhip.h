@@ -91,9 +91,8 @@ typedef struct {
 
 /* Build options (defaults when the pointer is NULL or a field is 0).
  * layout: 0 = 64-byte quantized 4-wide nodes (default), 1 = 128-byte fp32
- * 4-wide, 2 = 128-byte quantized 8-wide (one L2 line; single-level scenes,
- * host or device built, traced by every trace entry point including
- * batches; two-level scenes take 0 or 1 and return LRH_EINVAL for 2).
+ * 4-wide, 2 = 128-byte quantized 8-wide (one L2 line; every scene kind,
+ * builder and trace entry point).
  * builder: LRH_BUILD_HOST_SAH (default) builds on the host (binned SAH, the
  * stand-in for the reference's EMBREE_BINNED_SAH, bvhaccel.cpp:144-156);
  * LRH_BUILD_DEVICE_LBVH builds on the GPU (Morton order + radix tree,

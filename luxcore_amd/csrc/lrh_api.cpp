@@ -35,6 +35,7 @@ using lrh::Error;
 
 static_assert(sizeof(lrh_node4q) == 64, "node4q must be 64 bytes");
 static_assert(sizeof(lrh_node4f) == 128, "node4f must be 128 bytes");
+static_assert(sizeof(lrh_node8q) == 128, "node8q must be 128 bytes");
 static_assert(sizeof(lrh_tri) == 48, "tri record must be 48 bytes");
 static_assert(sizeof(lrh_motion) == 32, "motion record must be 32 bytes");
 static_assert(sizeof(lrh_motion_seg) == 304 && offsetof(lrh_motion_seg, start) == LRH_SEG_HOT_BYTES,
@@ -533,7 +534,8 @@ void buildRoot(lrh_scene *s, ObjectData &od, HostTree &ht, std::vector<lrh_inst>
 	lrh::WideBVH w;
 	lrh::buildWideBVH(od.boxes, rp, 0, 0, &w);
 	if (w.nodeCount() > s->rootReserve) throw Error(LRH_EINVAL, "root tree larger than its reservation");
-	if (s->layout) std::copy(w.fnodes.begin(), w.fnodes.end(), ht.f.begin());
+	if (s->layout == 2) std::copy(w.q8nodes.begin(), w.q8nodes.end(), ht.q8.begin());
+	else if (s->layout) std::copy(w.fnodes.begin(), w.fnodes.end(), ht.f.begin());
 	else std::copy(w.qnodes.begin(), w.qnodes.end(), ht.q.begin());
 	instOut.resize(od.inst.size());
 	for (size_t i = 0; i < w.order.size(); ++i) instOut[i] = od.inst[w.order[i]];
@@ -547,8 +549,8 @@ void buildRoot(lrh_scene *s, ObjectData &od, HostTree &ht, std::vector<lrh_inst>
 	uint32_t maxD = 0;
 	for (size_t q = 0; q < bfs.size(); ++q) {
 		const uint32_t i = bfs[q];
-		for (int c = 0; c < 4; ++c) {
-			const uint32_t r = s->layout ? w.fnodes[i].child[c] : w.qnodes[i].child[c];
+		for (int c = 0; c < (s->layout == 2 ? 8 : 4); ++c) {
+			const uint32_t r = s->layout == 2 ? w.q8nodes[i].child[c] : s->layout ? w.fnodes[i].child[c] : w.qnodes[i].child[c];
 			if (r == LRH_CHILD_EMPTY || (r & LRH_CHILD_LEAF) || r >= nn) continue;
 			depthOf[r] = depthOf[i] + 1;
 			maxD = std::max(maxD, depthOf[r]);
@@ -972,7 +974,6 @@ int lrh_scene_create_mbvh(lrh_device *dev, const lrh_mesh *meshes, unsigned int 
 		s->dev = dev;
 		s->params = toParams(opts);
 		s->layout = s->params.layout;
-		if (s->layout == 2) throw Error(LRH_EINVAL, "two-level scenes use layouts 0 and 1");
 		s->twoLevel = true;
 		const double t0 = now();
 		HostTree ht;
@@ -1065,7 +1066,7 @@ int lrh_scene_update(lrh_scene *s, const lrh_object *objects, unsigned int nObje
 		dev->activate();
 		HIPCHK(hipStreamSynchronize(dev->stream));
 		// root nodes live in the reserved prefix of the node array
-		const size_t nb = s->rootReserve * (s->layout ? sizeof(lrh_node4f) : sizeof(lrh_node4q));
+		const size_t nb = s->rootReserve * (s->layout == 2 ? sizeof(lrh_node8q) : s->layout ? sizeof(lrh_node4f) : sizeof(lrh_node4q));
 		HIPCHK(hipMemcpyAsync(s->nodes.p, ht.data(s->layout), nb, hipMemcpyHostToDevice, dev->stream));
 		HIPCHK(hipMemcpyAsync(s->insts.p, instOut.data(), instOut.size() * sizeof(lrh_inst), hipMemcpyHostToDevice,
 				dev->stream));

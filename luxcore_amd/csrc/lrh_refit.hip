@@ -187,12 +187,18 @@ __global__ void k_refitLevel(const uint32_t *levelNodes, uint32_t count, uint32_
 	const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
 	if (e >= count) return;
 	const uint32_t idx = levelNodes[e];
-	uint32_t child[4];
-	int n = 0;
+	uint32_t child[8];
+	for (int c = 0; c < 8; ++c) child[c] = LRH_CHILD_EMPTY;
+	int n = 0, width = 4;
 	if (layout == 0) {
 		const lrh_node4q *q = reinterpret_cast<const lrh_node4q *>(nodes) + idx;
 		n = q->nchild;
 		for (int c = 0; c < 4; ++c) child[c] = q->child[c];
+	} else if (layout == 2) {
+		const lrh_node8q *q = reinterpret_cast<const lrh_node8q *>(nodes) + idx;
+		n = q->nchild;
+		width = 8;
+		for (int c = 0; c < 8; ++c) child[c] = q->child[c];
 	} else {
 		const lrh_node4f *f = reinterpret_cast<const lrh_node4f *>(nodes) + idx;
 		for (int c = 0; c < 4; ++c) {
@@ -200,8 +206,8 @@ __global__ void k_refitLevel(const uint32_t *levelNodes, uint32_t count, uint32_
 			if (child[c] != LRH_CHILD_EMPTY) n = c + 1;
 		}
 	}
-	if (n > 4) n = 4;
-	GBox cb[4];
+	if (n > width) n = width;
+	GBox cb[8];
 	GBox u;
 	for (int k = 0; k < 3; ++k) { u.lo[k] = INFINITY; u.hi[k] = -INFINITY; }
 	for (int c = 0; c < n; ++c) {
@@ -226,6 +232,8 @@ __global__ void k_refitLevel(const uint32_t *levelNodes, uint32_t count, uint32_
 	}
 	if (layout == 0)
 		quantizeNode(cb, n, child, reinterpret_cast<lrh_node4q *>(nodes) + idx);
+	else if (layout == 2)
+		quantizeNode8(cb, n, child, reinterpret_cast<lrh_node8q *>(nodes) + idx);
 	else
 		floatNode(cb, n, child, reinterpret_cast<lrh_node4f *>(nodes) + idx);
 }

@@ -654,6 +654,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void l
 	traceBody<2, false, false, false, true>(P);
 }
 
+// two-level traces over 8-wide nodes: the compiler's choice is 105 VGPRs
+// (4 waves per SIMD); built for 5 (94 VGPRs, no spills; at 6 it spills 10)
+template <bool ANY, bool BATCH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void lrh_trace28_kernel(
+		LrhTraceParams P) {
+	traceBody<2, true, ANY, false, BATCH>(P);
+}
+
 template <bool ANY, bool COUNT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void lrh_trace2q_kernel(LrhTraceParams P) {
 	traceBody<0, true, ANY, COUNT, false>(P);
@@ -674,13 +682,15 @@ typedef void (*KernelFn)(LrhTraceParams);
 	                         : (const void *)&lrh_trace2_kernel<L, A, C, B>)                  \
 	     : (const void *)&lrh_trace_kernel<L, A, C, B>)
 static const void *kernelPtr(int layout, bool two, bool any, bool count, bool batch) {
-	// 8-wide nodes: single-level scenes (plain and batched)
+	// 8-wide nodes
 	static const void *wide8[2][2] = {{LRH_K(2, false, false, false, false), LRH_K(2, false, false, true, false)},
 			{LRH_K(2, false, true, false, false), LRH_K(2, false, true, true, false)}};
-	static const void *wide8Batched = (const void *)&lrh_trace8b_kernel;
+	static const void *wide8Two[2][2] = {
+			{(const void *)&lrh_trace28_kernel<false, false>, (const void *)&lrh_trace2_kernel<2, false, true, false>},
+			{(const void *)&lrh_trace28_kernel<true, false>, (const void *)&lrh_trace2_kernel<2, true, true, false>}};
 	if (layout == 2) {
-		if (two) return nullptr;
-		return batch ? wide8Batched : wide8[any ? 1 : 0][count ? 1 : 0];
+		if (batch) return two ? (const void *)&lrh_trace28_kernel<false, true> : (const void *)&lrh_trace8b_kernel;
+		return two ? wide8Two[any ? 1 : 0][count ? 1 : 0] : wide8[any ? 1 : 0][count ? 1 : 0];
 	}
 	static const void *plain[2][2][2][2] = {
 		{{{LRH_K(0, false, false, false, false), LRH_K(0, false, false, true, false)},

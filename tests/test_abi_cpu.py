@@ -211,11 +211,12 @@ def test_split_triangles_cover_their_triangles(layout, budget):
     assert h.info()["primCount"] == g.ntris
 
 
-def test_host_builder_two_level():
+@pytest.mark.parametrize("layout", [0, 2])
+def test_host_builder_two_level(layout):
     sc = scenes.instanced_scene(n_inst=30, motion_frac=0.3, freq=6, extent=10.0, seed=2)
-    h = D.host_scene_mbvh(sc)
+    h = D.host_scene_mbvh(sc, layout=layout)
     nb, pb = h.download()
-    nodes = nb.view(NODE_Q)
+    nodes = nb.view(node_dtype(layout))
     ntri = sc.meshes[0].ntris
     tris = pb[: ntri * 48].view(TRI)
     insts = pb[ntri * 48:].view(INST)
@@ -234,7 +235,7 @@ def test_host_builder_two_level():
     # root tree occupies the first slots; leaf trees follow
     dec_ok = True
     try:
-        check_tree(nodes, boxes_of_child, 0, nodes.shape[0], 30, 0)
+        check_tree(nodes, boxes_of_child, 0, nodes.shape[0], 30, layout)
     except AssertionError:
         dec_ok = False
     assert dec_ok
@@ -268,9 +269,8 @@ def test_object_boxes_equal_oracle_leaf_boxes():
 
 def test_documented_limits_fail_cleanly():
     """Size limits are checked before anything touches a device: 2^31-1 rays
-    per trace, query 0/1, maxLeaf 1..8 (0: default), layouts 0..2 (8-wide
-    layout 2 single-level only); updates and refits need a two-level device
-    scene with the same objects."""
+    per trace, query 0/1, maxLeaf 1..8 (0: default), layouts 0..2; updates
+    and refits need a two-level device scene with the same objects."""
     L = D.lib()
     host = D.host_scene_bvh(scenes.plain_scene(n_random=10))
     fake = ctypes.c_void_p(1 << 20)
@@ -285,8 +285,6 @@ def test_documented_limits_fail_cleanly():
         with pytest.raises(D.LrhError):
             D.host_scene_bvh(scenes.plain_scene(n_random=10), **bad)
     sc = scenes.instanced_scene(n_inst=6, motion_frac=0.5, freq=4, extent=5.0, seed=3)
-    with pytest.raises(D.LrhError, match="two-level"):
-        D.host_scene_mbvh(sc, layout=2)
     two = D.host_scene_mbvh(sc)
     with pytest.raises(D.LrhError):
         two.object_boxes(5)  # wrong object count

@@ -79,7 +79,7 @@ def test_motion_bar_sees_a_one_ulp_sine(gpu_device, full_instanced):
     assert_parity(compare(ref[static], got[static]))
 
 
-@pytest.mark.parametrize("layout", [0, 1])
+@pytest.mark.parametrize("layout", [0, 1, 2])
 def test_dense_instances_and_motion(gpu_device, layout):
     _case("dense", layout=layout)
     sc = scenes.instanced_scene(n_inst=60, motion_frac=0.5, freq=20, extent=8.0, seed=21)
@@ -108,12 +108,13 @@ def test_plain_object_and_shared_mesh(gpu_device):
     assert st["hits"] > 10000
 
 
-def test_update_moves_instances(gpu_device):
+@pytest.mark.parametrize("layout", [0, 2])
+def test_update_moves_instances(gpu_device, layout):
     """MBVHAccel::Update: new matrices for the same objects; results follow."""
-    _case("update")
+    _case("update", layout=layout)
     sc = scenes.instanced_scene(n_inst=40, motion_frac=0.25, freq=10, extent=8.0, seed=31)
     rays = scenes.instanced_rays(100000, extent=8.0, seed=32)
-    dev_sc = gpu_device.scene_mbvh(sc)
+    dev_sc = gpu_device.scene_mbvh(sc, layout=layout)
     _check(oracle.OracleMBVH(sc), gpu_device, dev_sc, rays)
     sc2 = scenes.instanced_scene(n_inst=40, motion_frac=0.25, freq=10, extent=8.0, seed=33)
     sc2.obj_kind[:] = sc.obj_kind

@@ -57,7 +57,7 @@ def _check_trace(dev, dev_scene, sc, rays):
     return st
 
 
-@pytest.mark.parametrize("layout", [0, 1])
+@pytest.mark.parametrize("layout", [0, 1, 2])
 def test_refit_boxes_and_trace(gpu_device, layout):
     _case("refit", layout=layout)
     sc = scenes.instanced_scene(n_inst=400, motion_frac=0.3, freq=20, extent=40.0, seed=31)

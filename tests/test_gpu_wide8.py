@@ -107,15 +107,18 @@ def test_wide8_noalloc_equals_trace(gpu_device):
 
 
 def test_wide8_batched_equals_plain_traces(gpu_device):
-    """A batch over 8-wide scenes (its own kernel, built for 6 waves) gives
-    each buffer the hits of lrh_trace on it, closest and any hit; a batch
-    mixing node layouts is refused."""
+    """A batch over 8-wide scenes, single- and two-level, gives each buffer
+    the hits of lrh_trace on it, closest and any hit; a batch mixing node
+    layouts is refused."""
     _case("wide8_batched")
     meshes = [scenes.plain_scene(seed=1, n_random=3000), scenes.heightfield_scene(cells=128, n_spheres=2, seed=3)]
     scs = [gpu_device.scene_bvh(m, layout=2) for m in meshes]
+    inst = scenes.instanced_scene(n_inst=40, motion_frac=0.25, freq=10, extent=8.0, seed=31)
+    scs.append(gpu_device.scene_mbvh(inst, layout=2))  # two-level: the batch runs the two-level kernel
     items, refs = [], []
-    for i, (si, n, q) in enumerate(((0, 30000, 0), (1, 20000, 1), (1, 45000, 0), (0, 7000, 1))):
-        rays = scenes.plain_rays(n, seed=70 + i)
+    for i, (si, n, q) in enumerate(((0, 30000, 0), (1, 20000, 1), (1, 45000, 0), (0, 7000, 1), (2, 25000, 0),
+                                    (2, 9000, 1))):
+        rays = scenes.plain_rays(n, seed=70 + i) if si < 2 else scenes.instanced_rays(n, extent=8.0, seed=70 + i)
         rb = gpu_device.upload(rays)
         hb = gpu_device.alloc(n * HIT_DTYPE.itemsize)
         items.append((scs[si], rb, hb, n, q))
