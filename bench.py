@@ -135,16 +135,19 @@ def cpu_baseline(meshes, rays, target_s):
             "host": host_info()}
 
 
-def two_level_bytes(c):
+NODE_READ_BYTES = {0: 64, 1: 128, 2: 96}  # bytes a node visit reads, per layout
+
+
+def two_level_bytes(c, layout=0):
     """Algorithmic bytes per ray of a two-level trace from the counting kernel:
-    Ray + RayHit, 64-byte node visits, 48-byte triangle records, and per
-    object entry its 16-byte instance record plus the 64-byte matrix (static
-    instance) or the 32-byte motion record and the 112 bytes of the segment
-    a sample reads (motion)."""
+    Ray + RayHit, node visits (64 bytes 4-wide, 96 read of a 128-byte 8-wide
+    node), 48-byte triangle records, and per object entry its 16-byte
+    instance record plus the 64-byte matrix (static instance) or the 32-byte
+    motion record and the 112 bytes of the segment a sample reads (motion)."""
     rays = max(c["rays"], 1)
     tris = c["prim_tests"] - c["object_entries"]
-    b = (64 * c["node_visits"] + 48 * tris + 16 * c["object_entries"] + 64 * c["matrix_entries"]
-         + (32 + 112) * c["motion_entries"])
+    b = (NODE_READ_BYTES[layout] * c["node_visits"] + 48 * tris + 16 * c["object_entries"]
+         + 64 * c["matrix_entries"] + (32 + 112) * c["motion_entries"])
     return RAY_BYTES + HIT_BYTES + b / rays
 
 
@@ -171,17 +174,17 @@ def extra_workloads(args, dev):
     # -- two level: 1,000 instances of a 50,000-triangle sphere, 200 with 2-key motion
     t0 = time.time()
     isc = scenes.instanced_scene()
-    sc = dev.scene_mbvh(isc)
+    sc = dev.scene_mbvh(isc, layout=2)
     rays = scenes.instanced_rays(1 << 22)
     n = rays.shape[0]
     rb = dev.upload(rays)
     hb = dev.alloc(n * HIT_DTYPE.itemsize)
     tl = {"workload": "1,000 instances of a 50,000-tri displaced sphere (T*R*S, scale 0.5-2, 200^3 box), 200 "
-                      "with 2-key motion; 4,194,304 rays uniform in the box, time ~ U[0,1]", "rays": n}
+                      "with 2-key motion; 4,194,304 rays uniform in the box, time ~ U[0,1]; 8-wide nodes", "rays": n}
     for name, q in (("closest", 0), ("any", 1)):
         c = dev.trace_counters(sc, rb, hb, n, q)
         wall, lat = _event_timed(dev, lambda: dev.trace(sc, rb, hb, n, q), steps, 3)
-        tl[name] = _rate_line(n, wall, lat, steps, two_level_bytes(c))
+        tl[name] = _rate_line(n, wall, lat, steps, two_level_bytes(c, 2))
         tl[name]["per_ray"] = {k: round(c[k] / max(c["rays"], 1), 3)
                                for k in ("node_visits", "prim_tests", "object_entries", "motion_entries")}
     out["two_level"] = tl
@@ -412,7 +415,7 @@ def run_refit(args, dev):
     sc = scenes.instanced_scene()
     rays = scenes.instanced_rays(1 << 22, seed=78)
     edits = [_moved(sc, rng, 100.0) for _ in range(6)]
-    scene = dev.scene_mbvh(sc)
+    scene = dev.scene_mbvh(sc, layout=args.layout)
     n = rays.shape[0]
     rb = dev.upload(rays)
     hb = dev.alloc(n * HIT_DTYPE.itemsize)
@@ -537,7 +540,7 @@ def run_configs(args, dev):
     if "plain" in sets:
         _configs_plain(args, dev, rate, cpu, out)
     if "heightfield" in sets:
-        _configs_heightfield(dev, rate, out)
+        _configs_heightfield(args, dev, rate, out)
     if "instanced" in sets:
         _configs_instanced(args, dev, rate, cpu, out)
     value = out["heightfield"]["bounce_closest"]["mrays"] if "heightfield" in out else None
@@ -552,7 +555,7 @@ def _configs_plain(args, dev, rate, cpu, out):
     import oracle  # the checker, timed here only as the CPU baseline
     threads = os.cpu_count() or 1
     meshes = scenes.plain_scene()
-    sc = dev.scene_bvh(meshes)
+    sc = dev.scene_bvh(meshes, layout=args.layout)
     rays = scenes.plain_rays(1 << 20)
     out["plain"] = {"triangles": sum(m.ntris for m in meshes), "closest": rate(sc, rays, 0),
                     "any": rate(sc, rays, 1)}
@@ -562,9 +565,9 @@ def _configs_plain(args, dev, rate, cpu, out):
     log(f"plain: {out['plain']}")
 
 
-def _configs_heightfield(dev, rate, out):
+def _configs_heightfield(args, dev, rate, out):
     meshes = scenes.heightfield_scene()
-    sc = dev.scene_bvh(meshes)
+    sc = dev.scene_bvh(meshes, layout=args.layout)
     prim = scenes.pinhole_rays(1920, 1080)
     h = dev.trace_numpy(sc, prim)
     bounce = scenes.bounce_rays(meshes, prim, h, seed=5)
@@ -580,7 +583,7 @@ def _configs_instanced(args, dev, rate, cpu, out):
     import oracle  # the checker, timed here only as the CPU baseline
     threads = os.cpu_count() or 1
     isc = scenes.instanced_scene()
-    sc = dev.scene_mbvh(isc)
+    sc = dev.scene_mbvh(isc, layout=args.layout)
     rays = scenes.instanced_rays(1 << 22)
     out["instanced"] = {"objects": int(len(isc.obj_kind)), "motion_objects": int((isc.obj_kind == 2).sum()),
                         "closest": rate(sc, rays, 0), "any": rate(sc, rays, 1)}
@@ -702,8 +705,7 @@ def main():
     ap.add_argument("--cells", type=int, default=3240)
     ap.add_argument("--layout", type=int, default=None, choices=(0, 1, 2),
                     help="node layout: 0 4-wide 64B quantized, 1 4-wide 128B fp32, 2 8-wide 128B quantized "
-                         "(default: 2 for the headline over the host SAH tree and the batched and served modes, "
-                         "else 0; two-level scenes take 0 or 1)")
+                         "(default 2)")
     ap.add_argument("--max-leaf", type=int, default=4)
     ap.add_argument("--builder", choices=("sah", "lbvh", "ploc"), default="sah",
                     help="headline tree: host binned SAH (default), or a device build: LBVH (radix tree) or "
@@ -733,8 +735,7 @@ def main():
     if args.steps is None:
         args.steps = 1000 if args.mode == "headline" else 20
     if args.layout is None:
-        args.layout = 2 if (args.mode == "headline" and args.builder == "sah") or args.mode in ("batched", "served") \
-            else 0
+        args.layout = 2
     dev = D.Device(0)
     log(f"device {dev.name}, {dev.cu_count} CUs")
     modes = {"batched": run_batched, "scene": run_scene, "refit": run_refit, "configs": run_configs,
@@ -758,7 +759,7 @@ def main():
     nodes, prims, counted = dev.trace_stats(scene, rb, hb, n, 0, use_opts)
     # bytes a node visit reads: 64-byte 4-wide record, 128-byte fp32 record,
     # the 96 bytes of a 128-byte 8-wide record that hold its data
-    node_bytes = {0: 64, 1: 128, 2: 96}[args.layout]
+    node_bytes = NODE_READ_BYTES[args.layout]
     b_ray = RAY_BYTES + HIT_BYTES + node_bytes * nodes / counted + 48 * prims / counted
     log(f"per ray: {nodes / counted:.2f} node visits, {prims / counted:.2f} triangle tests, "
         f"B_ray = {b_ray:.1f} bytes")
