@@ -125,28 +125,50 @@ __global__ __launch_bounds__(kSortBlock) void k_sortCount(const uint64_t *keys, 
 }
 
 // ---- radix sort pass: stable scatter.  A tile is taken in rounds of 256
-// consecutive keys, one per thread in index order; a key's place is its
-// digit's table offset + the tile's earlier keys of that digit (earlier
-// rounds) + the earlier waves' keys of that digit in this round + its rank
-// among the lanes of its wave with the same digit (ballots over the eight
-// digit bits). ----
+// consecutive keys, one per thread in index order; a key's place within the
+// tile is its digit's start in the tile (a scan of the tile's digit counts)
+// + the tile's earlier keys of that digit (earlier rounds) + the earlier
+// waves' keys of that digit in this round + its rank among the lanes of its
+// wave with the same digit (ballots over the eight digit bits).  Keys and
+// values are first placed in LDS in that order, then written out in place
+// order, so consecutive threads write consecutive addresses of one digit's
+// run (its global start from the scanned [digit][tile] table). ----
 __global__ __launch_bounds__(kSortBlock) void k_sortScatter(const uint64_t *keysIn, const uint32_t *valsIn, uint32_t n,
 		uint32_t shift, uint32_t nTiles, const uint32_t *offsets, uint64_t *keysOut, uint32_t *valsOut, uint32_t *err) {
 	constexpr uint32_t kWaves = kSortBlock / 64;
 	__shared__ uint32_t run[256];
 	__shared__ uint32_t wcnt[kWaves][256];
-	__shared__ uint32_t off[256];
+	__shared__ uint32_t off[256];    // global start of each digit's run for this tile
+	__shared__ uint32_t start[256];  // each digit's start within the tile
+	__shared__ uint64_t sKey[kSortTile];
+	__shared__ uint32_t sVal[kSortTile];
 	const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
+	const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+	const uint32_t tileN = (uint32_t)((n - base) < kSortTile ? (n - base) : kSortTile);
+	// the tile's digit counts, scanned to starts
 	run[tid] = 0;
 	for (uint32_t w = 0; w < kWaves; ++w) wcnt[w][tid] = 0;
 	off[tid] = offsets[(size_t)tid * nTiles + blockIdx.x];
 	__syncthreads();
+	for (uint32_t j = tid; j < tileN; j += kSortBlock) atomicAdd(&run[digitOf(keysIn[base + j], shift)], 1u);
+	__syncthreads();
+	start[tid] = run[tid];
+	__syncthreads();
+	for (uint32_t o = 1; o < 256; o <<= 1) {  // inclusive scan (Hillis-Steele)
+		const uint32_t add = tid >= o ? start[tid - o] : 0u;
+		__syncthreads();
+		start[tid] += add;
+		__syncthreads();
+	}
+	start[tid] -= run[tid];  // exclusive
+	run[tid] = 0;
+	__syncthreads();
 	const uint64_t lanesBelow = (1ull << lane) - 1ull;
-	const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
 	for (uint32_t r = 0; r < kSortTile / kSortBlock; ++r) {
-		const uint64_t idx = base + (uint64_t)r * kSortBlock + tid;
-		const bool valid = idx < n;
-		const uint64_t key = valid ? keysIn[idx] : 0ull;
+		const uint32_t j = r * kSortBlock + tid;
+		const bool valid = j < tileN;
+		const uint64_t key = valid ? keysIn[base + j] : 0ull;
+		const uint32_t val = valid ? valsIn[base + j] : 0u;
 		const uint32_t dg = valid ? digitOf(key, shift) : 0u;
 		uint64_t same = __ballot(valid);
 		for (uint32_t b = 0; b < 8; ++b) {
@@ -160,12 +182,12 @@ __global__ __launch_bounds__(kSortBlock) void k_sortScatter(const uint64_t *keys
 		if (valid) {
 			uint32_t before = run[dg];
 			for (uint32_t w = 0; w < wave; ++w) before += wcnt[w][dg];
-			const uint32_t pos = off[dg] + before + rank;
-			if (pos < n) {
-				keysOut[pos] = key;
-				valsOut[pos] = valsIn[idx];
+			const uint32_t p = start[dg] + before + rank;
+			if (p < kSortTile) {
+				sKey[p] = key;
+				sVal[p] = val;
 			} else {
-				atomicOr(err, kErrUnsorted);  // a place outside the array: never written
+				atomicOr(err, kErrUnsorted);  // a place outside the tile: never written
 			}
 		}
 		__syncthreads();
@@ -176,6 +198,19 @@ __global__ __launch_bounds__(kSortBlock) void k_sortScatter(const uint64_t *keys
 		}
 		run[tid] += add;
 		__syncthreads();
+	}
+	// out in place order: one digit's run is contiguous in the tile and in
+	// the output
+	for (uint32_t p = tid; p < tileN; p += kSortBlock) {
+		const uint64_t key = sKey[p];
+		const uint32_t dg = digitOf(key, shift);
+		const uint32_t pos = off[dg] + (p - start[dg]);
+		if (pos < n) {
+			keysOut[pos] = key;
+			valsOut[pos] = sVal[p];
+		} else {
+			atomicOr(err, kErrUnsorted);
+		}
 	}
 }
 
