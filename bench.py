@@ -269,8 +269,23 @@ def run_batched(args, dev):
     total = wl.total_rays()
     log(f"batched workload: {sum(m[0].ntris for m in meshes)} tris in 8 scenes, {len(bufs)} buffers, "
         f"{total} rays ({time.time() - t0:.1f}s)")
+    build, build_table = {"layout": args.layout, "max_leaf": args.max_leaf}, None
+    if not args.layout_given:
+        # node layout and leaf size from the autotuner: each candidate build of
+        # the 8 scenes timed over the same buffers with its best launch shape
+        def make_items(scs):
+            return [(scs[si], wl.ray_bufs[i], wl.hit_bufs[i], c, q) for i, (si, c, q) in enumerate(bufs)]
+        best, build_table = B.tune_build(dev, meshes, make_items, builds=((0, 4), (1, 4), (2, 4), (2, 2)),
+                                         shapes=[(256, 8, 8), (128, 16, 16), (256, 8, 12), (128, 8, 12)], log=log)
+        build = {"layout": best["layout"], "max_leaf": best["max_leaf"]}
+        if (build["layout"], build["max_leaf"]) != (args.layout, args.max_leaf):
+            for sc in wl.scenes:
+                sc.close()
+            wl.scenes = [dev.scene_bvh(m, layout=build["layout"], max_leaf=build["max_leaf"]) for m in meshes]
+        log(f"autotuned build: layout {build['layout']} max_leaf {build['max_leaf']}")
     items = wl.items()
-    key = B.signature(dev, [m[0].ntris for m in meshes], [c for _, c, _ in bufs], f"batched-v3-layout{args.layout}")
+    key = B.signature(dev, [m[0].ntris for m in meshes], [c for _, c, _ in bufs],
+                      f"batched-v3-layout{build['layout']}-leaf{build['max_leaf']}")
     opts, table = B.autotune(dev, items, key=key, log=log)
     log(f"autotuned shape: block {opts.blockSize} lds {opts.ldsStack} refill {opts.refillThreshold}")
     # plain: one lrh_trace per buffer, same launch shape
@@ -309,7 +324,9 @@ def run_batched(args, dev):
             "config": {"workload": "8 scenes 10k..5M tris (soups + sphere grids), 64 buffers log-uniform 4Ki..4Mi "
                                    "rays, mixed closest/any", "rays_per_step": total,
                        "shape": {"block": opts.blockSize, "lds": opts.ldsStack, "refill": opts.refillThreshold,
-                                 "blocks": b.blocks}},
+                                 "blocks": b.blocks},
+                       "build": build},
+            "build_autotune": build_table,
             "plain_mrays": round(plain, 2), "batched_over_plain": round(value / plain, 3),
             "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                            "p99": round(float(np.percentile(lat, 99)), 4), "per": "one batched launch"}}
@@ -734,6 +751,7 @@ def main():
 
     if args.steps is None:
         args.steps = 1000 if args.mode == "headline" else 20
+    args.layout_given = args.layout is not None
     if args.layout is None:
         args.layout = 2
     dev = D.Device(0)

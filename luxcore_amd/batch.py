@@ -78,9 +78,11 @@ def autotune(dev, items, shapes=None, cache_path=DEFAULT_CACHE, key=None, log=No
     return D.trace_opts(best.blockSize, best.ldsStack, 0, best.refillThreshold), table
 
 
-def tune_build(dev, scene_meshes, make_items, builds=((0, 4), (0, 2), (1, 4)), shapes=None, log=None):
-    """Times each (layout, max_leaf) build with its best launch shape;
-    make_items(scenes) -> batch items.  Returns (best build, results)."""
+def tune_build(dev, scene_meshes, make_items, builds=((0, 4), (1, 4), (2, 4), (2, 2)), shapes=None, log=None):
+    """Times each (layout, max_leaf) build -- node layout 0 (4-wide
+    quantized), 1 (4-wide fp32) or 2 (8-wide quantized), leaf size -- with
+    its best launch shape; make_items(scenes) -> batch items.  Returns (best
+    build, results)."""
     shapes = shapes or [(256, 8, 8), (256, 8, 12), (256, 4, 12), (128, 8, 12)]
     res = []
     for layout, leaf in builds:

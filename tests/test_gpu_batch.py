@@ -153,6 +153,23 @@ def test_autotuner_picks_and_caches(gpu_device, small_batch, tmp_path):
         assert sorted(ln.split()[0] for ln in f if ln.strip()) == ["k1", "k2"]
 
 
+def test_build_autotuner_times_every_layout(gpu_device, small_batch):
+    """The node layout and leaf size are autotuned too: each candidate build
+    of the scenes is timed over the same buffers with its best launch shape,
+    and the fastest is returned."""
+    meshes, wl = small_batch
+    _case("tune_build")
+
+    def make_items(scs):
+        return [(scs[si], wl.ray_bufs[i], wl.hit_bufs[i], c, q) for i, (si, c, q) in enumerate(wl.buffers)]
+    builds = ((0, 4), (1, 4), (2, 4), (2, 2))
+    best, res = B.tune_build(gpu_device, meshes, make_items, builds=builds, shapes=[(256, 8, 8), (128, 16, 16)])
+    assert [(r["layout"], r["max_leaf"]) for r in res] == list(builds)
+    assert all(r["ms"] > 0 for r in res)
+    assert best == min(res, key=lambda r: r["ms"])
+    print(res, flush=True)
+
+
 def _more_keys(sc, n_keys=4, seed=71):
     """The same objects with n_keys motion keys each (the arrays of motion
     systems grow, so an edit with them moves the scene's device arrays)."""
