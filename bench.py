@@ -273,9 +273,8 @@ def run_batched(args, dev):
     if not args.layout_given:
         # node layout and leaf size from the autotuner: each candidate build of
         # the 8 scenes timed over the same buffers with its best launch shape
-        def make_items(scs):
-            return [(scs[si], wl.ray_bufs[i], wl.hit_bufs[i], c, q) for i, (si, c, q) in enumerate(bufs)]
-        best, build_table = B.tune_build(dev, meshes, make_items, builds=((0, 4), (1, 4), (2, 4), (2, 2)),
+        tune_items = [(si, wl.ray_bufs[i], wl.hit_bufs[i], c, q) for i, (si, c, q) in enumerate(bufs)]
+        best, build_table = B.tune_build(dev, meshes, tune_items, builds=((0, 4), (1, 4), (2, 4), (2, 2)),
                                          shapes=[(256, 8, 8), (128, 16, 16), (256, 8, 12), (128, 8, 12)], log=log)
         build = {"layout": best["layout"], "max_leaf": best["max_leaf"]}
         if (build["layout"], build["max_leaf"]) != (args.layout, args.max_leaf):

@@ -442,6 +442,20 @@ int lrh_server_destroy(lrh_server *srv);
 int lrh_autotune_batch(lrh_device *dev, const lrh_batch_item *items, unsigned int nItems,
 		const lrh_trace_opts *candidates, unsigned int nCandidates, const char *cachePath, const char *key,
 		lrh_trace_opts *best, float *bestMs, float *candidateMs);
+/* Build autotuner of a batched trace: for each candidate build option set
+ * (node layout, leaf size, ...) builds the nScenes single-level scenes from
+ * their meshes (scene j: sceneMeshes[j][0 .. sceneMeshCount[j])), runs
+ * lrh_autotune_batch over the items with each item's scene replaced by the
+ * candidate's build of scene itemScene[i] (the items' own scene fields are
+ * ignored; shapes / nShapes as lrh_autotune_batch's candidates, no cache),
+ * releases the scenes, and keeps the fastest candidate: bestCandidate gets
+ * its index, bestShape its launch shape, candidateMs (optional, one per
+ * candidate) each candidate's best time.  (No reference counterpart: the
+ * reference's tree layout is fixed, bvhaccel.cpp:144-156.) */
+int lrh_autotune_build(lrh_device *dev, const lrh_mesh *const *sceneMeshes, const unsigned int *sceneMeshCount,
+		unsigned int nScenes, const lrh_batch_item *items, const unsigned int *itemScene, unsigned int nItems,
+		const lrh_build_opts *candidates, unsigned int nCandidates, const lrh_trace_opts *shapes,
+		unsigned int nShapes, unsigned int *bestCandidate, lrh_trace_opts *bestShape, float *candidateMs);
 
 /* 1 if all work before the event has completed, 0 if not yet, < 0 on error. */
 int lrh_event_query(lrh_event *ev);

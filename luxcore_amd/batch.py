@@ -78,24 +78,40 @@ def autotune(dev, items, shapes=None, cache_path=DEFAULT_CACHE, key=None, log=No
     return D.trace_opts(best.blockSize, best.ldsStack, 0, best.refillThreshold), table
 
 
-def tune_build(dev, scene_meshes, make_items, builds=((0, 4), (1, 4), (2, 4), (2, 2)), shapes=None, log=None):
-    """Times each (layout, max_leaf) build -- node layout 0 (4-wide
-    quantized), 1 (4-wide fp32) or 2 (8-wide quantized), leaf size -- with
-    its best launch shape; make_items(scenes) -> batch items.  Returns (best
-    build, results)."""
+def tune_build(dev, scene_meshes, items, builds=((0, 4), (1, 4), (2, 4), (2, 2)), shapes=None, log=None):
+    """Build autotuner (lrh_autotune_build, in the library): each (layout,
+    max_leaf) build -- node layout 0 (4-wide quantized), 1 (4-wide fp32) or
+    2 (8-wide quantized), leaf size -- of the scenes is timed over the items
+    with its best launch shape.  items: (scene index, ray buffer, hit buffer,
+    count, query).  Returns (best build, results)."""
     shapes = shapes or [(256, 8, 8), (256, 8, 12), (256, 4, 12), (128, 8, 12)]
-    res = []
-    for layout, leaf in builds:
-        sc = [dev.scene_bvh(m, layout=layout, max_leaf=leaf) for m in scene_meshes]
-        items = make_items(sc)
-        opts, table = autotune(dev, items, shapes=shapes, cache_path=None)
-        best = min(table, key=lambda r: r["ms"])
-        res.append({"layout": layout, "max_leaf": leaf, **best})
-        if log:
-            log(f"build layout {layout} leaf {leaf}: {best}")
-        for s in sc:
-            s.close()
-    return min(res, key=lambda r: r["ms"]), res
+    keep = []
+    mesh_arrs = (ctypes.POINTER(D.LrhMesh) * len(scene_meshes))()
+    counts = (ctypes.c_uint * len(scene_meshes))()
+    for j, meshes in enumerate(scene_meshes):
+        arr, k = D._mesh_structs(meshes)
+        keep += [arr, k]
+        mesh_arrs[j] = ctypes.cast(arr, ctypes.POINTER(D.LrhMesh))
+        counts[j] = len(meshes)
+    arr = D._batch_items([(None, rb, hb, c, q) for (_, rb, hb, c, q) in items])
+    idx = (ctypes.c_uint * len(items))(*[si for (si, _, _, _, _) in items])
+    cand = (D.LrhBuildOpts * len(builds))()
+    for i, (layout, leaf) in enumerate(builds):
+        cand[i] = D._build_opts(layout=layout, max_leaf=leaf)
+    sh = (D.LrhTraceOpts * len(shapes))()
+    for i, (b, l, r) in enumerate(shapes):
+        sh[i] = D.trace_opts(b, l, 0, r)
+    best_c = ctypes.c_uint()
+    best_o = D.LrhTraceOpts()
+    ms = (ctypes.c_float * len(builds))()
+    D.check(D.lib().lrh_autotune_build(dev.h, mesh_arrs, counts, len(scene_meshes), arr, idx, len(items), cand,
+                                       len(builds), sh, len(shapes), ctypes.byref(best_c), ctypes.byref(best_o), ms))
+    res = [{"layout": layout, "max_leaf": leaf, "ms": float(ms[i])} for i, (layout, leaf) in enumerate(builds)]
+    if log:
+        for r in res:
+            log(f"build layout {r['layout']} leaf {r['max_leaf']}: {r['ms']:.3f} ms")
+    best = dict(res[best_c.value], block=best_o.blockSize, lds=best_o.ldsStack, refill=best_o.refillThreshold)
+    return best, res
 
 
 class BatchedWorkload:

@@ -855,4 +855,55 @@ int lrh_autotune_batch(lrh_device *dev, const lrh_batch_item *items, unsigned in
 	});
 }
 
+int lrh_autotune_build(lrh_device *dev, const lrh_mesh *const *sceneMeshes, const unsigned int *sceneMeshCount,
+		unsigned int nScenes, const lrh_batch_item *items, const unsigned int *itemScene, unsigned int nItems,
+		const lrh_build_opts *candidates, unsigned int nCandidates, const lrh_trace_opts *shapes,
+		unsigned int nShapes, unsigned int *bestCandidate, lrh_trace_opts *bestShape, float *candidateMs) {
+	if (!dev || !sceneMeshes || !sceneMeshCount || !nScenes || !items || !itemScene || !nItems || !candidates ||
+			!nCandidates || !bestCandidate || !bestShape)
+		return lrhSetError(LRH_EINVAL, "null argument");
+	if (shapes && !nShapes) return lrhSetError(LRH_EINVAL, "empty shape list");
+	for (unsigned i = 0; i < nItems; ++i)
+		if (itemScene[i] >= nScenes) return lrhSetError(LRH_EINVAL, "item " + std::to_string(i) + ": scene index out of range");
+	return guard([&]() {
+		float bestT = INFINITY;
+		unsigned bestC = 0;
+		lrh_trace_opts bestO{};
+		std::vector<lrh_scene *> scs(nScenes, nullptr);
+		auto release = [&]() {
+			for (auto &sc : scs)
+				if (sc) {
+					(void)lrh_scene_destroy(sc);
+					sc = nullptr;
+				}
+		};
+		for (unsigned c = 0; c < nCandidates; ++c) {
+			if (candidateMs) candidateMs[c] = NAN;
+			for (unsigned j = 0; j < nScenes; ++j) {
+				if (lrh_scene_create_bvh(dev, sceneMeshes[j], sceneMeshCount[j], &candidates[c], &scs[j]) != LRH_OK) {
+					const std::string why = lrh_last_error();
+					release();
+					throw SrvError(LRH_EINVAL, "candidate " + std::to_string(c) + ", scene " + std::to_string(j) + ": " + why);
+				}
+			}
+			std::vector<lrh_batch_item> it(items, items + nItems);
+			for (unsigned i = 0; i < nItems; ++i) it[i].scene = scs[itemScene[i]];
+			lrh_trace_opts o{};
+			float ms = INFINITY;
+			const int rc = lrh_autotune_batch(dev, it.data(), nItems, shapes, shapes ? nShapes : 0, nullptr, nullptr,
+					&o, &ms, nullptr);
+			release();
+			if (rc != LRH_OK) throw SrvError(rc, std::string("candidate ") + std::to_string(c) + ": " + lrh_last_error());
+			if (candidateMs) candidateMs[c] = ms;
+			if (ms < bestT) {
+				bestT = ms;
+				bestC = c;
+				bestO = o;
+			}
+		}
+		*bestCandidate = bestC;
+		*bestShape = bestO;
+	});
+}
+
 }  // extern "C"

@@ -192,6 +192,11 @@ SIGNATURES = {
     "lrh_server_plain_batch": ([_vp, ctypes.POINTER(LrhTraceOpts), ctypes.c_uint, ctypes.POINTER(LrhPlainBatchResult)],
                                ctypes.c_int),
     "lrh_server_destroy": ([_vp], ctypes.c_int),
+    "lrh_autotune_build": ([_vp, ctypes.POINTER(ctypes.POINTER(LrhMesh)), ctypes.POINTER(ctypes.c_uint), ctypes.c_uint,
+                            ctypes.POINTER(LrhBatchItem), ctypes.POINTER(ctypes.c_uint), ctypes.c_uint,
+                            ctypes.POINTER(LrhBuildOpts), ctypes.c_uint, ctypes.POINTER(LrhTraceOpts), ctypes.c_uint,
+                            ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(LrhTraceOpts),
+                            ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
     "lrh_autotune_batch": ([_vp, ctypes.POINTER(LrhBatchItem), ctypes.c_uint, ctypes.POINTER(LrhTraceOpts),
                             ctypes.c_uint, ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(LrhTraceOpts),
                             ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
@@ -421,7 +426,7 @@ def host_scene_mbvh(inst_scene, layout=0, max_leaf=4, threads=0, trav_cost=1.0, 
 def _batch_items(items):
     arr = (LrhBatchItem * max(1, len(items)))()
     for i, (scene, rb, hb, count, query) in enumerate(items):
-        arr[i].scene = scene.h
+        arr[i].scene = scene.h if scene is not None else None
         arr[i].rays = rb.ptr if hasattr(rb, "ptr") else rb
         arr[i].hits = hb.ptr if hasattr(hb, "ptr") else hb
         arr[i].count = count

@@ -50,6 +50,8 @@ def test_library_exports_every_declared_function():
 
 
 def test_bad_arguments_fail_cleanly():
+    assert D.lib().lrh_autotune_build(None, None, None, 0, None, None, 0, None, 0, None, 0, None, None,
+                                      None) == D.LRH_EINVAL
     assert D.lib().lrh_trace(None, None, None, 0, 0, None) == -1
     assert D.lib().lrh_scene_get_info(None, None) == -1
     assert b"null" in D.lib().lrh_last_error()

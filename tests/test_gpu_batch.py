@@ -159,15 +159,19 @@ def test_build_autotuner_times_every_layout(gpu_device, small_batch):
     and the fastest is returned."""
     meshes, wl = small_batch
     _case("tune_build")
-
-    def make_items(scs):
-        return [(scs[si], wl.ray_bufs[i], wl.hit_bufs[i], c, q) for i, (si, c, q) in enumerate(wl.buffers)]
+    items = [(si, wl.ray_bufs[i], wl.hit_bufs[i], c, q) for i, (si, c, q) in enumerate(wl.buffers)]
     builds = ((0, 4), (1, 4), (2, 4), (2, 2))
-    best, res = B.tune_build(gpu_device, meshes, make_items, builds=builds, shapes=[(256, 8, 8), (128, 16, 16)])
+    best, res = B.tune_build(gpu_device, meshes, items, builds=builds, shapes=[(256, 8, 8), (128, 16, 16)])
+    print(best, res, flush=True)
     assert [(r["layout"], r["max_leaf"]) for r in res] == list(builds)
     assert all(r["ms"] > 0 for r in res)
-    assert best == min(res, key=lambda r: r["ms"])
-    print(res, flush=True)
+    fastest = min(res, key=lambda r: r["ms"])
+    assert (best["layout"], best["max_leaf"], best["ms"]) == (fastest["layout"], fastest["max_leaf"], fastest["ms"])
+    assert (best["block"], best["lds"]) in ((256, 8), (128, 16))
+    # a scene index outside the scenes is refused before anything is built
+    bad = [(len(meshes), *items[0][1:])]
+    with pytest.raises(D.LrhError, match="out of range"):
+        B.tune_build(gpu_device, meshes, bad, builds=builds[:1])
 
 
 def _more_keys(sc, n_keys=4, seed=71):
