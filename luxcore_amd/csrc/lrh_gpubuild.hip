@@ -1,24 +1,29 @@
 /*
- * Linear BVH build on the device, written into the 4-wide node records of
- * lrh_layout.h.  The per-element steps are lrh_lbvh.h's (shared with the
- * CPU emulation); here they run as kernels:
+ * BVH builds on the device (linear BVH and PLOC), written into the 4-wide or
+ * 8-wide node records of lrh_layout.h.  The per-element steps are
+ * lrh_lbvh.h's (shared with the CPU emulation); here they run as kernels:
  *   1. leaf boxes and the bounds of their centres;
- *   2. 63-bit Morton codes, sorted with the triangle id as value by a stable
- *      LSD radix sort (8-bit digits: per-tile digit counts, one exclusive
- *      scan of the [digit][tile] table, a scatter that ranks equal digits
- *      in index order with wave ballots), then checked to be in order;
- *   3. binary radix tree over the sorted codes;
- *   4. boxes bottom-up, one launch per depth (depths by pointer jumping),
- *      and surface-area costs, which decide whether a node of at most
- *      maxLeaf triangles becomes one leaf;
- *   5. breadth-first collapse into 4-wide nodes, child slots from an
+ *   2. 63-bit Morton codes over a cube of cells, sorted with the triangle id
+ *      as value by a stable LSD radix sort (8-bit digits: per-tile digit
+ *      counts, one exclusive scan of the [digit][tile] table, a scatter that
+ *      ranks equal digits in index order with wave ballots, stages the tile
+ *      in LDS in sorted order and writes each digit's run coalesced), then
+ *      checked to be in order;
+ *   3. a binary tree over the sorted leaves: the radix tree (LBVH) or
+ *      agglomerative clustering in rounds of mutual nearest neighbours
+ *      within 16 places (PLOC), brought into the radix tree's form;
+ *   4. boxes bottom-up -- LBVH: one launch per depth (depths by pointer
+ *      jumping, nodes ordered by depth); PLOC: one launch per round -- and
+ *      surface-area costs, which decide whether a node of at most maxLeaf
+ *      triangles becomes one leaf;
+ *   5. breadth-first collapse into 4- or 8-wide nodes, child slots from an
  *      exclusive scan, so the node order is deterministic;
  *   6. triangle records in leaf order, bit for bit the host records.
  * Every kernel runs a bounded amount of work: no block waits on another
  * (the sort and the scans are sequences of kernels, not decoupled
  * look-back), every search loop has a step bound, and a violated
- * precondition sets a bit of an error word that the host reads after the
- * sort and after the tree, reporting it as an error instead of going on.
+ * precondition sets a bit of an error word (or fails a host check of a
+ * count read back) that the host reports as an error instead of going on.
  */
 #include <hip/hip_runtime.h>
 
