@@ -1,5 +1,5 @@
 /*
- * The per-element steps of the linear BVH build, shared by the device
+ * The per-element steps of the device BVH builds, shared by the device
  * kernels (lrh_gpubuild.hip) and their sequential CPU emulation (the test
  * suite's check of the build's algorithm and of its error paths):
  *   1. a triangle's leaf box: Triangle::WorldBound expanded by
@@ -8,9 +8,11 @@
  *   2. the 63-bit Morton code of its centre (21 bits per axis);
  *   3. a node of the binary radix tree over the sorted codes (Karras,
  *      "Maximizing Parallelism in the Construction of BVHs, Octrees, and k-d
- *      Trees", HPG 2012), equal codes ordered by position;
- *   4. the expansion of a binary node into up to four children (largest
- *      area first) and the 4-wide record it becomes;
+ *      Trees", HPG 2012), equal codes ordered by position; or the nearest
+ *      neighbour and partner of a cluster in a PLOC round;
+ *   4. a node's surface-area cost and leaf decision, the expansion of a
+ *      binary node into up to four or eight children (largest area first)
+ *      and the wide record it becomes;
  *   5. the triangle record, bit for bit the host builder's.
  * Every search loop is bounded: keys that are not in order make the range
  * search fail (radixNode returns false) instead of looping.

@@ -7,8 +7,8 @@
  * traversing (the reference launches one thread per ray with no refill,
  * include/luxrays/accelerators/bvh.cl:228-260).
  *
- * Traversal is ordered (children sorted by entry distance) over 4-wide
- * nodes, with a per-lane stack whose top `ldsDepth` entries live in LDS and
+ * Traversal is ordered (children sorted by entry distance) over 4-wide or
+ * 8-wide nodes, with a per-lane stack whose top `ldsDepth` entries live in LDS and
  * deeper entries spill to a global scratch array; popped entries whose entry
  * distance is not below the current hit are dropped without a fetch.  The
  * reference walks a skip-pointer array in memory order instead
