@@ -656,7 +656,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void l
 
 // two-level traces over 8-wide nodes: the compiler's choice is 105 VGPRs
 // (4 waves per SIMD); built for 5 (94 VGPRs, no spills; at 6 it spills 10),
-// the batched form (per-lane item view) for 4 (at 5 it spills 27)
+// the batched form (per-lane item view) for 4 (at 5 it spills 37)
 template <bool ANY, bool BATCH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BATCH ? 4 : 5))) void lrh_trace28_kernel(
 		LrhTraceParams P) {
