@@ -232,15 +232,16 @@ def extra_workloads(args, dev):
     w.run(serve.poisson_specs(1000.0, 0.004, len(scs), seed=99))
     w.close()
     specs = serve.poisson_specs(600.0, 0.5, len(scs), seed=600)
-    srv = serve.Server(dev, scs, opts=sopts, max_rays_per_tick=3 << 19)
+    srv = serve.Server(dev, scs, opts=sopts, max_rays_per_tick=2 << 20)
     swall = srv.run(specs, time_limit=120.0)
-    row = serve.served_row(srv, swall, 600.0, 3 << 19)
+    row = serve.served_row(srv, swall, 600.0, 2 << 20)
     srv.close()
     plain = serve.plain_comparison(dev, scs, sopts, n_sessions=16)
     out["served"] = {
         "workload": "Poisson arrivals (600 sessions/s for 0.5 s) over the 8 batched scenes; 512x512 camera + <=8 "
                     "bounces (cosine, RR 0.7 after 2) + 1 shadow ray per bounce, GPU shading (lrh_shade), 2 "
-                    "cohorts, 1.5Mi rays per tick, 4 GiB HBM budget",
+                    "cohorts, 2Mi rays per tick, each tick's launch on 80% of the persistent grid (the "
+                    "server's default), 4 GiB HBM budget",
         "run": row,
         "latency_ms": {"p50": row["p50_ms"], "p99": row["p99_ms"], "samples": row["wavefronts"],
                        "per": "one session wavefront: ready -> completion of the tick that traced it"},

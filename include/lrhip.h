@@ -137,7 +137,10 @@ typedef struct {
 
 /* Tuning of the trace kernel; all zero = built-in defaults.
  * blockSize in {64,128,256}; ldsStack a power of two in 4..64 entries per
- * lane; blocksPerCU 0 = occupancy-derived persistent grid. */
+ * lane; blocksPerCU 0 = occupancy-derived persistent grid, of which a launch
+ * takes gridPercent (1..100, 0 = all of it): a caller whose launches run
+ * beside its own work on other streams (the served cohorts) leaves that work
+ * room to start while a launch drains. */
 typedef struct {
 	unsigned int blockSize;
 	unsigned int ldsStack;
@@ -145,6 +148,7 @@ typedef struct {
 	unsigned int refillThreshold;  /* refill a wave when fewer than this many lanes are live; 0 = default */
 	unsigned int leafTris;         /* triangles a lane tests per traversal pass; the rest of its leaf waits
 	                                  for the next pass; 0 = default (the whole leaf) */
+	unsigned int gridPercent;      /* share of the occupancy-derived grid (blocksPerCU 0); 0 = 100 */
 } lrh_trace_opts;
 
 /* One ray buffer of a batched launch (lrh_trace_batch). */
@@ -373,7 +377,8 @@ typedef struct {
 	unsigned int mode;                /* LRH_SERVE_SHADE (default) or LRH_SERVE_BOUNCE */
 	float extent;                     /* scene box [0, extent]^3 for camera placement, default 100 */
 	float light[3];                   /* light centre, default (extent/2, extent/2, 1.5 extent) */
-	lrh_trace_opts opts;              /* launch shape of the batched trace (zeros: defaults) */
+	lrh_trace_opts opts;              /* launch shape of the batched trace (zeros: defaults; with more than
+	                                     one cohort a zero gridPercent and blocksPerCU become gridPercent 80) */
 } lrh_server_config;
 typedef struct {
 	unsigned int sid;                 /* caller's session id (unique per server) */
@@ -422,7 +427,8 @@ int lrh_server_radiance(lrh_server *srv, unsigned int sid, float *rgb, unsigned 
 /* Copy every ray buffer the server traces into a device arena of maxBytes
  * (call before the first session is admitted), then time the same buffers as
  * one plain lrh_trace_batch and as one lrh_trace each: the served form's
- * throughput next to a single plain batch of the same rays. */
+ * throughput next to a single plain batch of the same rays (opts NULL: the
+ * server's launch shape over the whole grid). */
 int lrh_server_capture(lrh_server *srv, unsigned long long maxBytes);
 int lrh_server_plain_batch(lrh_server *srv, const lrh_trace_opts *opts, unsigned int reps,
 		lrh_plain_batch_result *result);

@@ -1264,6 +1264,8 @@ LaunchShape chooseShape(lrh_device *dev, unsigned layout, bool two, bool any, bo
 	if (L.leafTris > 8) throw Error(LRH_EINVAL, "leafTris must be 0..8");
 	L.ldsBytes = (size_t)L.lds * L.block * sizeof(uint2) + extraLds;
 	if (L.ldsBytes > 65536) throw Error(LRH_ERANGE, "LDS stack, node cache and item table exceed 64 KiB per block");
+	const unsigned pct = (o && o->gridPercent) ? o->gridPercent : 100u;
+	if (pct > 100) throw Error(LRH_EINVAL, "gridPercent must be 0..100");
 	int perCU = (o && o->blocksPerCU) ? (int)o->blocksPerCU : 0;
 	if (!perCU) {
 		const uint64_t key = ((uint64_t)layout << 40) | ((uint64_t)two << 39) | ((uint64_t)any << 38) |
@@ -1277,6 +1279,7 @@ LaunchShape chooseShape(lrh_device *dev, unsigned layout, bool two, bool any, bo
 			perCU = nb > 0 ? nb : 1;
 		} else
 			perCU = it->second;
+		perCU = std::max(1, (int)(((unsigned)perCU * pct + 50u) / 100u));
 	}
 	const unsigned persistent = (unsigned)perCU * (unsigned)dev->cuCount;
 	const unsigned need = (unsigned)(((uint64_t)count + L.block - 1) / L.block);

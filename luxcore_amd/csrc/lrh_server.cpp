@@ -490,6 +490,11 @@ int lrh_server_create(lrh_device *dev, lrh_scene *const *scenes, unsigned int nS
 		if (!c.hbmBudget) c.hbmBudget = 4ull << 30;
 		if (!c.maxRaysPerTick) c.maxRaysPerTick = 3u << 20;
 		if (!c.cohorts) c.cohorts = 2;
+		// each cohort's launch takes 80% of the persistent grid, so the other
+		// cohort's tick starts on the rest and takes over the CUs its waves
+		// leave while it drains (a tick carries about one live ray per lane
+		// of a full grid: its drain is a large part of it)
+		if (!c.opts.gridPercent && !c.opts.blocksPerCU && c.cohorts > 1) c.opts.gridPercent = 80;
 		if (!(c.extent > 0.f)) c.extent = 100.f;
 		if (c.light[0] == 0.f && c.light[1] == 0.f && c.light[2] == 0.f) {
 			c.light[0] = 0.5f * c.extent;
@@ -717,7 +722,10 @@ int lrh_server_plain_batch(lrh_server *s, const lrh_trace_opts *opts, unsigned i
 			chk(lrh_event_elapsed(a, b, &ms), "lrh_event_elapsed");
 			return (double)ms / r;
 		};
-		const lrh_trace_opts *o = opts ? opts : &s->cfg.opts;
+		// default: the served shape over the whole grid (nothing runs beside a plain batch)
+		lrh_trace_opts po = s->cfg.opts;
+		po.gridPercent = 0;
+		const lrh_trace_opts *o = opts ? opts : &po;
 		res->batchMs = timeIt([&]() {
 			chk(lrh_trace_batch(s->dev, items.data(), (unsigned)items.size(), o, nullptr), "lrh_trace_batch");
 		});

@@ -60,7 +60,7 @@ class LrhSceneInfo(ctypes.Structure):
 class LrhTraceOpts(ctypes.Structure):
     _fields_ = [("blockSize", ctypes.c_uint), ("ldsStack", ctypes.c_uint), ("blocksPerCU", ctypes.c_uint),
                 ("refillThreshold", ctypes.c_uint),
-                ("leafTris", ctypes.c_uint)]
+                ("leafTris", ctypes.c_uint), ("gridPercent", ctypes.c_uint)]
 
 
 class LrhCamera(ctypes.Structure):
@@ -344,11 +344,11 @@ def _build_opts(layout=0, max_leaf=4, threads=0, trav_cost=1.0, builder=BUILD_HO
     return o
 
 
-def trace_opts(block=0, lds=0, blocks_per_cu=0, refill=0, leaf_tris=0):
+def trace_opts(block=0, lds=0, blocks_per_cu=0, refill=0, leaf_tris=0, grid_percent=0):
     """lrh_trace_opts; zeros select the defaults."""
     o = LrhTraceOpts()
     o.blockSize, o.ldsStack, o.blocksPerCU, o.refillThreshold = block, lds, blocks_per_cu, refill
-    o.leafTris = leaf_tris
+    o.leafTris, o.gridPercent = leaf_tris, grid_percent
     return o
 
 

@@ -232,7 +232,7 @@ HIPAccelerator::HIPAccelerator(const Context *c, AcceleratorType t) : ctx(c), ty
 	const std::string layout = cfg.Get("hip.bvh.layout", "QUANTIZED");
 	if (layout == "QUANTIZED") buildOpts.layout = 0;
 	else if (layout == "FP32") buildOpts.layout = 1;
-	else if (layout == "WIDE8") buildOpts.layout = 2;  // single-level (BVH) scenes only
+	else if (layout == "WIDE8") buildOpts.layout = 2;  // both levels of an MBVH scene too
 	else throw std::runtime_error("hip.bvh.layout must be QUANTIZED, FP32 or WIDE8: " + layout);
 	const int leaf = cfg.GetInt("hip.bvh.maxleafsize", 4);
 	if (leaf < 1 || leaf > 8) throw std::runtime_error("hip.bvh.maxleafsize must be 1..8");
@@ -250,6 +250,7 @@ HIPAccelerator::HIPAccelerator(const Context *c, AcceleratorType t) : ctx(c), ty
 	traceOpts.ldsStack = (unsigned)cfg.GetInt("hip.kernel.ldsstack", 0);
 	traceOpts.refillThreshold = (unsigned)cfg.GetInt("hip.kernel.refill", 0);
 	traceOpts.leafTris = (unsigned)cfg.GetInt("hip.kernel.leaftris", 0);
+	traceOpts.gridPercent = (unsigned)cfg.GetInt("hip.kernel.gridpercent", 0);
 	const std::string upd = cfg.Get("hip.accelerator.update", "REBUILD");
 	if (upd == "REBUILD") updateMode = UPDATE_REBUILD;
 	else if (upd == "REFIT") updateMode = UPDATE_REFIT;

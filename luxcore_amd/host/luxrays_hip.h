@@ -25,12 +25,12 @@
  *   context.verbose                   true                                                   context.cpp:49
  * and the device's own:
  *   hip.bvh.layout                    QUANTIZED (64-byte nodes, default) | FP32 (128-byte nodes) |
- *                                     WIDE8 (128-byte 8-wide nodes, BVH accelerator only)
+ *                                     WIDE8 (128-byte 8-wide nodes)
  *   hip.bvh.maxleafsize               1..8 (default 4)
  *   hip.bvh.travcost                  SAH traversal cost per triangle test (default 1.0)
  *   hip.bvh.splitbudget               early split clipping, extra references per triangle (default 4; < 0 off)
- *   hip.kernel.blocksize / hip.kernel.ldsstack / hip.kernel.refill / hip.kernel.leaftris
- *                                     launch shape of the trace (defaults: the library's)
+ *   hip.kernel.blocksize / hip.kernel.ldsstack / hip.kernel.refill / hip.kernel.leaftris /
+ *   hip.kernel.gridpercent            launch shape of the trace (defaults: the library's)
  *   hip.accelerator.update            REBUILD (default: MBVHAccel::Update's host rebuild) | REFIT (GPU refit)
  *
  * Meshes are owned by the application, accelerators by the DataSet, devices

@@ -48,9 +48,10 @@ def test_wide8_heightfield_against_oracle(gpu_device, hf):
 
 
 def test_wide8_launch_shapes_and_masks(gpu_device, hf):
-    """Every LDS ring depth and launch shape gives the default's hits bit for
-    bit (8-wide visits push up to seven entries: deep traversals spill);
-    masked rays keep their hit records."""
+    """Every LDS ring depth and launch shape (a share of the persistent grid
+    included) gives the default's hits bit for bit (8-wide visits push up to
+    seven entries: deep traversals spill); masked rays keep their hit
+    records; a grid share above 100% is refused."""
     meshes, prim, ob, ref_p, bounce = hf
     _case("wide8_shapes")
     sc = gpu_device.scene_bvh(meshes, layout=2)
@@ -62,10 +63,13 @@ def test_wide8_launch_shapes_and_masks(gpu_device, hf):
     base = gpu_device.trace_numpy(sc, rays, hits_init=init)
     masked = (rays["flags"] & 1) != 0
     assert np.all(base["t"][masked] == -3.0) and np.all(base["meshIndex"][masked] == 0xABCDEF)
-    for block, lds, refill in ((256, 4, 8), (256, 16, 8), (128, 32, 16), (64, 64, 4), (256, 4, 1)):
-        o = D.trace_opts(block, lds, 0, refill)
+    for block, lds, refill, pct in ((256, 4, 8, 0), (256, 16, 8, 0), (128, 32, 16, 0), (64, 64, 4, 0),
+                                    (256, 4, 1, 0), (256, 8, 12, 80), (64, 8, 8, 1)):
+        o = D.trace_opts(block, lds, 0, refill, grid_percent=pct)
         got = gpu_device.trace_numpy(sc, rays, opts=o, hits_init=init)
-        assert np.array_equal(got, base), (block, lds, refill)
+        assert np.array_equal(got, base), (block, lds, refill, pct)
+    with pytest.raises(D.LrhError, match="gridPercent"):
+        gpu_device.trace_numpy(sc, rays[:64], opts=D.trace_opts(grid_percent=101))
     sc.close()
 
 

@@ -225,3 +225,20 @@ def test_cpp_bvh_refuses_instances(gpu_device, tmp_path):
     scenes.instanced_rays(64, extent=10.0, seed=65).tofile(tmp_path / "rays.bin")
     p = run_check(tmp_path, "--accel", "bvh", expect=1)
     assert "BVHAccel supports only plain triangle meshes" in p.stderr
+
+
+@pytest.mark.gpu
+def test_cpp_wide8_two_level_and_grid_share(gpu_device, tmp_path):
+    """hip.bvh.layout = WIDE8 on an MBVH scene and a launch on half the
+    persistent grid (hip.kernel.gridpercent): the C-ABI 8-wide scene's hits."""
+    print("[case] cpp wide8 two-level + grid share", flush=True)
+    sc = scenes.instanced_scene(n_inst=30, motion_frac=0.3, freq=8, extent=10.0, seed=66)
+    rays = scenes.instanced_rays(60_000, extent=10.0, seed=67)
+    write_meshes(tmp_path, sc.meshes)
+    write_objects(tmp_path / "objects.bin", sc)
+    rays.tofile(tmp_path / "rays.bin")
+    out = run_check(tmp_path, "--prop", "hip.bvh.layout = WIDE8", "--prop", "hip.kernel.gridpercent = 50")
+    assert "accel MBVH" in out.stdout
+    dsc = gpu_device.scene_mbvh(sc, layout=2)
+    assert np.array_equal(read_hits(tmp_path), gpu_device.trace_numpy(dsc, rays))
+    dsc.close()
