@@ -439,7 +439,8 @@ int lrh_server_destroy(lrh_server *srv);
  * (resident batches, median of 3 launches after 2 warm-ups) and keeps the
  * fastest; candidateMs (optional, one per candidate) gets each time, NaN for
  * a shape the device refuses.  With cachePath and key (no spaces), the result
- * is read from / stored in a text cache ("key block lds refill ms" per line),
+ * is read from / stored in a text cache ("key block lds refill ms blocksPerCU
+ * leafTris gridPercent" per line; the last three may be absent),
  * so a serving process tunes once per workload signature; a cached answer
  * times nothing (candidateMs all NaN).  (No reference counterpart: the
  * reference's work-group size is a fixed setting,

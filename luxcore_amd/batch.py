@@ -59,8 +59,9 @@ def autotune(dev, items, shapes=None, cache_path=DEFAULT_CACHE, key=None, log=No
     cand = shapes if shapes is not None else SHAPES
     arr = D._batch_items(items)
     carr = (D.LrhTraceOpts * len(cand))()
-    for i, (b, l, r) in enumerate(cand):
-        carr[i] = D.trace_opts(b, l, 0, r)
+    for i, shape in enumerate(cand):
+        b, l, r = shape[:3]
+        carr[i] = D.trace_opts(b, l, 0, r, grid_percent=shape[3] if len(shape) > 3 else 0)
     best = D.LrhTraceOpts()
     best_ms = ctypes.c_float()
     ms = (ctypes.c_float * len(cand))()
@@ -70,12 +71,13 @@ def autotune(dev, items, shapes=None, cache_path=DEFAULT_CACHE, key=None, log=No
                                        len(cand) if shapes is not None else 0,
                                        cache_path.encode() if cache_path else None, key.encode() if key else None,
                                        ctypes.byref(best), ctypes.byref(best_ms), ms))
-    table = [{"block": b, "lds": l, "refill": r, "ms": float(ms[i])} for i, (b, l, r) in enumerate(cand)
-             if not np.isnan(ms[i])]
+    table = [{"block": sh[0], "lds": sh[1], "refill": sh[2], "grid_percent": sh[3] if len(sh) > 3 else 0,
+              "ms": float(ms[i])} for i, sh in enumerate(cand) if not np.isnan(ms[i])]
     if log:
         for row in table:
             log(f"autotune: {row}")
-    return D.trace_opts(best.blockSize, best.ldsStack, 0, best.refillThreshold), table
+    return D.trace_opts(best.blockSize, best.ldsStack, best.blocksPerCU, best.refillThreshold, best.leafTris,
+                        best.gridPercent), table
 
 
 def tune_build(dev, scene_meshes, items, builds=((0, 4), (1, 4), (2, 4), (2, 2)), shapes=None, log=None):

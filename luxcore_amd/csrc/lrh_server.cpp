@@ -779,7 +779,8 @@ int lrh_autotune_batch(lrh_device *dev, const lrh_batch_item *items, unsigned in
 						cand.push_back(o);
 					}
 		}
-		// cache file: one line per key, "key block lds refill ms"
+		// cache file: one line per key, "key block lds refill ms blocksPerCU
+		// leafTris gridPercent" (the last three may be absent: zero)
 		if (cachePath && !k.empty()) {
 			std::ifstream in(cachePath);
 			std::string line;
@@ -789,10 +790,15 @@ int lrh_autotune_batch(lrh_device *dev, const lrh_batch_item *items, unsigned in
 				unsigned blk = 0, lds = 0, ref = 0;
 				float ms = 0;
 				if (ls >> kk >> blk >> lds >> ref >> ms && kk == k) {
+					unsigned bpc = 0, leaf = 0, pct = 0;
+					if (ls >> bpc && ls >> leaf) ls >> pct;
 					std::memset(best, 0, sizeof(*best));
 					best->blockSize = blk;
 					best->ldsStack = lds;
 					best->refillThreshold = ref;
+					best->blocksPerCU = bpc;
+					best->leafTris = leaf;
+					best->gridPercent = pct;
 					if (bestMs) *bestMs = ms;
 					if (candidateMs)
 						for (size_t i = 0; i < cand.size(); ++i) candidateMs[i] = NAN;
@@ -855,7 +861,8 @@ int lrh_autotune_batch(lrh_device *dev, const lrh_batch_item *items, unsigned in
 				std::ofstream out(tmp);
 				if (!out) throw SrvError(LRH_EINVAL, std::string("cannot write the autotune cache ") + tmp);
 				for (const auto &l : keep) out << l << "\n";
-				out << k << " " << bo.blockSize << " " << bo.ldsStack << " " << bo.refillThreshold << " " << bestT << "\n";
+				out << k << " " << bo.blockSize << " " << bo.ldsStack << " " << bo.refillThreshold << " " << bestT << " "
+					<< bo.blocksPerCU << " " << bo.leafTris << " " << bo.gridPercent << "\n";
 			}
 			if (std::rename(tmp.c_str(), cachePath) != 0)
 				throw SrvError(LRH_EINVAL, std::string("cannot replace the autotune cache ") + cachePath);

@@ -138,19 +138,29 @@ def test_autotuner_picks_and_caches(gpu_device, small_batch, tmp_path):
     _, wl = small_batch
     _case("autotune")
     path = str(tmp_path / "tune.txt")
-    shapes = [(256, 8, 12), (128, 8, 8), (256, 4, 16)]
+    shapes = [(256, 8, 12), (128, 8, 8), (256, 4, 16), (256, 8, 12, 50)]
     opts, table = B.autotune(gpu_device, wl.items(), shapes=shapes, cache_path=path, key="k1")
-    assert len(table) == 3 and os.path.exists(path)
+    assert len(table) == 4 and os.path.exists(path)
     best = min(table, key=lambda r: r["ms"])
-    assert (opts.blockSize, opts.ldsStack, opts.refillThreshold) == (best["block"], best["lds"], best["refill"])
-    # the second call answers from the cache and times nothing
+
+    def fields(o):
+        return o.blockSize, o.ldsStack, o.refillThreshold, o.gridPercent
+    want = (best["block"], best["lds"], best["refill"], best["grid_percent"])
+    assert fields(opts) == want
+    # the second call answers from the cache (every field of the shape) and times nothing
     opts2, table2 = B.autotune(gpu_device, wl.items(), shapes=shapes, cache_path=path, key="k1")
-    assert table2 == []
-    assert (opts2.blockSize, opts2.ldsStack, opts2.refillThreshold) == (best["block"], best["lds"], best["refill"])
+    assert table2 == [] and fields(opts2) == want
     # another key is tuned and stored beside the first
-    B.autotune(gpu_device, wl.items(), shapes=shapes[:1], cache_path=path, key="k2")
+    B.autotune(gpu_device, wl.items(), shapes=shapes[3:], cache_path=path, key="k2")
     with open(path) as f:
         assert sorted(ln.split()[0] for ln in f if ln.strip()) == ["k1", "k2"]
+    opts3, _ = B.autotune(gpu_device, wl.items(), shapes=shapes, cache_path=path, key="k2")
+    assert fields(opts3) == (256, 8, 12, 50)
+    # a line of five fields (no grid share) reads as grid share 0
+    with open(path, "a") as f:
+        f.write("k3 128 4 16 1.5\n")
+    opts4, table4 = B.autotune(gpu_device, wl.items(), shapes=shapes, cache_path=path, key="k3")
+    assert table4 == [] and fields(opts4) == (128, 4, 16, 0)
 
 
 def test_build_autotuner_times_every_layout(gpu_device, small_batch):
