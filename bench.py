@@ -735,6 +735,9 @@ def main():
     ap.add_argument("--lds", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--refill", type=int, default=0)
+    ap.add_argument("--serve-setting", default=None, metavar="RATE,TICK_MI,COHORTS",
+                    help="--mode served: run one setting (sessions per second, Mi rays per tick, cohorts) "
+                         "instead of the sweep, e.g. 1200,2.5,2")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true",

@@ -351,6 +351,22 @@ int lrh_shade(lrh_scene *scene, const lrh_material *materials, unsigned int nMat
 		const void *shadowHits /* hits of the previous step's shadow rays, or NULL */, void *paths,
 		void *nextRays, void *shadowRays, const lrh_shade_params *params,
 		unsigned int *liveCount /* device counter or NULL: += live rays written */, void *stream);
+/* One shading step of up to LRH_SHADE_BATCH_MAX ray buffers in one launch:
+ * item k gets exactly what lrh_shade gives on its arguments (same materials,
+ * lights and live counter for every item; scenes on one device).  No
+ * reference counterpart: it is the hit-consumption step of several
+ * wavefronts at once, as lrh_trace_batch is their trace. */
+#define LRH_SHADE_BATCH_MAX 16
+typedef struct {
+	lrh_scene *scene;
+	const void *rays, *hits;
+	const void *shadowHits;           /* or NULL */
+	void *paths, *nextRays, *shadowRays;
+	lrh_shade_params params;
+} lrh_shade_item;
+int lrh_shade_batch(const lrh_shade_item *items, unsigned int nItems, const lrh_material *materials,
+		unsigned int nMaterials, const lrh_light_tri *lights, unsigned int nLights, unsigned int *liveCount,
+		void *stream);
 /* ---- served path-tracing sessions (continuous batching) ----
  * No reference counterpart as a component: it batches what SLG's path tracer
  * does one device at a time, one ray buffer per wavefront traced and its hits

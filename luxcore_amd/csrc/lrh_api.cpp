@@ -1767,6 +1767,51 @@ static_assert(sizeof(lrh_material) == sizeof(LrhMaterial), "lrh_material layout"
 static_assert(sizeof(lrh_light_tri) == sizeof(LrhLightTri), "lrh_light_tri layout");
 static_assert(sizeof(lrh_path) == sizeof(LrhPath) && sizeof(lrh_path) == 48, "lrh_path layout");
 
+}  // extern "C"
+
+namespace {
+// checks of one lrh_shade call and its kernel parameters
+int shadeParams(lrh_scene *s, const lrh_material *materials, unsigned int nMaterials, const lrh_light_tri *lights,
+		unsigned int nLights, const void *rays, const void *hits, const void *shadowHits, void *paths,
+		void *nextRays, void *shadowRays, const lrh_shade_params &prm, unsigned int *liveCount, LrhShadeParams &P) {
+	if (s->twoLevel) return fail(LRH_EINVAL, "shading needs a single-level scene");
+	const unsigned count = prm.count;
+	if (count && (!rays || !hits || !paths || !nextRays || !shadowRays))
+		return fail(LRH_EINVAL, "null ray, hit or path buffer");
+	if ((nMaterials && !materials) || (nLights && !lights)) return fail(LRH_EINVAL, "null material or light array");
+	if (((uintptr_t)rays & 15u) || ((uintptr_t)nextRays & 15u) || ((uintptr_t)shadowRays & 15u) ||
+			((uintptr_t)hits & 3u) || ((uintptr_t)shadowHits & 3u) || ((uintptr_t)paths & 3u) ||
+			((uintptr_t)materials & 3u) || ((uintptr_t)lights & 3u))
+		return fail(LRH_EINVAL, "misaligned buffer");
+	if (prm.maxDepth == 0) return fail(LRH_EINVAL, "maxDepth must be at least 1");
+	if (!(prm.rrSurvive > 0.f && prm.rrSurvive <= 1.f)) return fail(LRH_EINVAL, "rrSurvive must be in (0, 1]");
+	P.rays = (const lrh_ray_rec *)rays;
+	P.hits = (const lrh_hit_rec *)hits;
+	P.shadowHits = (const lrh_hit_rec *)shadowHits;
+	P.paths = (LrhPath *)paths;
+	P.next = (lrh_ray_rec *)nextRays;
+	P.shadow = (lrh_ray_rec *)shadowRays;
+	P.materials = (const LrhMaterial *)materials;
+	P.nMaterials = nMaterials;
+	P.lights = (const LrhLightTri *)lights;
+	P.nLights = nLights;
+	P.tris = (const lrh_tri *)s->tris.p;
+	P.invPerm = (const uint32_t *)s->invPerm.p;
+	P.meshTriBase = (const uint32_t *)s->meshTriBase.p;
+	P.nMeshes = s->nMeshes;
+	P.count = count;
+	P.maxDepth = prm.maxDepth;
+	P.rrDepth = prm.rrDepth;
+	P.rrSurvive = prm.rrSurvive;
+	P.seed = prm.seed;
+	P.pixelBase = prm.pixelBase;
+	P.liveCount = liveCount;
+	return LRH_OK;
+}
+}  // namespace
+
+extern "C" {
+
 int lrh_paths_init(lrh_device *dev, void *paths, unsigned int count, void *stream) {
 	if (!dev || (count && !paths)) return fail(LRH_EINVAL, "null argument");
 	if ((uintptr_t)paths & 3u) return fail(LRH_EINVAL, "misaligned path buffer");
@@ -1781,43 +1826,44 @@ int lrh_shade(lrh_scene *s, const lrh_material *materials, unsigned int nMateria
 		unsigned int nLights, const void *rays, const void *hits, const void *shadowHits, void *paths,
 		void *nextRays, void *shadowRays, const lrh_shade_params *prm, unsigned int *liveCount, void *stream) {
 	if (!s || !s->dev || !prm) return fail(LRH_EINVAL, "null argument");
-	if (s->twoLevel) return fail(LRH_EINVAL, "shading needs a single-level scene");
-	const unsigned count = prm->count;
-	if (count && (!rays || !hits || !paths || !nextRays || !shadowRays))
-		return fail(LRH_EINVAL, "null ray, hit or path buffer");
-	if ((nMaterials && !materials) || (nLights && !lights)) return fail(LRH_EINVAL, "null material or light array");
-	if (((uintptr_t)rays & 15u) || ((uintptr_t)nextRays & 15u) || ((uintptr_t)shadowRays & 15u) ||
-			((uintptr_t)hits & 3u) || ((uintptr_t)shadowHits & 3u) || ((uintptr_t)paths & 3u) ||
-			((uintptr_t)materials & 3u) || ((uintptr_t)lights & 3u))
-		return fail(LRH_EINVAL, "misaligned buffer");
-	if (prm->maxDepth == 0) return fail(LRH_EINVAL, "maxDepth must be at least 1");
-	if (!(prm->rrSurvive > 0.f && prm->rrSurvive <= 1.f)) return fail(LRH_EINVAL, "rrSurvive must be in (0, 1]");
-	if (count == 0) return LRH_OK;
+	LrhShadeParams P;
+	const int rc = shadeParams(s, materials, nMaterials, lights, nLights, rays, hits, shadowHits, paths, nextRays,
+			shadowRays, *prm, liveCount, P);
+	if (rc != LRH_OK || prm->count == 0) return rc;
 	return guarded([&]() {
 		s->dev->activate();
-		LrhShadeParams P;
-		P.rays = (const lrh_ray_rec *)rays;
-		P.hits = (const lrh_hit_rec *)hits;
-		P.shadowHits = (const lrh_hit_rec *)shadowHits;
-		P.paths = (LrhPath *)paths;
-		P.next = (lrh_ray_rec *)nextRays;
-		P.shadow = (lrh_ray_rec *)shadowRays;
-		P.materials = (const LrhMaterial *)materials;
-		P.nMaterials = nMaterials;
-		P.lights = (const LrhLightTri *)lights;
-		P.nLights = nLights;
-		P.tris = (const lrh_tri *)s->tris.p;
-		P.invPerm = (const uint32_t *)s->invPerm.p;
-		P.meshTriBase = (const uint32_t *)s->meshTriBase.p;
-		P.nMeshes = s->nMeshes;
-		P.count = count;
-		P.maxDepth = prm->maxDepth;
-		P.rrDepth = prm->rrDepth;
-		P.rrSurvive = prm->rrSurvive;
-		P.seed = prm->seed;
-		P.pixelBase = prm->pixelBase;
-		P.liveCount = liveCount;
 		HIPCHK(lrhLaunchShade(P, stream ? (hipStream_t)stream : s->dev->stream));
+	});
+}
+
+int lrh_shade_batch(const lrh_shade_item *items, unsigned int nItems, const lrh_material *materials,
+		unsigned int nMaterials, const lrh_light_tri *lights, unsigned int nLights, unsigned int *liveCount,
+		void *stream) {
+	if (!items || nItems == 0) return fail(LRH_EINVAL, "null or empty item list");
+	if (nItems > LRH_SHADE_BATCH_MAX) return fail(LRH_ERANGE, "more than LRH_SHADE_BATCH_MAX items");
+	LrhShadeBatchArgs A{};
+	lrh_device *dev = nullptr;
+	uint64_t blocks = 0;
+	for (unsigned k = 0; k < nItems; ++k) {
+		const lrh_shade_item &it = items[k];
+		if (!it.scene || !it.scene->dev) return fail(LRH_EINVAL, "null scene in item " + std::to_string(k));
+		if (dev && it.scene->dev != dev) return fail(LRH_EINVAL, "items on different devices");
+		dev = it.scene->dev;
+		LrhShadeParams P;
+		const int rc = shadeParams(it.scene, materials, nMaterials, lights, nLights, it.rays, it.hits,
+				it.shadowHits, it.paths, it.nextRays, it.shadowRays, it.params, liveCount, P);
+		if (rc != LRH_OK) return rc;
+		if (P.count == 0) continue;  // nothing to shade: no blocks
+		A.blockStart[A.n] = (uint32_t)blocks;
+		A.p[A.n++] = P;
+		blocks += (P.count + 255u) / 256u;
+	}
+	if (A.n == 0) return LRH_OK;
+	if (blocks > 0xFFFFFFFFull) return fail(LRH_ERANGE, "batch too large");
+	A.blockStart[A.n] = (uint32_t)blocks;
+	return guarded([&]() {
+		dev->activate();
+		HIPCHK(lrhLaunchShadeBatch(A, stream ? (hipStream_t)stream : dev->stream));
 	});
 }
 

@@ -64,6 +64,14 @@ struct LrhShadeParams {
 hipError_t lrhLaunchCamera(const LrhCameraParams &C, lrh_ray_rec *rays, hipStream_t s);
 hipError_t lrhLaunchPathsInit(LrhPath *paths, uint32_t count, hipStream_t s);
 hipError_t lrhLaunchShade(const LrhShadeParams &P, hipStream_t s);
+// several shading steps in one launch: item k owns blocks [blockStart[k], blockStart[k+1])
+constexpr int kShadeBatchMax = 16;
+struct LrhShadeBatchArgs {
+	uint32_t n;
+	uint32_t blockStart[kShadeBatchMax + 1];
+	LrhShadeParams p[kShadeBatchMax];
+};
+hipError_t lrhLaunchShadeBatch(const LrhShadeBatchArgs &A, hipStream_t s);
 hipError_t lrhLaunchBounce(const LrhBounceParams &P, hipStream_t s);
 
 #endif

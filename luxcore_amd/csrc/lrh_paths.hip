@@ -280,6 +280,18 @@ __global__ void lrh_shade_kernel(LrhShadeParams P) {
 	countLive(P.liveCount, live);
 }
 
+__global__ void lrh_shade_batch_kernel(LrhShadeBatchArgs A) {
+	// the item of this block (uniform over the block: scalar loads of the
+	// kernel arguments)
+	uint32_t k = 0;
+	while (k + 1 < A.n && blockIdx.x >= A.blockStart[k + 1]) ++k;
+	const LrhShadeParams &P = A.p[k];
+	const uint32_t i = (blockIdx.x - A.blockStart[k]) * blockDim.x + threadIdx.x;
+	uint32_t live = 0;
+	if (i < P.count) live = shadeOne(P, i);
+	countLive(P.liveCount, live);
+}
+
 __global__ void lrh_paths_init_kernel(LrhPath *paths, uint32_t count) {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= count) return;
@@ -310,6 +322,12 @@ hipError_t lrhLaunchPathsInit(LrhPath *paths, uint32_t count, hipStream_t s) {
 
 hipError_t lrhLaunchShade(const LrhShadeParams &P, hipStream_t s) {
 	hipLaunchKernelGGL(lrh_shade_kernel, dim3((P.count + 255) / 256), dim3(256), 0, s, P);
+	return hipGetLastError();
+}
+
+hipError_t lrhLaunchShadeBatch(const LrhShadeBatchArgs &A, hipStream_t s) {
+	if (A.n == 0 || A.n > (uint32_t)kShadeBatchMax) return hipErrorInvalidValue;
+	hipLaunchKernelGGL(lrh_shade_batch_kernel, dim3(A.blockStart[A.n]), dim3(256), 0, s, A);
 	return hipGetLastError();
 }
 

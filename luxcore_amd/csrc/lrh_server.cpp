@@ -271,8 +271,9 @@ struct lrh_server {
 		if (s->step >= 1) items.push_back({sc, s->b.shadow, s->b.shadowHits, n, LRH_QUERY_ANY});
 	}
 
-	// after the tick's trace: the session's next wavefront, generated on the GPU
-	bool advance(Session *s) {
+	// after the tick's trace: the session's next wavefront, generated on the
+	// GPU (shade mode: an item of the tick's one lrh_shade_batch launch)
+	bool advance(Session *s, std::vector<lrh_shade_item> &shade) {
 		lrh_scene *sc = scenes[s->spec.scene];
 		const unsigned last = cfg.maxBounces;
 		if (cfg.mode == LRH_SERVE_SHADE) {
@@ -280,16 +281,21 @@ struct lrh_server {
 			// vertex); one more step after the last vertex only gathers the
 			// light its shadow rays found
 			const unsigned nxt = 1 - s->cur;
-			lrh_shade_params p{};
-			p.count = n;
-			p.maxDepth = last + 1;
-			p.rrDepth = cfg.rrStart + 1;
-			p.rrSurvive = cfg.rrSurvive;
-			p.seed = s->spec.seed;
-			p.pixelBase = 0;
-			chk(lrh_shade(sc, (const lrh_material *)materials, nMaterials, (const lrh_light_tri *)lights, nLights,
-					s->b.rays[s->cur], s->b.hits, s->step >= 1 ? s->b.shadowHits : nullptr, s->b.paths,
-					s->b.rays[nxt], s->b.shadow, &p, (unsigned *)counter, s->stream), "lrh_shade");
+			lrh_shade_item it{};
+			it.scene = sc;
+			it.rays = s->b.rays[s->cur];
+			it.hits = s->b.hits;
+			it.shadowHits = s->step >= 1 ? s->b.shadowHits : nullptr;
+			it.paths = s->b.paths;
+			it.nextRays = s->b.rays[nxt];
+			it.shadowRays = s->b.shadow;
+			it.params.count = n;
+			it.params.maxDepth = last + 1;
+			it.params.rrDepth = cfg.rrStart + 1;
+			it.params.rrSurvive = cfg.rrSurvive;
+			it.params.seed = s->spec.seed;
+			it.params.pixelBase = 0;
+			shade.push_back(it);
 			s->cur = nxt;
 		} else if (s->step <= last) {
 			const unsigned nxt = 1 - s->cur;
@@ -421,7 +427,12 @@ struct lrh_server {
 			}
 		}
 		std::vector<Session *> still, finished;
-		for (Session *s : coh) (advance(s) ? still : finished).push_back(s);
+		std::vector<lrh_shade_item> shade;
+		for (Session *s : coh) (advance(s, shade) ? still : finished).push_back(s);
+		for (size_t k = 0; k < shade.size(); k += LRH_SHADE_BATCH_MAX)
+			chk(lrh_shade_batch(shade.data() + k, (unsigned)std::min<size_t>(LRH_SHADE_BATCH_MAX, shade.size() - k),
+					(const lrh_material *)materials, nMaterials, (const lrh_light_tri *)lights, nLights,
+					(unsigned *)counter, st), "lrh_shade_batch");
 		chk(lrh_event_record(f.e1, st), "lrh_event_record");
 		f.traced = coh;
 		f.finished = finished;

@@ -75,6 +75,15 @@ class LrhShadeParams(ctypes.Structure):
                 ("pad", ctypes.c_uint)]
 
 
+class LrhShadeItem(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_void_p), ("rays", ctypes.c_void_p), ("hits", ctypes.c_void_p),
+                ("shadowHits", ctypes.c_void_p), ("paths", ctypes.c_void_p), ("nextRays", ctypes.c_void_p),
+                ("shadowRays", ctypes.c_void_p), ("params", LrhShadeParams)]
+
+
+SHADE_BATCH_MAX = 16
+
+
 # lrh_material, lrh_light_tri, lrh_path records
 MATERIAL_DTYPE = np.dtype([("kd", "<f4", 3), ("le", "<f4", 3)])
 LIGHT_DTYPE = np.dtype([("p0", "<f4", 3), ("e1", "<f4", 3), ("e2", "<f4", 3), ("le", "<f4", 3)])
@@ -176,6 +185,7 @@ SIGNATURES = {
     "lrh_paths_init": ([_vp, _vp, ctypes.c_uint, _vp], ctypes.c_int),
     "lrh_shade": ([_vp, _vp, ctypes.c_uint, _vp, ctypes.c_uint, _vp, _vp, _vp, _vp, _vp, _vp,
                    ctypes.POINTER(LrhShadeParams), _vp, _vp], ctypes.c_int),
+    "lrh_shade_batch": ([_vp, ctypes.c_uint, _vp, ctypes.c_uint, _vp, ctypes.c_uint, _vp, _vp], ctypes.c_int),
     "lrh_event_query": ([_vp], ctypes.c_int),
     "lrh_server_create": ([_vp, _pp, ctypes.c_uint, ctypes.POINTER(LrhServerConfig), _pp], ctypes.c_int),
     "lrh_server_submit": ([_vp, ctypes.POINTER(LrhSessionSpec), ctypes.c_uint], ctypes.c_int),
@@ -630,6 +640,18 @@ class Device:
                               lights_buf.ptr if lights_buf else None, n_lights, rays_buf.ptr, hits_buf.ptr,
                               shadow_hits_buf.ptr if shadow_hits_buf else None, paths_buf.ptr, next_buf.ptr,
                               shadow_buf.ptr, ctypes.byref(p), live_counter, stream))
+
+    def shade_batch(self, items, materials_buf, n_materials, lights_buf, n_lights, live_counter=None, stream=None):
+        """lrh_shade_batch: several shading steps in one launch; items are
+        (scene, rays, hits, shadow_hits or None, paths, next, shadow, count,
+        max_depth, rr_depth, rr_survive, seed, pixel_base)."""
+        arr = (LrhShadeItem * max(1, len(items)))()
+        for k, (sc, rb, hb, shb, pb, nb, sb, cnt, md, rd, rs, seed, pbase) in enumerate(items):
+            arr[k] = LrhShadeItem(sc.h, rb.ptr, hb.ptr, shb.ptr if shb else None, pb.ptr, nb.ptr, sb.ptr,
+                                  LrhShadeParams(cnt, md, rd, rs, seed, pbase, 0))
+        check(lib().lrh_shade_batch(arr if items else None, len(items), materials_buf.ptr if materials_buf else None,
+                                    n_materials, lights_buf.ptr if lights_buf else None, n_lights, live_counter,
+                                    stream))
 
     def stats(self):
         r = ctypes.c_ulonglong()

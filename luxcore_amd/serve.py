@@ -207,17 +207,20 @@ def bench(args, dev, log):
     w.close()
     plain = plain_comparison(dev, scenes, opts, log=log)
     rows = []
-    for cohorts in (2, 3):
-        for tick_rays in (3 << 20, 5 << 19, 2 << 20, 3 << 19):
-            for rate in (150.0, 600.0, 1200.0):
-                specs = poisson_specs(rate, 0.5, len(scenes), seed=int(rate))
-                srv = Server(dev, scenes, opts=opts, max_rays_per_tick=tick_rays, cohorts=cohorts)
-                wall = srv.run(specs, time_limit=120.0)
-                row = served_row(srv, wall, rate, tick_rays)
-                row["cohorts"] = cohorts
-                log(f"served: {row}")
-                rows.append(row)
-                srv.close()
+    settings = [(rate, tick_rays, cohorts) for cohorts in (2, 3) for tick_rays in (3 << 20, 5 << 19, 2 << 20, 3 << 19)
+                for rate in (150.0, 600.0, 1200.0)]
+    if getattr(args, "serve_setting", None):
+        r, t, c = args.serve_setting.split(",")
+        settings = [(float(r), int(float(t) * (1 << 20)), int(c))]
+    for rate, tick_rays, cohorts in settings:
+        specs = poisson_specs(rate, 0.5, len(scenes), seed=int(rate))
+        srv = Server(dev, scenes, opts=opts, max_rays_per_tick=tick_rays, cohorts=cohorts)
+        wall = srv.run(specs, time_limit=120.0)
+        row = served_row(srv, wall, rate, tick_rays)
+        row["cohorts"] = cohorts
+        log(f"served: {row}")
+        rows.append(row)
+        srv.close()
     ok = [r for r in rows if r["p99_ms"] < 5.0]
     best = max(ok or rows, key=lambda r: r["trace_mrays"])
     return {"metric": "Mrays/s served: continuous batching of 512x512 path-tracing sessions; p50/p99 wavefront "

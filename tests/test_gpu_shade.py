@@ -105,3 +105,59 @@ def test_shade_rejects_bad_arguments(gpu_device, tmp_path):
         gpu_device.shade(scene, None, 0, None, 0, b, b, None, b, b, b, 16, rr_survive=0.0)
     b.free()
     scene.close()
+
+
+def test_shade_batch_equals_per_buffer_shading(gpu_device, tmp_path):
+    """lrh_shade_batch: every item gets exactly lrh_shade's paths, next rays
+    and shadow rays, and the live counter the same total; items of different
+    sizes (one not a multiple of the block, one empty), with and without the
+    previous step's shadow hits."""
+    print("[case] shade batch", flush=True)
+    sf = _scene(tmp_path)
+    dev = gpu_device
+    scene = P.scene_on_device(dev, sf)
+    mats, lights = sf.materials(), sf.light_triangles()
+    mb, lb = dev.upload(mats), dev.upload(lights)
+    orig, target, up, fov = sf.camera
+    sets = []
+    for k, (w, h) in enumerate(((96, 64), (50, 37), (128, 96), (0, 0))):
+        n = w * h
+        cap = max(n, 1)
+        s = {"n": n, "rb": dev.alloc(cap * 48), "hb": dev.alloc(cap * 20), "shb": dev.alloc(cap * 20),
+             "pb": dev.alloc(cap * D.PATH_DTYPE.itemsize), "seed": 100 + k}
+        if n:
+            dev.camera_rays(s["rb"], orig, target, up, fov, w, h)
+            dev.paths_init(s["pb"], n)
+            dev.trace(scene, s["rb"], s["hb"], n, 0)
+            # shadow hits of a previous step: the same rays traced as any hit
+            dev.trace(scene, s["rb"], s["shb"], n, 1)
+        s["shadow"] = k % 2 == 1
+        sets.append(s)
+    dev.sync()
+    outs = {}
+    for mode in ("one", "batch"):
+        counter = dev.upload(np.zeros(2, np.uint32))
+        items = []
+        for k, s in enumerate(sets):
+            cap = max(s["n"], 1)
+            pb = dev.upload(dev.download(s["pb"], D.PATH_DTYPE, cap))
+            nb, sb = dev.alloc(cap * 48), dev.alloc(cap * 48)
+            items.append((scene, s["rb"], s["hb"], s["shb"] if s["shadow"] else None, pb, nb, sb, s["n"], 6, 3, 0.7,
+                          s["seed"], 7 * k))
+        if mode == "one":
+            for it in items:
+                sc, rb, hb, shb, pb, nb, sb, n, md, rd, rs, seed, pbase = it
+                dev.shade(sc, mb, len(mats), lb, len(lights), rb, hb, shb, pb, nb, sb, n, max_depth=md,
+                          rr_depth=rd, rr_survive=rs, seed=seed, pixel_base=pbase, live_counter=counter.ptr)
+        else:
+            dev.shade_batch(items, mb, len(mats), lb, len(lights), live_counter=counter.ptr)
+        dev.sync()
+        outs[mode] = ([(dev.download(it[4], D.PATH_DTYPE, it[7]).tobytes(), dev.download(it[5], RAY_DTYPE, it[7]).tobytes(),
+                        dev.download(it[6], RAY_DTYPE, it[7]).tobytes()) if it[7] else () for it in items],
+                      int(dev.download(counter, np.uint32, 2)[0]))
+    assert outs["batch"][1] == outs["one"][1] > 0
+    for k, (a, b) in enumerate(zip(outs["one"][0], outs["batch"][0])):
+        assert a == b, k
+    with pytest.raises(D.LrhError, match="LRH_SHADE_BATCH_MAX"):
+        dev.shade_batch([items[0]] * (D.SHADE_BATCH_MAX + 1), mb, len(mats), lb, len(lights))
+    scene.close()
