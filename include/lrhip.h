@@ -320,7 +320,11 @@ int lrh_rays_bounce(lrh_scene *scene, const void *rays, const void *hits, unsign
  * Pass the hits of the previous step's shadow rays whenever that step wrote
  * shadow rays: light still pending without them is dropped.  One step after
  * the last vertex (all paths ended, next rays masked) gathers the light of
- * the last shadow rays. */
+ * the last shadow rays.  A path that had already ended when the previous
+ * step ran (lrh_path.pad = 1) and has no light pending is skipped: its slots
+ * still hold the masked rays the two steps before wrote, so nextRays must be
+ * the buffer of the step before the previous one (the ray buffers alternate,
+ * as in the wavefront loop) and shadowRays the same buffer every step. */
 typedef struct {
 	float kd[3];      /* diffuse reflectance */
 	float le[3];      /* emitted radiance (0: not a light) */
@@ -333,7 +337,8 @@ typedef struct {
 	float thr[3];     /* path throughput */
 	float rad[3];     /* radiance gathered so far (the pixel sample) */
 	float dl[3];      /* direct light waiting for its shadow ray */
-	unsigned int depth, alive, pad;
+	unsigned int depth, alive;
+	unsigned int pad;  /* 1 once a shading step has seen the path ended (lrh_paths_init: 0) */
 } lrh_path;
 typedef struct {
 	unsigned int count;          /* paths in the wavefront */

@@ -155,16 +155,27 @@ __device__ __forceinline__ void cosineDir(const float n[3], float u1, float u2, 
 
 __device__ uint32_t shadeOne(const LrhShadeParams &P, uint32_t i) {
 	LrhPath path = P.paths[i];
-	const lrh_ray_rec ray = P.rays[i];
-	// the previous step's shadow ray: its light counts if nothing blocked it
 	const bool pending = path.dl[0] != 0.f || path.dl[1] != 0.f || path.dl[2] != 0.f;
+	// a path already ended when the previous step ran (pad = 1) and with no
+	// light pending: that step and the one that ended it masked its slots of
+	// both ray buffers and of the shadow buffer, so there is nothing to do
+	if (!path.alive && path.pad && !pending) return 0;
+	// the previous step's shadow ray: its light counts if nothing blocked it
 	if (P.shadowHits && pending && P.shadowHits[i].meshIndex == 0xffffffffu)
 		for (int k = 0; k < 3; ++k) path.rad[k] = path.rad[k] + path.dl[k];
 	for (int k = 0; k < 3; ++k) path.dl[k] = 0.f;
 	lrh_ray_rec *next = P.next + i;
 	lrh_ray_rec *shadow = P.shadow + i;
+	if (!path.alive) {
+		path.pad = 1u;
+		writeMasked(next);
+		writeMasked(shadow);
+		P.paths[i] = path;
+		return 0;
+	}
+	const lrh_ray_rec ray = P.rays[i];
 	const lrh_hit_rec hit = P.hits[i];
-	if (!path.alive || (ray.flags & 1u) || hit.meshIndex >= P.nMeshes ||
+	if ((ray.flags & 1u) || hit.meshIndex >= P.nMeshes ||
 			hit.triangleIndex >= P.meshTriBase[hit.meshIndex + 1] - P.meshTriBase[hit.meshIndex]) {
 		path.alive = 0u;
 		writeMasked(next);

@@ -90,6 +90,9 @@ def shade_step(meshes, materials, lights, rays, hits, shadow_hits, paths, max_de
     n = rays.shape[0]
     paths = paths.copy()
     pend = np.any(paths["dl"] != 0, axis=1)
+    # pad: 1 once a step has seen the path ended (the device then skips it
+    # in later steps, its masked ray slots being written already)
+    paths["pad"][paths["alive"] == 0] = 1
     if shadow_hits is not None:
         add = pend & (shadow_hits["meshIndex"] == NULL)
         paths["rad"][add] = (paths["rad"][add] + paths["dl"][add]).astype(F)
