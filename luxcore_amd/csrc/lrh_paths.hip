@@ -7,8 +7,14 @@
  * (include/slg/engines/pathoclbase/kernels/pathoclbase_kernels_micro.cl:
  * 723-844, 1045-1075), reduced to what a synthetic session needs.
  *
+ * The shading step (lrh_shade, and lrh_shade_batch for several wavefronts in
+ * one launch) consumes hits as SLG's hit-consumption and direct-light
+ * micro-kernels do, reduced to Lambertian surfaces and triangle lights.
+ *
  * Terminated paths become masked rays (RAY_FLAGS_MASKED), so a wavefront
- * keeps its size and the tracer skips them.  Random numbers are a hash of
+ * keeps its size and the tracer skips them; a path that ended before the
+ * previous step is not touched again (its slots are masked already).
+ * Random numbers are a hash of
  * (session seed, pixel, bounce, dimension): a session's rays do not depend on
  * what else is in flight.
  */

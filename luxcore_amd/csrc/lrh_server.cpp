@@ -18,8 +18,11 @@
  * while the HBM budget and the per-tick ray budget allow, gathers the ready
  * wavefront of every live session of the cohort (closest-hit rays + the
  * shadow rays of the previous vertex) into ONE lrh_trace_batch launch, then
- * generates each session's next wavefront on the GPU and retires finished
- * sessions.  Sessions that arrive while others are on the GPU join at their
+ * generates every session's next wavefront on the GPU (shade mode: one
+ * lrh_shade_batch launch for the whole tick) and retires finished sessions.
+ * Each cohort's launch takes 80% of the persistent grid unless the caller's
+ * launch shape says otherwise, so the other cohort's tick runs beside it and
+ * fills the CUs its waves leave while it drains.  Sessions that arrive while others are on the GPU join at their
  * cohort's next tick.  The host needs no result of a tick to build the next
  * one, so while one cohort's tick drains the other cohort's is already
  * queued; a tick completes when its end event does.
