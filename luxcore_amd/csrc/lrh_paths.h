@@ -71,6 +71,8 @@ struct LrhShadeBatchArgs {
 	uint32_t blockStart[kShadeBatchMax + 1];
 	LrhShadeParams p[kShadeBatchMax];
 };
+// passed by value as the kernel's argument: within the 4 KiB argument limit
+static_assert(sizeof(LrhShadeBatchArgs) <= 4096, "shade batch arguments exceed 4 KiB");
 hipError_t lrhLaunchShadeBatch(const LrhShadeBatchArgs &A, hipStream_t s);
 hipError_t lrhLaunchBounce(const LrhBounceParams &P, hipStream_t s);
 
